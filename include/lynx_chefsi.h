@@ -1,0 +1,198 @@
+/*
+ * lynx_chefsi.h -- C ABI of the MI355X-native Chebyshev-filtered subspace
+ * iteration (CheFSI) library, liblynx_chefsi.so.
+ *
+ * The library replaces the hot path of LYNX's real-space DFT eigensolver:
+ * the Hamiltonian apply (FD stencil + local potential + nonlocal projectors)
+ * and the CheFSI solve built on it.  Every entry point takes plain pointers
+ * and sizes.  Arrays are column-major: a block of wavefunctions is (Nd x ncol)
+ * with leading dimension ld >= Nd, and the grid index is x fastest
+ * (loc = i + j*nx + k*nx*ny).  Complex data is interleaved (re, im) doubles
+ * or floats.
+ *
+ * Errors: every call returns a lynx_status; the message of the last failure
+ * on a context is available from lynx_last_error().  No call exits the
+ * process.  Calls on one context are serialised on its HIP stream; contexts
+ * are independent.
+ */
+#ifndef LYNX_CHEFSI_H
+#define LYNX_CHEFSI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lynx_ctx lynx_ctx;
+typedef int lynx_status;
+
+enum lynx_status_code {
+    LYNX_OK = 0,
+    LYNX_E_ARG = 1,          /* invalid argument or size                     */
+    LYNX_E_HIP = 2,          /* HIP runtime / rocBLAS / rocSOLVER failure    */
+    LYNX_E_CHOLESKY = 3,     /* overlap matrix not positive definite         */
+    LYNX_E_EIG = 4,          /* dense eigensolver did not converge           */
+    LYNX_E_NONFINITE = 5,    /* NaN or Inf produced                          */
+    LYNX_E_IO = 6,           /* pseudopotential file unreadable or malformed */
+    LYNX_E_STATE = 7,        /* call made before the required set-up         */
+    LYNX_E_UNSUPPORTED = 8   /* valid input outside what this build handles */
+};
+
+/* Storage / arithmetic type of a wavefunction block. */
+enum lynx_dtype {
+    LYNX_F64 = 0,            /* real double (Gamma point)                    */
+    LYNX_C128 = 1,           /* complex double (k-point, Bloch)              */
+    LYNX_F32 = 2,            /* real float                                   */
+    LYNX_C64 = 3             /* complex float                                */
+};
+
+enum lynx_mem { LYNX_MEM_HOST = 0, LYNX_MEM_DEVICE = 1 };
+enum lynx_cell { LYNX_CELL_ORTHOGONAL = 0, LYNX_CELL_NONORTHOGONAL = 1 };
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the process-wide GPUContext singleton (GPUContext.cuh:699-709):
+ * one context per caller, owning a HIP stream, device workspaces and the
+ * operator data.  device_id = -1 creates a host-only context: grid, stencil
+ * and projector set-up and their getters work, GPU work returns
+ * LYNX_E_STATE. */
+lynx_status lynx_ctx_create(lynx_ctx **out, int device_id);
+void lynx_ctx_destroy(lynx_ctx *ctx);
+const char *lynx_last_error(const lynx_ctx *ctx);
+const char *lynx_status_string(lynx_status s);
+/* The hipStream_t on which the context enqueues all work. */
+lynx_status lynx_ctx_stream(const lynx_ctx *ctx, void **stream);
+/* Blocks until all work queued on the context's stream has finished. */
+lynx_status lynx_ctx_synchronize(lynx_ctx *ctx);
+
+/* ------------------------------------------------------------ grid (a1) */
+
+/* Grid, lattice and FD stencil.  Replaces Lattice(latvec, cell_type)
+ * (Lattice.cpp:8-54), FDGrid (FDGrid.cpp:6-35) and FDStencil(order, grid,
+ * lattice) (FDStencil.cpp:16-130).  latvec holds the three lattice vectors as
+ * rows (Bohr); boundaries are periodic in all directions. */
+lynx_status lynx_set_grid(lynx_ctx *ctx, int nx, int ny, int nz, const double latvec[9],
+                          int cell_type, int fd_order);
+/* Mesh spacing, volume element and FDStencil::max_eigval_half_lap(). */
+lynx_status lynx_get_grid_info(const lynx_ctx *ctx, double h[3], double *dV,
+                               double *max_eig_half_lap);
+/* Scaled coefficients, each 3 x (fd_order/2 + 1), row-major by axis:
+ * D1 = D1_coeff_{x,y,z}, D2 = D2_coeff_{x,y,z}, D2m = D2_coeff_{xy,xz,yz}. */
+lynx_status lynx_get_stencil(const lynx_ctx *ctx, double *D1, double *D2, double *D2m);
+
+/* ------------------------------------------------------ nonlocal (a16) */
+
+/* Reads psp8 files and builds the Kleinman-Bylander projectors of all atom
+ * images whose rc sphere touches the grid.  Replaces
+ * Pseudopotential::load_psp8 (Pseudopotential.cpp:28-333),
+ * Crystal::compute_nloc_influence (Crystal.cpp:136-255) and
+ * NonlocalProjector::setup (NonlocalProjector.cpp:13-119).
+ * Atoms must be listed type-major (all atoms of type 0 first, ...).
+ * coords: 3 per atom; fractional if coords_fractional, else Cartesian. */
+lynx_status lynx_set_atoms(lynx_ctx *ctx, int ntypes, const char *const *psp8_paths,
+                           int natom, const int *atom_type, const double *coords,
+                           int coords_fractional);
+
+/* Direct form: projectors already built (e.g. by the caller's own
+ * NonlocalProjector).  One entry per influence image, in accumulation order:
+ *   img_atom[i]   physical atom the image belongs to (0..n_atom-1)
+ *   img_ndc[i]    grid points in the image's rc sphere
+ *   img_nproj[i]  projectors of that atom
+ *   gpos          concatenated flat grid indices, sum(img_ndc) entries
+ *   chi           concatenated (ndc x nproj) column-major blocks
+ *   img_shift     3 per image, periodic shift in lattice-aligned coordinates
+ *   ip_displ      n_atom+1 projector offsets of the physical atoms
+ *   gamma         ip_displ[n_atom] KB energies, one per projector
+ * Mirrors the GPU flattening of NonlocalProjector (Hamiltonian.cu:342-495). */
+lynx_status lynx_set_nonlocal(lynx_ctx *ctx, int n_img, const int *img_atom,
+                              const int *img_ndc, const int *img_nproj, const int *gpos,
+                              const double *chi, const double *img_shift, int n_atom,
+                              const int *ip_displ, const double *gamma);
+lynx_status lynx_get_nonlocal_sizes(const lynx_ctx *ctx, int *n_img, long *n_gpos,
+                                    long *n_chi, int *n_atom, int *n_proj);
+lynx_status lynx_get_nonlocal(const lynx_ctx *ctx, int *img_atom, int *img_ndc,
+                              int *img_nproj, int *gpos, double *chi, double *img_shift,
+                              int *ip_displ, double *gamma);
+
+/* ------------------------------------------------- potential, k-point */
+
+/* Effective potential, Nd doubles (host or device pointer); the context
+ * keeps its own device copy.  Replaces upload_Veff_sync
+ * (EigenSolver.cu:1028-1033). */
+lynx_status lynx_set_veff(lynx_ctx *ctx, const double *veff, int mem);
+/* k-point in the lattice-aligned reciprocal basis (k_i = 2 pi k_red_i / L_i),
+ * as passed to Hamiltonian::apply_kpt (Hamiltonian.hpp:78-80) and
+ * NonlocalProjector::set_kpoint.  Used by the complex dtypes. */
+lynx_status lynx_set_kpoint(lynx_ctx *ctx, const double kpt[3]);
+
+/* ----------------------------------------------------- operator (a2-a7) */
+
+/* y = -1/2 lap psi + (Veff + c) psi + V_nl psi for ncol columns.
+ * Replaces Hamiltonian::apply (Hamiltonian.hpp:64-65) for LYNX_F64/F32 and
+ * Hamiltonian::apply_kpt (Hamiltonian.hpp:78-80) for LYNX_C128/C64.
+ * ld_psi, ld_y >= Nd (0 means Nd).  y is overwritten. */
+lynx_status lynx_h_apply(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, void *y,
+                         int ld_y, int ncol, double c, int mem);
+
+/* ------------------------------------------------------ filter (a8) */
+
+/* Degree-m scaled Chebyshev filter Y = p_m(H) X on [lambda_c, eig_max],
+ * normalised at eig_min.  Replaces EigenSolver::chebyshev_filter
+ * (EigenSolver.cpp:558-619, GPU chebyshev_filter_gpu EigenSolver.cu:755-783).
+ * x is read-only; y receives the result. */
+lynx_status lynx_cheb_filter(lynx_ctx *ctx, int dtype, const void *x, int ld_x, void *y,
+                             int ld_y, int ncol, int degree, double lambda_c,
+                             double eig_min, double eig_max, int mem);
+
+/* -------------------------------------------------- subspace (a9-a12) */
+
+/* S = dV X^H X, Cholesky S = R^H R, X <- X R^-1.  EigenSolver.cpp:621-641.
+ * Returns LYNX_E_CHOLESKY when S is not positive definite. */
+lynx_status lynx_orthogonalize(lynx_ctx *ctx, int dtype, void *x, int ld, int ncol, int mem);
+/* Hs = dV X^H H X, Hermitian part.  EigenSolver.cpp:643-660.  hs: ncol x ncol
+ * column-major host array (complex interleaved for C128). */
+lynx_status lynx_project_hamiltonian(lynx_ctx *ctx, int dtype, const void *x, int ld,
+                                     int ncol, void *hs, int mem);
+
+/* -------------------------------------------------------- solve (a13) */
+
+/* One CheFSI pass: filter, Cholesky-QR orthonormalisation, Rayleigh-Ritz,
+ * rotation.  psi (in/out) holds nband columns with leading dimension ld;
+ * eigvals (host, nband doubles) receives the Ritz values in ascending order;
+ * lambda_c_out (host, may be NULL) receives eigvals[nband-1] + 0.1.
+ * Replaces EigenSolver::solve (EigenSolver.hpp:38-41, EigenSolver.cpp:738-832)
+ * for LYNX_F64 and EigenSolver::solve_kpt (EigenSolver.hpp:49-53) for
+ * LYNX_C128. */
+lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int nband,
+                              double *eigvals, double lambda_c, double eig_min,
+                              double eig_max, int degree, double *lambda_c_out, int mem);
+
+/* ------------------------------------------------------ Lanczos (a14) */
+
+/* Spectral bounds of H by Lanczos from a glibc srand(seed)/rand() start
+ * vector: returns (lambda_min - 0.1, lambda_max * 1.01).  Replaces
+ * EigenSolver::lanczos_bounds (EigenSolver.cpp:724-732, Lanczos.hpp:104-198);
+ * the reference seeds with 1.  iters (may be NULL) gets the step count. */
+lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, double *eig_max,
+                                double tol, int max_iter, unsigned seed, int *iters);
+
+/* ------------------------------------------------------ diagnostics */
+
+/* Host-only helpers, usable without a GPU (no context needed). */
+/* FDStencil::compute_weights (FDStencil.cpp:23-44): w1, w2 of fd_order/2+1. */
+lynx_status lynx_fd_weights(int fd_order, double *w1, double *w2);
+/* psp8 header summary: zion, lmax, lloc, nproj_per_atom, rc_max. */
+lynx_status lynx_psp8_info(const char *path, double *zion, int *lmax, int *lloc,
+                           int *nproj_per_atom, double *rc_max);
+
+/* Timing of the most recent filter call on the context: total and
+ * per-kernel-class milliseconds measured with HIP events on its stream.
+ * times[0] total, [1] stencil steps, [2] nonlocal gather, [3] nonlocal
+ * scatter; launches[0..3] the matching launch counts. */
+lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int launches[4]);
+/* Enable (1) or disable (0) HIP-event timing of filter calls. */
+lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LYNX_CHEFSI_H */

@@ -1,0 +1,1064 @@
+// C ABI of liblynx_chefsi.so: context, operator set-up and the CheFSI
+// orchestration (filter, Cholesky-QR, Rayleigh-Ritz, rotation, Lanczos) on
+// one HIP stream.  Dense N x N factorisations stay in FP64 on the GPU
+// through rocSOLVER; everything Nd-sized runs in the project's own kernels.
+#include "lynx_chefsi.h"
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace lynx;
+
+namespace {
+
+#define HIP_OK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            fail(LYNX_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+#define BLAS_OK(expr)                                                                        \
+    do {                                                                                     \
+        rocblas_status s_ = (expr);                                                          \
+        if (s_ != rocblas_status_success)                                                    \
+            fail(LYNX_E_HIP, std::string(#expr) + ": " + rocblas_status_to_string(s_));      \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    void *get(size_t bytes)
+    {
+        if (bytes > n) {
+            if (p) HIP_OK(hipFree(p));
+            p = nullptr;
+            n = 0;
+            HIP_OK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+            n = bytes;
+        }
+        return p;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v, hipStream_t st)
+{
+    b.get(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
+struct lynx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+
+    bool has_grid = false;
+    Lattice lat{};
+    Stencil st{};
+    int nd = 0;
+
+    bool has_veff = false;
+    DevBuf veff64, veff32;
+    double kpt[3] = {0, 0, 0};
+
+    bool has_nl = false;
+    NonlocalHost nlh;
+    gpu::NlDev nl64, nl32;                         // views for double / float kernels
+    DevBuf d_atom_pt_off, d_pt_gpos, d_pt_img, d_pt_atom, d_cov_off, d_cov_gpos, d_ent_pt;
+    DevBuf d_chi64, d_chi32, d_gam64, d_gam32, d_ph64, d_ph32;
+    DevBuf beta;
+
+    DevBuf wA, wB, wC, lz[3];                      // Nd-sized work blocks
+    DevBuf s1, s2, gwork, dvals, evals, info, partials;
+    std::vector<double> hpart;
+
+    gpu::StepCfg cfg{};
+
+    bool timing = false;
+    double t_last[4] = {0, 0, 0, 0};
+    int n_last[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_marks;     // (class, start index)
+
+    ~lynx_ctx()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        DevBuf *bufs[] = {&veff64, &veff32, &d_atom_pt_off, &d_pt_gpos, &d_pt_img, &d_pt_atom, &d_cov_off,
+                          &d_cov_gpos, &d_ent_pt, &d_chi64, &d_chi32, &d_gam64, &d_gam32, &d_ph64, &d_ph32,
+                          &beta, &wA, &wB, &wC, &lz[0], &lz[1], &lz[2], &s1, &s2, &gwork, &dvals, &evals,
+                          &info, &partials};
+        for (DevBuf *b : bufs) b->release();
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (blas) rocblas_destroy_handle(blas);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class F>
+lynx_status guarded(lynx_ctx *ctx, F &&f)
+{
+    try {
+        if (ctx && ctx->device >= 0) HIP_OK(hipSetDevice(ctx->device));
+        f();
+        return LYNX_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = e.msg;
+        return e.status;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return LYNX_E_HIP;
+    }
+}
+
+void need_grid(const lynx_ctx *c)
+{
+    if (!c->has_grid) fail(LYNX_E_STATE, "lynx_set_grid has not been called");
+}
+
+void need_device(const lynx_ctx *c)
+{
+    if (c->device < 0) fail(LYNX_E_STATE, "host-only context (device -1) cannot run GPU work");
+}
+
+void need_gpu_stencil(const lynx_ctx *c)
+{
+    need_device(c);
+    need_grid(c);
+    if (c->st.fdn != gpu::kMaxR) fail(LYNX_E_UNSUPPORTED, "GPU stencil kernels are built for FD order 12");
+    if (!c->lat.orth) fail(LYNX_E_UNSUPPORTED, "non-orthogonal cells are not supported by the GPU stencil yet");
+    if (c->st.nx < gpu::kMaxR || c->st.ny < gpu::kMaxR || c->st.nz < gpu::kMaxR)
+        fail(LYNX_E_ARG, "grid dimensions must be at least the stencil radius");
+}
+
+void need_veff(const lynx_ctx *c)
+{
+    if (!c->has_veff) fail(LYNX_E_STATE, "lynx_set_veff has not been called");
+}
+
+void check_dtype(int dtype)
+{
+    if (dtype < 0 || dtype > 3) fail(LYNX_E_ARG, "unknown dtype");
+}
+
+gpu::StencilGeom geom(const lynx_ctx *c)
+{
+    gpu::StencilGeom g{};
+    g.nx = c->st.nx; g.ny = c->st.ny; g.nz = c->st.nz; g.R = c->st.fdn;
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k <= gpu::kMaxR; ++k) g.c[a][k] = k <= c->st.fdn ? c->st.D2[a][k] : 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double th = c->kpt[a] * c->lat.L[a];
+        g.phase[a][0][0] = std::cos(th);
+        g.phase[a][0][1] = -std::sin(th);
+        g.phase[a][1][0] = std::cos(th);
+        g.phase[a][1][1] = std::sin(th);
+    }
+    return g;
+}
+
+const void *veff_for(const lynx_ctx *c, int e)
+{
+    return gpu::elem_single(e) ? c->veff32.p : c->veff64.p;
+}
+
+const gpu::NlDev &nl_for(const lynx_ctx *c, int e) { return gpu::elem_single(e) ? c->nl32 : c->nl64; }
+
+// ---------------------------------------------------------------- timing --
+int ev_index(lynx_ctx *c, size_t i)
+{
+    while (c->ev_pool.size() <= i) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    return (int)i;
+}
+
+struct Marker {
+    lynx_ctx *c;
+    int cls;
+    size_t idx = 0;
+    Marker(lynx_ctx *ctx, int k) : c(ctx), cls(k)
+    {
+        if (!c->timing) return;
+        idx = c->ev_marks.size() * 2;
+        ev_index(c, idx + 1);
+        HIP_OK(hipEventRecord(c->ev_pool[idx], c->stream));
+    }
+    ~Marker()
+    {
+        if (!c->timing) return;
+        if (hipEventRecord(c->ev_pool[idx + 1], c->stream) == hipSuccess)
+            c->ev_marks.push_back({cls, (int)idx});
+    }
+};
+
+void timing_begin(lynx_ctx *c) { c->ev_marks.clear(); }
+
+void timing_end(lynx_ctx *c)
+{
+    if (!c->timing) return;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; ++k) { c->t_last[k] = 0; c->n_last[k] = 0; }
+    for (auto &m : c->ev_marks) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        c->t_last[m.first] += ms;
+        c->n_last[m.first] += 1;
+    }
+    if (!c->ev_marks.empty()) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, c->ev_pool[c->ev_marks.front().second],
+                                   c->ev_pool[c->ev_marks.back().second + 1]));
+        c->t_last[0] = ms;
+        c->n_last[0] = (int)c->ev_marks.size();
+    }
+}
+
+// ------------------------------------------------------- nonlocal upload --
+void upload_nonlocal(lynx_ctx *c)
+{
+    const NonlocalHost &h = c->nlh;
+    const double dV = c->st.dV;
+    int maxp = 0;
+    for (int i = 0; i < h.n_img; ++i) maxp = std::max(maxp, h.img_nproj[i]);
+    for (int a = 0; a < h.n_atom; ++a) maxp = std::max(maxp, h.ip_displ[a + 1] - h.ip_displ[a]);
+    if (maxp > 32) fail(LYNX_E_UNSUPPORTED, "more than 32 projectors per atom");
+    const int npm = maxp <= 8 ? 8 : (maxp <= 16 ? 16 : 32);
+
+    std::vector<long> img_goff(h.n_img + 1, 0), img_coff(h.n_img + 1, 0);
+    for (int i = 0; i < h.n_img; ++i) {
+        img_goff[i + 1] = img_goff[i] + h.img_ndc[i];
+        img_coff[i + 1] = img_coff[i] + (long)h.img_ndc[i] * h.img_nproj[i];
+    }
+    std::vector<int> atom_pt_off(h.n_atom + 1, 0), pt_gpos, pt_img, pt_atom;
+    std::vector<double> chi;
+    for (int a = 0; a < h.n_atom; ++a) {
+        for (int i = 0; i < h.n_img; ++i) {
+            if (h.img_atom[i] != a) continue;
+            const int ndc = h.img_ndc[i], np = h.img_nproj[i];
+            for (int g = 0; g < ndc; ++g) {
+                pt_gpos.push_back(h.gpos[img_goff[i] + g]);
+                pt_img.push_back(i);
+                pt_atom.push_back(a);
+                for (int j = 0; j < npm; ++j)
+                    chi.push_back(j < np ? h.chi[img_coff[i] + (long)j * ndc + g] : 0.0);
+            }
+        }
+        atom_pt_off[a + 1] = (int)pt_gpos.size();
+    }
+    const int npts = (int)pt_gpos.size();
+    std::vector<double> gam((size_t)h.n_atom * npm, 0.0);
+    for (int a = 0; a < h.n_atom; ++a)
+        for (int j = 0; j < h.ip_displ[a + 1] - h.ip_displ[a]; ++j)
+            gam[(size_t)a * npm + j] = dV * h.gamma[h.ip_displ[a] + j];
+    // coverage lists: entries of each covered grid point, in gather order
+    std::vector<int> order(npts);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pt_gpos[x] < pt_gpos[y]; });
+    std::vector<int> cov_off{0}, cov_gpos, ent_pt;
+    for (int k = 0; k < npts; ++k) {
+        const int pt = order[k];
+        if (cov_gpos.empty() || cov_gpos.back() != pt_gpos[pt]) {
+            if (!cov_gpos.empty()) cov_off.push_back((int)ent_pt.size());
+            cov_gpos.push_back(pt_gpos[pt]);
+        }
+        ent_pt.push_back(pt);
+    }
+    if (!cov_gpos.empty()) cov_off.push_back((int)ent_pt.size());
+    for (int g : pt_gpos)
+        if (g < 0 || g >= c->nd) fail(LYNX_E_ARG, "nonlocal grid index out of range");
+
+    hipStream_t s = c->stream;
+    upload(c->d_atom_pt_off, atom_pt_off, s);
+    upload(c->d_pt_gpos, pt_gpos, s);
+    upload(c->d_pt_img, pt_img, s);
+    upload(c->d_pt_atom, pt_atom, s);
+    upload(c->d_cov_off, cov_off, s);
+    upload(c->d_cov_gpos, cov_gpos, s);
+    upload(c->d_ent_pt, ent_pt, s);
+    upload(c->d_chi64, chi, s);
+    upload(c->d_gam64, gam, s);
+    std::vector<float> chi32(chi.begin(), chi.end()), gam32(gam.begin(), gam.end());
+    upload(c->d_chi32, chi32, s);
+    upload(c->d_gam32, gam32, s);
+    HIP_OK(hipStreamSynchronize(s));
+
+    for (gpu::NlDev *v : {&c->nl64, &c->nl32}) {
+        v->n_atom = h.n_atom;
+        v->npm = npm;
+        v->n_pts = npts;
+        v->n_cov = (int)cov_gpos.size();
+        v->n_img = h.n_img;
+        v->atom_pt_off = c->d_atom_pt_off.as<int>();
+        v->pt_gpos = c->d_pt_gpos.as<int>();
+        v->pt_img = c->d_pt_img.as<int>();
+        v->pt_atom = c->d_pt_atom.as<int>();
+        v->cov_off = c->d_cov_off.as<int>();
+        v->cov_gpos = c->d_cov_gpos.as<int>();
+        v->ent_pt = c->d_ent_pt.as<int>();
+    }
+    c->nl64.chi = c->d_chi64.p; c->nl64.gamma = c->d_gam64.p;
+    c->nl32.chi = c->d_chi32.p; c->nl32.gamma = c->d_gam32.p;
+    c->has_nl = h.n_img > 0;
+}
+
+// Bloch factors e^{-i k . R_img} of the images for the current k-point
+// (NonlocalProjector.cpp:268-272).
+void upload_phases(lynx_ctx *c)
+{
+    const NonlocalHost &h = c->nlh;
+    std::vector<double> ph(2 * std::max(1, h.n_img));
+    for (int i = 0; i < h.n_img; ++i) {
+        const double *sh = &h.img_shift[3 * i];
+        const double th = -(c->kpt[0] * sh[0] + c->kpt[1] * sh[1] + c->kpt[2] * sh[2]);
+        ph[2 * i] = std::cos(th);
+        ph[2 * i + 1] = std::sin(th);
+    }
+    upload(c->d_ph64, ph, c->stream);
+    std::vector<float> ph32(ph.begin(), ph.end());
+    upload(c->d_ph32, ph32, c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->nl64.phase = c->d_ph64.p;
+    c->nl32.phase = c->d_ph32.p;
+}
+
+// ------------------------------------------------------------- operator --
+// y = a * (H - shift) x + b * old, H = -1/2 lap + Veff + V_nl.
+void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long ldo, void *y, long ldy,
+              int ncol, double a, double b, double shift)
+{
+    const bool nl = c->has_nl;
+    const gpu::NlDev &nd = nl_for(c, e);
+    if (nl) {
+        c->beta.get((size_t)ncol * nd.n_atom * nd.npm * gpu::elem_size(e));
+        Marker m(c, 2);
+        gpu::nl_gather(e, nd, x, ldx, c->beta.p, ncol, c->stream);
+    }
+    {
+        Marker m(c, 1);
+        gpu::stencil_step(e, geom(c), x, ldx, old, ldo, y, ldy, veff_for(c, e), ncol, a, b, shift, c->cfg,
+                          c->stream);
+    }
+    if (nl) {
+        Marker m(c, 3);
+        gpu::nl_scatter(e, nd, c->beta.p, y, ldy, ncol, a, c->stream);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+// EigenSolver.cpp:558-619 as a sequence of fused steps.  x is read only; the
+// final block lands in y; w is a work block of the same shape.
+void filter(lynx_ctx *c, int e, const void *x, long ldx, void *y, long ldy, void *w, long ldw, int ncol,
+            int degree, double lambda_c, double eig_min, double eig_max)
+{
+    if (degree < 1) fail(LYNX_E_ARG, "degree must be >= 1");
+    const double ehalf = (eig_max - lambda_c) / 2.0;
+    const double cen = (eig_max + lambda_c) / 2.0;
+    if (!(ehalf > 0)) fail(LYNX_E_ARG, "eig_max must exceed lambda_cutoff");
+    const double sigma1 = ehalf / (eig_min - cen);
+    double sigma = sigma1;
+    // odd steps write D1, even steps D2; the last step must land in y
+    void *D1 = (degree % 2 == 1) ? y : w;
+    void *D2 = (degree % 2 == 1) ? w : y;
+    const long ld1 = (degree % 2 == 1) ? ldy : ldw;
+    const long ld2 = (degree % 2 == 1) ? ldw : ldy;
+    apply_op(c, e, x, ldx, nullptr, 0, D1, ld1, ncol, sigma / ehalf, 0.0, -cen);
+    const void *prev = D1;   // Y_{k-1}
+    long ldp = ld1;
+    const void *pprev = x;   // Y_{k-2}
+    long ldpp = ldx;
+    for (int k = 2; k <= degree; ++k) {
+        const double sn = 1.0 / (2.0 / sigma1 - sigma);
+        const double gam = 2.0 * sn / ehalf;
+        void *dst = (k % 2 == 1) ? D1 : D2;
+        const long ldd = (k % 2 == 1) ? ld1 : ld2;
+        apply_op(c, e, prev, ldp, pprev, ldpp, dst, ldd, ncol, gam, -sigma * sn, -cen);
+        pprev = prev; ldpp = ldp;
+        prev = dst; ldp = ldd;
+        sigma = sn;
+    }
+}
+
+long padded_ld(long nd) { return (nd + 63) / 64 * 64; }
+
+void *block(DevBuf &b, int e, long ld, int ncol) { return b.get((size_t)ld * ncol * gpu::elem_size(e)); }
+
+// ------------------------------------------------------- dense FP64 steps --
+void check_info(lynx_ctx *c, int status, const char *what)
+{
+    int info = 0;
+    HIP_OK(hipMemcpyAsync(&info, c->info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (info != 0) fail(status, std::string(what) + " failed (info=" + std::to_string(info) + ")");
+}
+
+// S = dV X^H X; R^H R = S; returns R^-1 (upper, zero below) in c->s1.
+void cholesky_inverse(lynx_ctx *c, int e, const void *X, long ldx, int n)
+{
+    const size_t es = gpu::elem_size(e);
+    void *S = c->s1.get((size_t)n * n * es);
+    c->gwork.get(gpu::gram_work_bytes(e, c->nd, n));
+    gpu::gram(e, X, ldx, X, ldx, c->nd, n, c->st.dV, S, c->gwork.p, c->stream);
+    c->info.get(sizeof(int));
+    if (e == gpu::F64) {
+        BLAS_OK(rocsolver_dpotrf(c->blas, rocblas_fill_upper, n, (double *)S, n, c->info.as<rocblas_int>()));
+        check_info(c, LYNX_E_CHOLESKY, "Cholesky factorisation of the overlap matrix");
+        BLAS_OK(rocsolver_dtrtri(c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, n, (double *)S, n,
+                                 c->info.as<rocblas_int>()));
+    } else {
+        BLAS_OK(rocsolver_zpotrf(c->blas, rocblas_fill_upper, n, (rocblas_double_complex *)S, n,
+                                 c->info.as<rocblas_int>()));
+        check_info(c, LYNX_E_CHOLESKY, "Cholesky factorisation of the overlap matrix");
+        BLAS_OK(rocsolver_ztrtri(c->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, n,
+                                 (rocblas_double_complex *)S, n, c->info.as<rocblas_int>()));
+    }
+    check_info(c, LYNX_E_CHOLESKY, "triangular inverse");
+    gpu::zero_lower(e, S, n, c->stream);
+}
+
+// Hermitian eigensolve of c->s2 (n x n) in place; eigenvalues to host.
+void subspace_eig(lynx_ctx *c, int e, int n, double *eig_host)
+{
+    double *D = (double *)c->dvals.get(n * sizeof(double));
+    double *E = (double *)c->evals.get(n * sizeof(double));
+    c->info.get(sizeof(int));
+    if (e == gpu::F64)
+        BLAS_OK(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, (double *)c->s2.p, n, D,
+                                 E, c->info.as<rocblas_int>()));
+    else
+        BLAS_OK(rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                 (rocblas_double_complex *)c->s2.p, n, D, E, c->info.as<rocblas_int>()));
+    check_info(c, LYNX_E_EIG, "subspace eigensolve");
+    HIP_OK(hipMemcpyAsync(eig_host, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(eig_host[i])) fail(LYNX_E_NONFINITE, "non-finite Ritz value");
+}
+
+// Sturm-sequence bisection for the extremal eigenvalues of the symmetric
+// tridiagonal matrix (d, e); the reference obtains all of them with dsterf.
+double tridiag_extreme(const double *d, const double *e, int n, bool largest)
+{
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; ++i) {
+        double r = 0;
+        if (i > 0) r += std::fabs(e[i - 1]);
+        if (i < n - 1) r += std::fabs(e[i]);
+        lo = std::min(lo, d[i] - r);
+        hi = std::max(hi, d[i] + r);
+    }
+    const double span = hi - lo + 1e-300;
+    lo -= 1e-12 * span;
+    hi += 1e-12 * span;
+    auto below = [&](double x) {
+        int cnt = 0;
+        double q = d[0] - x;
+        if (q < 0) ++cnt;
+        for (int i = 1; i < n; ++i) {
+            if (q == 0.0) q = 1e-300;
+            q = d[i] - x - e[i - 1] * e[i - 1] / q;
+            if (q < 0) ++cnt;
+        }
+        return cnt;
+    };
+    for (int it = 0; it < 200 && hi - lo > 4e-16 * (std::fabs(lo) + std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        const int k = below(mid);
+        if (!largest) { if (k >= 1) hi = mid; else lo = mid; }
+        else { if (k >= n) hi = mid; else lo = mid; }
+    }
+    return 0.5 * (lo + hi);
+}
+
+double host_sum(lynx_ctx *c, int nb)
+{
+    c->hpart.resize(nb);
+    HIP_OK(hipMemcpyAsync(c->hpart.data(), c->partials.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double s = 0;
+    for (int i = 0; i < nb; ++i) s += c->hpart[i];
+    return s;
+}
+
+}  // namespace
+
+// ======================================================================= API
+
+extern "C" {
+
+const char *lynx_status_string(lynx_status s)
+{
+    switch (s) {
+        case LYNX_OK: return "ok";
+        case LYNX_E_ARG: return "invalid argument";
+        case LYNX_E_HIP: return "HIP runtime or library failure";
+        case LYNX_E_CHOLESKY: return "overlap matrix not positive definite";
+        case LYNX_E_EIG: return "eigensolver failure";
+        case LYNX_E_NONFINITE: return "non-finite value";
+        case LYNX_E_IO: return "pseudopotential file error";
+        case LYNX_E_STATE: return "required set-up call missing";
+        case LYNX_E_UNSUPPORTED: return "unsupported input";
+        default: return "unknown status";
+    }
+}
+
+lynx_status lynx_ctx_create(lynx_ctx **out, int device_id)
+{
+    if (!out) return LYNX_E_ARG;
+    *out = nullptr;
+    lynx_ctx *c = new lynx_ctx();
+    c->device = device_id;
+    if (device_id == -1) {   // host-only context: set-up and inspection, no GPU
+        *out = c;
+        return LYNX_OK;
+    }
+    lynx_status s = guarded(c, [&] {
+        int n = 0;
+        HIP_OK(hipGetDeviceCount(&n));
+        if (device_id < 0 || device_id >= n) fail(LYNX_E_ARG, "no such HIP device");
+        HIP_OK(hipSetDevice(device_id));
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        BLAS_OK(rocblas_create_handle(&c->blas));
+        BLAS_OK(rocblas_set_stream(c->blas, c->stream));
+    });
+    if (s != LYNX_OK) {
+        delete c;
+        return s;
+    }
+    *out = c;
+    return LYNX_OK;
+}
+
+void lynx_ctx_destroy(lynx_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+    delete ctx;
+}
+
+const char *lynx_last_error(const lynx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+lynx_status lynx_ctx_stream(const lynx_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return LYNX_E_ARG;
+    *stream = ctx->stream;
+    return LYNX_OK;
+}
+
+lynx_status lynx_ctx_synchronize(lynx_ctx *ctx)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] { HIP_OK(hipStreamSynchronize(ctx->stream)); });
+}
+
+lynx_status lynx_set_grid(lynx_ctx *ctx, int nx, int ny, int nz, const double latvec[9], int cell_type,
+                          int fd_order)
+{
+    if (!ctx || !latvec) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        if (nx <= 0 || ny <= 0 || nz <= 0) fail(LYNX_E_ARG, "grid dimensions must be positive");
+        if ((long)nx * ny * nz > (1L << 31) - 1) fail(LYNX_E_ARG, "grid too large");
+        if (cell_type != LYNX_CELL_ORTHOGONAL && cell_type != LYNX_CELL_NONORTHOGONAL)
+            fail(LYNX_E_ARG, "unknown cell type");
+        Lattice lat;
+        lattice_init(lat, latvec, cell_type == LYNX_CELL_ORTHOGONAL);
+        Stencil st;
+        stencil_init(st, fd_order, nx, ny, nz, lat);
+        ctx->lat = lat;
+        ctx->st = st;
+        ctx->nd = nx * ny * nz;
+        ctx->has_grid = true;
+        ctx->has_veff = false;
+        ctx->has_nl = false;
+        ctx->nlh = NonlocalHost{};
+    });
+}
+
+lynx_status lynx_get_grid_info(const lynx_ctx *ctx, double h[3], double *dV, double *max_eig)
+{
+    if (!ctx) return LYNX_E_ARG;
+    if (!ctx->has_grid) return LYNX_E_STATE;
+    if (h) for (int a = 0; a < 3; ++a) h[a] = ctx->st.h[a];
+    if (dV) *dV = ctx->st.dV;
+    if (max_eig) *max_eig = ctx->st.max_eig;
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_stencil(const lynx_ctx *ctx, double *D1, double *D2, double *D2m)
+{
+    if (!ctx) return LYNX_E_ARG;
+    if (!ctx->has_grid) return LYNX_E_STATE;
+    const int m = ctx->st.fdn + 1;
+    for (int a = 0; a < 3; ++a)
+        for (int p = 0; p < m; ++p) {
+            if (D1) D1[a * m + p] = ctx->st.D1[a][p];
+            if (D2) D2[a * m + p] = ctx->st.D2[a][p];
+            if (D2m) D2m[a * m + p] = ctx->st.D2m[a][p];
+        }
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_atoms(lynx_ctx *ctx, int ntypes, const char *const *paths, int natom, const int *atom_type,
+                           const double *coords, int frac)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_grid(ctx);
+        if (ntypes <= 0 || natom < 0 || !paths || (natom > 0 && (!atom_type || !coords)))
+            fail(LYNX_E_ARG, "bad atom arguments");
+        std::vector<Psp8> psps(ntypes);
+        for (int t = 0; t < ntypes; ++t) psp8_load(paths[t], psps[t]);
+        std::vector<int> types(atom_type, atom_type + natom);
+        for (int i = 0; i < natom; ++i) {
+            if (types[i] < 0 || types[i] >= ntypes) fail(LYNX_E_ARG, "atom type out of range");
+            if (i > 0 && types[i] < types[i - 1]) fail(LYNX_E_ARG, "atoms must be listed type-major");
+        }
+        // Crystal.cpp:284-295: lattice-aligned coordinates
+        std::vector<double> pos(3 * natom);
+        for (int i = 0; i < natom; ++i) {
+            const double *v = coords + 3 * i;
+            double *o = &pos[3 * i];
+            if (frac) {
+                if (ctx->lat.orth) {
+                    for (int j = 0; j < 3; ++j)
+                        o[j] = v[0] * ctx->lat.latvec[j] + v[1] * ctx->lat.latvec[3 + j] + v[2] * ctx->lat.latvec[6 + j];
+                } else {
+                    for (int j = 0; j < 3; ++j) o[j] = v[j] * ctx->lat.L[j];
+                }
+            } else if (ctx->lat.orth) {
+                for (int j = 0; j < 3; ++j) o[j] = v[j];
+            } else {
+                // cart -> fractional (rows of latvec are vectors) -> scaled
+                double inv[9], f[3];
+                const double *a = ctx->lat.latvec;
+                const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                                   a[2] * (a[3] * a[7] - a[4] * a[6]);
+                inv[0] = (a[4] * a[8] - a[5] * a[7]) / det; inv[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+                inv[2] = (a[1] * a[5] - a[2] * a[4]) / det; inv[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+                inv[4] = (a[0] * a[8] - a[2] * a[6]) / det; inv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+                inv[6] = (a[3] * a[7] - a[4] * a[6]) / det; inv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+                inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+                for (int j = 0; j < 3; ++j) f[j] = v[0] * inv[j] + v[1] * inv[3 + j] + v[2] * inv[6 + j];
+                for (int j = 0; j < 3; ++j) o[j] = f[j] * ctx->lat.L[j];
+            }
+        }
+        build_nonlocal(ctx->lat, ctx->st, psps, types, pos, ctx->nlh);
+        if (ctx->device >= 0) {
+            upload_nonlocal(ctx);
+            upload_phases(ctx);
+        }
+    });
+}
+
+lynx_status lynx_set_nonlocal(lynx_ctx *ctx, int n_img, const int *img_atom, const int *img_ndc,
+                              const int *img_nproj, const int *gpos, const double *chi, const double *img_shift,
+                              int n_atom, const int *ip_displ, const double *gamma)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_grid(ctx);
+        if (n_img < 0 || n_atom < 0) fail(LYNX_E_ARG, "negative sizes");
+        NonlocalHost h;
+        h.n_img = n_img;
+        h.n_atom = n_atom;
+        h.img_atom.assign(img_atom, img_atom + n_img);
+        h.img_ndc.assign(img_ndc, img_ndc + n_img);
+        h.img_nproj.assign(img_nproj, img_nproj + n_img);
+        long ng = 0, nc = 0;
+        for (int i = 0; i < n_img; ++i) {
+            if (h.img_atom[i] < 0 || h.img_atom[i] >= n_atom) fail(LYNX_E_ARG, "image atom out of range");
+            if (h.img_ndc[i] < 0 || h.img_nproj[i] < 0) fail(LYNX_E_ARG, "negative image size");
+            ng += h.img_ndc[i];
+            nc += (long)h.img_ndc[i] * h.img_nproj[i];
+        }
+        h.gpos.assign(gpos, gpos + ng);
+        h.chi.assign(chi, chi + nc);
+        h.img_shift.assign(img_shift, img_shift + 3L * n_img);
+        h.ip_displ.assign(ip_displ, ip_displ + n_atom + 1);
+        for (int a = 0; a < n_atom; ++a)
+            if (h.ip_displ[a + 1] < h.ip_displ[a]) fail(LYNX_E_ARG, "ip_displ must be non-decreasing");
+        for (int i = 0; i < n_img; ++i) {
+            const int a = h.img_atom[i];
+            if (h.img_nproj[i] != h.ip_displ[a + 1] - h.ip_displ[a])
+                fail(LYNX_E_ARG, "image projector count differs from its atom's");
+        }
+        h.gamma.assign(gamma, gamma + h.ip_displ[n_atom]);
+        for (int g : h.gpos)
+            if (g < 0 || g >= ctx->nd) fail(LYNX_E_ARG, "nonlocal grid index out of range");
+        ctx->nlh = std::move(h);
+        if (ctx->device >= 0) {
+            upload_nonlocal(ctx);
+            upload_phases(ctx);
+        }
+    });
+}
+
+lynx_status lynx_get_nonlocal_sizes(const lynx_ctx *ctx, int *n_img, long *n_gpos, long *n_chi, int *n_atom,
+                                    int *n_proj)
+{
+    if (!ctx) return LYNX_E_ARG;
+    const NonlocalHost &h = ctx->nlh;
+    if (n_img) *n_img = h.n_img;
+    if (n_gpos) *n_gpos = (long)h.gpos.size();
+    if (n_chi) *n_chi = (long)h.chi.size();
+    if (n_atom) *n_atom = h.n_atom;
+    if (n_proj) *n_proj = h.ip_displ.empty() ? 0 : h.ip_displ.back();
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_nonlocal(const lynx_ctx *ctx, int *img_atom, int *img_ndc, int *img_nproj, int *gpos,
+                              double *chi, double *img_shift, int *ip_displ, double *gamma)
+{
+    if (!ctx) return LYNX_E_ARG;
+    const NonlocalHost &h = ctx->nlh;
+    auto cp = [](auto *dst, const auto &v) {
+        if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(img_atom, h.img_atom); cp(img_ndc, h.img_ndc); cp(img_nproj, h.img_nproj); cp(gpos, h.gpos);
+    cp(chi, h.chi); cp(img_shift, h.img_shift); cp(ip_displ, h.ip_displ); cp(gamma, h.gamma);
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_veff(lynx_ctx *ctx, const double *veff, int mem)
+{
+    if (!ctx || !veff) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_device(ctx);
+        need_grid(ctx);
+        const size_t n = ctx->nd;
+        ctx->veff64.get(n * sizeof(double));
+        HIP_OK(hipMemcpyAsync(ctx->veff64.p, veff, n * sizeof(double),
+                              mem == LYNX_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              ctx->stream));
+        std::vector<double> hv(n);
+        HIP_OK(hipMemcpyAsync(hv.data(), ctx->veff64.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        for (double v : hv)
+            if (!std::isfinite(v)) fail(LYNX_E_NONFINITE, "non-finite effective potential");
+        std::vector<float> hf(hv.begin(), hv.end());
+        upload(ctx->veff32, hf, ctx->stream);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        ctx->has_veff = true;
+    });
+}
+
+lynx_status lynx_set_kpoint(lynx_ctx *ctx, const double kpt[3])
+{
+    if (!ctx || !kpt) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        for (int a = 0; a < 3; ++a) ctx->kpt[a] = kpt[a];
+        if (ctx->nlh.n_img > 0 && ctx->device >= 0) upload_phases(ctx);
+    });
+}
+
+lynx_status lynx_h_apply(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, void *y, int ld_y, int ncol,
+                         double c, int mem)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (ncol <= 0 || !psi || !y) fail(LYNX_E_ARG, "bad block arguments");
+        const long nd = ctx->nd;
+        const long ldp = ld_psi ? ld_psi : nd, ldy = ld_y ? ld_y : nd;
+        if (ldp < nd || ldy < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const size_t es = gpu::elem_size(dtype);
+        timing_begin(ctx);
+        if (mem == LYNX_MEM_DEVICE) {
+            apply_op(ctx, dtype, psi, ldp, nullptr, 0, y, ldy, ncol, 1.0, 0.0, c);
+        } else {
+            void *dx = block(ctx->wA, dtype, nd, ncol);
+            void *dy = block(ctx->wB, dtype, nd, ncol);
+            HIP_OK(hipMemcpy2DAsync(dx, nd * es, psi, ldp * es, nd * es, ncol, hipMemcpyHostToDevice, ctx->stream));
+            apply_op(ctx, dtype, dx, nd, nullptr, 0, dy, nd, ncol, 1.0, 0.0, c);
+            HIP_OK(hipMemcpy2DAsync(y, ldy * es, dy, nd * es, nd * es, ncol, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        timing_end(ctx);
+    });
+}
+
+lynx_status lynx_cheb_filter(lynx_ctx *ctx, int dtype, const void *x, int ld_x, void *y, int ld_y, int ncol,
+                             int degree, double lambda_c, double eig_min, double eig_max, int mem)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (ncol <= 0 || !x || !y) fail(LYNX_E_ARG, "bad block arguments");
+        const long nd = ctx->nd;
+        const long ldx = ld_x ? ld_x : nd, ldy = ld_y ? ld_y : nd;
+        if (ldx < nd || ldy < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const size_t es = gpu::elem_size(dtype);
+        const long ldw = padded_ld(nd);
+        timing_begin(ctx);
+        if (mem == LYNX_MEM_DEVICE) {
+            void *w = block(ctx->wC, dtype, ldw, ncol);
+            filter(ctx, dtype, x, ldx, y, ldy, w, ldw, ncol, degree, lambda_c, eig_min, eig_max);
+        } else {
+            void *dx = block(ctx->wA, dtype, nd, ncol);
+            void *dy = block(ctx->wB, dtype, nd, ncol);
+            void *w = block(ctx->wC, dtype, ldw, ncol);
+            HIP_OK(hipMemcpy2DAsync(dx, nd * es, x, ldx * es, nd * es, ncol, hipMemcpyHostToDevice, ctx->stream));
+            filter(ctx, dtype, dx, nd, dy, nd, w, ldw, ncol, degree, lambda_c, eig_min, eig_max);
+            HIP_OK(hipMemcpy2DAsync(y, ldy * es, dy, nd * es, nd * es, ncol, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        timing_end(ctx);
+    });
+}
+
+lynx_status lynx_orthogonalize(lynx_ctx *ctx, int dtype, void *x, int ld, int ncol, int mem)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_device(ctx);
+        need_grid(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128) fail(LYNX_E_UNSUPPORTED, "orthogonalisation runs in FP64");
+        if (ncol <= 0 || !x) fail(LYNX_E_ARG, "bad block arguments");
+        const long nd = ctx->nd, ldx = ld ? ld : nd;
+        if (ldx < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const size_t es = gpu::elem_size(dtype);
+        void *dx = x;
+        long ldd = ldx;
+        if (mem != LYNX_MEM_DEVICE) {
+            dx = block(ctx->wA, dtype, nd, ncol);
+            ldd = nd;
+            HIP_OK(hipMemcpy2DAsync(dx, nd * es, x, ldx * es, nd * es, ncol, hipMemcpyHostToDevice, ctx->stream));
+        }
+        cholesky_inverse(ctx, dtype, dx, ldd, ncol);
+        void *tmp = block(ctx->wB, dtype, padded_ld(nd), ncol);
+        gpu::rotate(dtype, dx, ldd, ctx->s1.p, ncol, tmp, padded_ld(nd), (int)nd, ctx->stream);
+        gpu::copy_cols(dtype, tmp, padded_ld(nd), dx, ldd, nd, ncol, ctx->stream);
+        if (mem != LYNX_MEM_DEVICE)
+            HIP_OK(hipMemcpy2DAsync(x, ldx * es, dx, nd * es, nd * es, ncol, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+lynx_status lynx_project_hamiltonian(lynx_ctx *ctx, int dtype, const void *x, int ld, int ncol, void *hs, int mem)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128) fail(LYNX_E_UNSUPPORTED, "projection runs in FP64");
+        if (ncol <= 0 || !x || !hs) fail(LYNX_E_ARG, "bad block arguments");
+        const long nd = ctx->nd, ldx = ld ? ld : nd;
+        if (ldx < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const size_t es = gpu::elem_size(dtype);
+        const void *dx = x;
+        long ldd = ldx;
+        if (mem != LYNX_MEM_DEVICE) {
+            void *t = block(ctx->wA, dtype, nd, ncol);
+            HIP_OK(hipMemcpy2DAsync(t, nd * es, x, ldx * es, nd * es, ncol, hipMemcpyHostToDevice, ctx->stream));
+            dx = t;
+            ldd = nd;
+        }
+        const long ldw = padded_ld(nd);
+        void *hx = block(ctx->wB, dtype, ldw, ncol);
+        apply_op(ctx, dtype, dx, ldd, nullptr, 0, hx, ldw, ncol, 1.0, 0.0, 0.0);
+        void *H = ctx->s2.get((size_t)ncol * ncol * es);
+        ctx->gwork.get(gpu::gram_work_bytes(dtype, (int)nd, ncol));
+        gpu::gram(dtype, dx, ldd, hx, ldw, (int)nd, ncol, ctx->st.dV, H, ctx->gwork.p, ctx->stream);
+        gpu::hermitian_part(dtype, H, ncol, ctx->stream);
+        HIP_OK(hipMemcpyAsync(hs, H, (size_t)ncol * ncol * es, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int nband, double *eigvals,
+                              double lambda_c, double eig_min, double eig_max, int degree, double *lambda_c_out,
+                              int mem)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128)
+            fail(LYNX_E_UNSUPPORTED, "the solve takes FP64 wavefunctions (real or complex)");
+        if (nband <= 0 || !psi || !eigvals) fail(LYNX_E_ARG, "bad block arguments");
+        const long nd = ctx->nd, ldp = ld ? ld : nd;
+        if (ldp < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        if (nband > nd) fail(LYNX_E_ARG, "more bands than grid points");
+        const size_t es = gpu::elem_size(dtype);
+        const long ldw = padded_ld(nd);
+        void *P = psi;
+        long ldP = ldp;
+        if (mem != LYNX_MEM_DEVICE) {
+            P = block(ctx->wA, dtype, nd, nband);
+            ldP = nd;
+            HIP_OK(hipMemcpy2DAsync(P, nd * es, psi, ldp * es, nd * es, nband, hipMemcpyHostToDevice, ctx->stream));
+        }
+        void *B = block(ctx->wB, dtype, ldw, nband);
+        void *C = block(ctx->wC, dtype, ldw, nband);
+        timing_begin(ctx);
+        // a8: Y = p_m(H) psi -> B
+        filter(ctx, dtype, P, ldP, B, ldw, C, ldw, nband, degree, lambda_c, eig_min, eig_max);
+        timing_end(ctx);
+        // a9: C = B R^-1 with dV B^H B = R^H R
+        cholesky_inverse(ctx, dtype, B, ldw, nband);
+        gpu::rotate(dtype, B, ldw, ctx->s1.p, nband, C, ldw, (int)nd, ctx->stream);
+        // a10: Hs = dV C^H H C (B <- H C)
+        apply_op(ctx, dtype, C, ldw, nullptr, 0, B, ldw, nband, 1.0, 0.0, 0.0);
+        void *H = ctx->s2.get((size_t)nband * nband * es);
+        ctx->gwork.get(gpu::gram_work_bytes(dtype, (int)nd, nband));
+        gpu::gram(dtype, C, ldw, B, ldw, (int)nd, nband, ctx->st.dV, H, ctx->gwork.p, ctx->stream);
+        gpu::hermitian_part(dtype, H, nband, ctx->stream);
+        // a11: Hs = Q diag(eig) Q^H
+        subspace_eig(ctx, dtype, nband, eigvals);
+        // a12: psi = C Q
+        gpu::rotate(dtype, C, ldw, H, nband, P, ldP, (int)nd, ctx->stream);
+        if (mem != LYNX_MEM_DEVICE)
+            HIP_OK(hipMemcpy2DAsync(psi, ldp * es, P, nd * es, nd * es, nband, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        if (lambda_c_out) *lambda_c_out = eigvals[nband - 1] + 0.1;
+    });
+}
+
+lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, double *eig_max, double tol,
+                                int max_iter, unsigned seed, int *iters)
+{
+    if (!ctx || !eig_min || !eig_max) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128) fail(LYNX_E_UNSUPPORTED, "Lanczos runs in FP64");
+        if (max_iter < 1) fail(LYNX_E_ARG, "max_iter must be positive");
+        const long n = ctx->nd;
+        const bool cplx = dtype == LYNX_C128;
+        const int e = dtype;
+        const size_t es = gpu::elem_size(e);
+        void *vjm1 = ctx->lz[0].get(n * es);
+        void *vj = ctx->lz[1].get(n * es);
+        void *vjp1 = ctx->lz[2].get(n * es);
+        double *part = (double *)ctx->partials.get(gpu::kPartialBlocks * sizeof(double));
+        (void)part;
+        // Lanczos.hpp:104-110: glibc rand() start vector, normalised on the host
+        const long nr = cplx ? 2 * n : n;
+        std::vector<double> v(nr);
+        std::srand(seed);
+        for (long i = 0; i < nr; ++i) v[i] = (double)std::rand() / RAND_MAX;
+        double s = 0;
+        if (cplx) for (long i = 0; i < n; ++i) s += v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1];
+        else for (long i = 0; i < n; ++i) s += v[i] * v[i];
+        const double vs = 1.0 / std::sqrt(s);
+        for (double &x : v) x *= vs;
+        HIP_OK(hipMemcpyAsync(vjm1, v.data(), nr * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        std::vector<double> a(max_iter + 2, 0.0), b(max_iter + 2, 0.0);
+        apply_op(ctx, e, vjm1, n, nullptr, 0, vj, n, 1, 1.0, 0.0, 0.0);
+        a[0] = host_sum(ctx, gpu::dot_partials(e, vjm1, vj, n, ctx->partials.as<double>(), ctx->stream));
+        int nb = gpu::lanczos_update(e, vj, vjm1, vjp1, a[0], 0.0, n, ctx->partials.as<double>(), ctx->stream);
+        b[0] = std::sqrt(host_sum(ctx, nb));
+        if (b[0] == 0.0) fail(LYNX_E_UNSUPPORTED, "Lanczos start vector spans an invariant subspace");
+        gpu::scale_copy(e, vj, vj, 1.0 / b[0], n, ctx->stream);
+        double emin = 0, emax = 0, pmin = 0, pmax = 0, errmin = tol + 1, errmax = tol + 1;
+        int j = 0;
+        while ((errmin > tol || errmax > tol) && j < max_iter) {
+            apply_op(ctx, e, vj, n, nullptr, 0, vjp1, n, 1, 1.0, 0.0, 0.0);
+            a[j + 1] = host_sum(ctx, gpu::dot_partials(e, vj, vjp1, n, ctx->partials.as<double>(), ctx->stream));
+            nb = gpu::lanczos_update(e, vjp1, vj, vjm1, a[j + 1], b[j], n, ctx->partials.as<double>(), ctx->stream);
+            b[j + 1] = std::sqrt(host_sum(ctx, nb));
+            if (b[j + 1] == 0.0) break;
+            gpu::scale_copy(e, vjp1, vj, 1.0 / b[j + 1], n, ctx->stream);
+            const int m = j + 2;
+            emin = tridiag_extreme(a.data(), b.data(), m, false);
+            emax = tridiag_extreme(a.data(), b.data(), m, true);
+            errmin = std::fabs(emin - pmin);
+            errmax = std::fabs(emax - pmax);
+            pmin = emin;
+            pmax = emax;
+            ++j;
+        }
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        *eig_min = emin - 0.1;
+        *eig_max = emax * 1.01;
+        if (iters) *iters = j;
+    });
+}
+
+lynx_status lynx_fd_weights(int fd_order, double *w1, double *w2)
+{
+    try {
+        std::vector<double> a, b;
+        fd_weights(fd_order, a, b);
+        for (size_t i = 0; i < a.size(); ++i) {
+            if (w1) w1[i] = a[i];
+            if (w2) w2[i] = b[i];
+        }
+        return LYNX_OK;
+    } catch (const Error &e) {
+        return e.status;
+    }
+}
+
+lynx_status lynx_psp8_info(const char *path, double *zion, int *lmax, int *lloc, int *nproj, double *rc_max)
+{
+    if (!path) return LYNX_E_ARG;
+    try {
+        Psp8 p;
+        psp8_load(path, p);
+        if (zion) *zion = p.zion;
+        if (lmax) *lmax = p.lmax;
+        if (lloc) *lloc = p.lloc;
+        if (nproj) *nproj = p.nproj_per_atom();
+        if (rc_max) *rc_max = p.rc_max();
+        return LYNX_OK;
+    } catch (const Error &e) {
+        return e.status;
+    }
+}
+
+lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int launches[4])
+{
+    if (!ctx) return LYNX_E_ARG;
+    for (int k = 0; k < 4; ++k) {
+        if (times) times[k] = ctx->t_last[k];
+        if (launches) launches[k] = ctx->n_last[k];
+    }
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
+{
+    if (!ctx) return LYNX_E_ARG;
+    ctx->timing = enable != 0;
+    return LYNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Launch interface of the HIP kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace lynx {
+namespace gpu {
+
+// Element types of wavefunction blocks; values match enum lynx_dtype.
+enum Elem { F64 = 0, C128 = 1, F32 = 2, C64 = 3 };
+inline size_t elem_size(int e) { return e == F64 ? 8 : e == C128 ? 16 : e == F32 ? 4 : 8; }
+inline bool elem_complex(int e) { return e == C128 || e == C64; }
+inline bool elem_single(int e) { return e == F32 || e == C64; }
+
+constexpr int kMaxR = 6;   // stencil radius handled by the fused kernels (FD order 12)
+
+// Grid, stencil and Bloch wrap for the fused stencil kernel.
+struct StencilGeom {
+    int nx, ny, nz, R;
+    double c[3][kMaxR + 1];          // D2 coefficients per axis
+    double phase[3][2][2];           // [axis][0: e^{-i th}, 1: e^{+i th}][re, im]
+};
+
+// Launch shape of the fused stencil step; chosen per grid shape by the
+// tuner in the C ABI layer.
+struct StepCfg {
+    int rp_div = 1;    // points per thread = base / rp_div
+    int nzch = 1;      // number of z chunks per column
+};
+
+// out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)
+void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
+                  void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
+                  const StepCfg &cfg, hipStream_t st);
+int stencil_points_per_thread(int e, const StepCfg &cfg);
+
+// Nonlocal projector data in device layout.
+//   points: all (image, sphere point) pairs grouped by physical atom;
+//   chi_pm: point-major projector rows padded to npm entries;
+//   coverage: unique grid points covered by any sphere, each listing its
+//   entries (indices into points) for the owner-computes scatter.
+struct NlDev {
+    int n_atom = 0, npm = 0, n_pts = 0, n_cov = 0, n_img = 0;
+    const int *atom_pt_off = nullptr;   // n_atom+1
+    const int *pt_gpos = nullptr;       // n_pts
+    const int *pt_img = nullptr;        // n_pts
+    const int *pt_atom = nullptr;       // n_pts
+    const void *chi = nullptr;          // n_pts*npm (double for F64/C128, float otherwise)
+    const void *gamma = nullptr;        // n_atom*npm, dV*Gamma (same precision as chi)
+    const int *cov_off = nullptr;       // n_cov+1
+    const int *cov_gpos = nullptr;      // n_cov
+    const int *ent_pt = nullptr;        // entries
+    const void *phase = nullptr;        // n_img complex e^{-i k.R_img} (complex types only)
+};
+
+// beta[n][atom*npm + j] = dV*Gamma_j * sum_points chi_j * x (with Bloch phase)
+void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int ncol,
+               hipStream_t st);
+// y[g] += a * sum_entries chi . beta (conjugate Bloch phase)
+void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
+                hipStream_t st);
+
+// C (n x n, ld n) = alpha * A^H B for A, B (nd x n); split-K partials in work.
+size_t gram_work_bytes(int e, int nd, int n);
+void gram(int e, const void *A, long lda, const void *B, long ldb, int nd, int n, double alpha,
+          void *C, void *work, hipStream_t st);
+// C (nd x n) = A (nd x n) * Q (n x n, ld n)
+void rotate(int e, const void *A, long lda, const void *Q, int n, void *C, long ldc, int nd,
+            hipStream_t st);
+
+// Small dense helpers on n x n column-major matrices.
+void hermitian_part(int e, void *H, int n, hipStream_t st);
+void zero_lower(int e, void *H, int n, hipStream_t st);
+
+// Vector helpers (Lanczos).  Partial sums per block are written to
+// partials (one double per block); returns the number of blocks used.
+int dot_partials(int e, const void *x, const void *y, long n, double *partials, hipStream_t st);
+// y -= a*x + b*z; z <- x; partial sums of |y|^2
+int lanczos_update(int e, void *y, void *x, void *z, double a, double b, long n, double *partials,
+                   hipStream_t st);
+void scale_copy(int e, const void *src, void *dst, double s, long n, hipStream_t st);
+
+// Copies ncol columns between leading dimensions (device to device).
+void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, int ncol,
+               hipStream_t st);
+constexpr int kPartialBlocks = 512;
+
+}  // namespace gpu
+}  // namespace lynx

@@ -1,0 +1,121 @@
+"""Synthetic test systems shared by the tests, smoke() and bench.py.
+
+Cells follow the workloads the CheFSI path is measured on: diamond Si (C.pot
+stands in for the absent Si psp8: same zion 4, lmax 1, lloc 4, two s and two
+p projectors), displaced supercells, Gaussian-well effective potentials.
+Random numbers come from numpy's PCG64 (seeded) unless a glibc rand() stream
+is asked for.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PSP_DIR = os.path.join(ROOT, "tests", "golden", "psp")
+
+DIAMOND = np.array([[0.00, 0.00, 0.00], [0.25, 0.25, 0.25], [0.50, 0.50, 0.00], [0.75, 0.75, 0.25],
+                    [0.50, 0.00, 0.50], [0.75, 0.25, 0.75], [0.00, 0.50, 0.50], [0.25, 0.75, 0.75]])
+A_SI = 10.26
+
+
+def psp(name="C.pot"):
+    return os.path.join(PSP_DIR, name)
+
+
+def supercell(n, base=DIAMOND):
+    """n x n x n supercell of the 8-atom cubic cell, fractional coordinates."""
+    out = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                out.append((base + np.array([i, j, k])) / n)
+    return np.concatenate(out)
+
+
+def displace(frac, a, sigma, seed):
+    """Gaussian displacement sigma (Bohr) per Cartesian component."""
+    rng = np.random.default_rng(seed)
+    return frac + rng.normal(0.0, sigma, frac.shape) / a
+
+
+def gaussian_veff(n, a, frac, depth=1.2, width=1.6, offset=0.3):
+    """Effective potential with a Gaussian well at every atom (periodic
+    minimum image), a stand-in for the atomic-superposition Veff."""
+    nx, ny, nz = n
+    h = np.array([a / nx, a / ny, a / nz])
+    x = np.arange(nx) * h[0]
+    y = np.arange(ny) * h[1]
+    z = np.arange(nz) * h[2]
+    V = np.full((nz, ny, nx), offset)
+    for f in np.asarray(frac) * a:
+        dx = (x - f[0] + a / 2) % a - a / 2
+        dy = (y - f[1] + a / 2) % a - a / 2
+        dz = (z - f[2] + a / 2) % a - a / 2
+        r2 = dz[:, None, None] ** 2 + dy[None, :, None] ** 2 + dx[None, None, :] ** 2
+        V -= depth * np.exp(-r2 / (2 * width ** 2))
+    return V.ravel()
+
+
+class System:
+    """One cubic cell: grid, atoms, Veff, and both the oracle operator and
+    (optionally) the GPU Hamiltonian built from the same inputs."""
+
+    def __init__(self, n, a, frac, psp_name="C.pot", order=12, with_nonlocal=True, kpt=(0.0, 0.0, 0.0)):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.n = tuple(n)
+        self.a = a
+        self.frac = np.asarray(frac, float)
+        self.order = order
+        self.latvec = np.diag([a, a, a]).astype(float)
+        self.psp_files = [psp(psp_name)]
+        self.types = np.zeros(len(self.frac), np.int32)
+        self.with_nonlocal = with_nonlocal and len(self.frac) > 0
+        self.veff = gaussian_veff(self.n, a, self.frac)
+        self.nd = int(np.prod(self.n))
+        self.kpt = tuple(kpt)
+
+    # -- oracle ---------------------------------------------------------------
+    def oracle_op(self):
+        import oracle
+        lat = oracle.Lattice(self.latvec, True)
+        g = oracle.Grid(*self.n, lat)
+        if self.with_nonlocal:
+            ps = [oracle.load_psp8(f) for f in self.psp_files]
+            return oracle.setup_system(g, ps, list(self.types), self.frac, self.order, self.kpt)
+        return oracle.Operator(g, self.order, kpt=self.kpt)
+
+    # -- GPU library -------------------------------------------------------------
+    def hamiltonian(self, device_id=0):
+        from lynx_amd import core
+        lat = core.Lattice(self.latvec, core.CellType.Orthogonal)
+        grid = core.FDGrid(*self.n, lat)
+        st = core.FDStencil(self.order, grid, lat)
+        dom = core.Domain(grid)
+        halo = core.HaloExchange(dom, st.FDn())
+        vnl = None
+        if self.with_nonlocal:
+            vnl = core.NonlocalProjector().setup_atoms(self.psp_files, self.types, self.frac, True)
+        H = core.Hamiltonian(device_id).setup(st, dom, grid, halo, vnl)
+        if any(abs(k) > 0 for k in self.kpt):
+            H.set_kpoint(self.kpt)
+        self.core = (lat, grid, st, dom, halo)
+        return H
+
+    def context(self, nstates):
+        from lynx_amd import core
+        lat, grid, st, dom, halo = self.core
+        return core.LynxContext(lat, grid, st, dom, halo, nstates)
+
+    def random_block(self, ncol, seed, complex_=False):
+        rng = np.random.default_rng(seed)
+        X = rng.uniform(-0.5, 0.5, (self.nd, ncol))
+        if complex_:
+            X = X + 1j * rng.uniform(-0.5, 0.5, (self.nd, ncol))
+        return np.asfortranarray(X)
+
+
+def si8(n=30, **kw):
+    return System(n, A_SI, DIAMOND, **kw)

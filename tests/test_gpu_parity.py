@@ -1,0 +1,260 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs).
+
+Bars: FP64 results agree with the oracle to 1e-11 relative (stencil, nonlocal,
+filter) and 1e-9 (Ritz values of a whole solve, Lanczos bounds); FP32 to 2e-5
+relative.  Every call goes through the C ABI of liblynx_chefsi.so.
+"""
+import numpy as np
+import pytest
+
+from systems import DIAMOND, A_SI, System, displace, si8
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def si8_24():
+    s = si8(24)
+    return s, s.oracle_op(), s.hamiltonian()
+
+
+def test_h_apply_local_only_real():
+    s = si8(20, with_nonlocal=False)
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(5, 1)
+    for c in (0.0, -0.37):
+        ref = op.apply(X, s.veff, c=c)
+        got = H.apply(H.halo, X.T.copy().ravel(), s.veff, c).reshape(5, -1).T
+        assert rel(got, ref) < 1e-12, c
+
+
+def test_h_apply_nonlocal_real(si8_24):
+    s, op, H = si8_24
+    X = s.random_block(6, 2)
+    ref = op.apply(X, s.veff)
+    got = H.apply(H.halo, X.T.copy(), s.veff).reshape(6, -1).T
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [(18, 20, 24), (60, 60, 20), (26, 90, 12)])
+def test_h_apply_bands_and_shapes(n):
+    # shapes that force several row bands and partial last bands
+    s = System(n, A_SI, DIAMOND)
+    s.latvec = np.diag([A_SI * n[0] / 24, A_SI * n[1] / 24, A_SI * n[2] / 24])
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(3, 3)
+    ref = op.apply(X, s.veff, c=0.25)
+    got = H.apply(H.halo, X.T.copy(), s.veff, 0.25).reshape(3, -1).T
+    assert rel(got, ref) < 1e-12
+
+
+def test_h_apply_kpoint():
+    k = (0.25 * 2 * np.pi / A_SI, -0.125 * 2 * np.pi / A_SI, 0.5 * 2 * np.pi / A_SI)
+    s = si8(20, kpt=k)
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(4, 4, complex_=True)
+    ref = op.apply(X, s.veff, c=0.1)
+    got = H.apply_kpt(H.halo, X.T.copy(), s.veff, k, 0.1).reshape(4, -1).T
+    assert rel(got, ref) < 1e-12
+    # Hermiticity of the Bloch Hamiltonian
+    HX = got
+    m = X.conj().T @ HX
+    assert np.abs(m - m.conj().T).max() < 1e-9 * np.abs(m).max()
+
+
+def test_h_apply_fp32(si8_24):
+    s, op, H = si8_24
+    X = s.random_block(4, 5)
+    ref = op.apply(X, s.veff)
+    got = H._apply(X.T.copy(), s.veff, 0.0, np.float32).reshape(4, -1).T
+    assert rel(got, ref) < 2e-6
+
+
+def test_filter_real(si8_24):
+    s, op, H = si8_24
+    from lynx_amd import _lib as L
+    X = s.random_block(8, 6)
+    lo, hi = -0.4, 45.0
+    lc = 0.6
+    for deg in (1, 2, 7, 20):
+        ref = op.cheb_filter(X, s.veff, lc, lo, hi, deg)
+        H.set_veff(s.veff)
+        Y = np.zeros_like(X)
+        L.check(L.lib().lynx_cheb_filter(H.ctx, L.F64, L.vptr(X), X.shape[0], L.vptr(Y), X.shape[0], 8, deg,
+                                         lc, lo, hi, L.MEM_HOST), H.ctx)
+        assert rel(Y, ref) < 1e-11, deg
+
+
+def test_filter_complex_kpoint():
+    k = (0.5 * 2 * np.pi / A_SI, 0.0, 0.25 * 2 * np.pi / A_SI)
+    s = si8(20, kpt=k)
+    op, H = s.oracle_op(), s.hamiltonian()
+    from lynx_amd import _lib as L
+    X = s.random_block(4, 7, complex_=True)
+    ref = op.cheb_filter(X, s.veff, 0.5, -0.5, 40.0, 12)
+    H.set_veff(s.veff)
+    Y = np.zeros_like(X)
+    L.check(L.lib().lynx_cheb_filter(H.ctx, L.C128, L.vptr(X), X.shape[0], L.vptr(Y), X.shape[0], 4, 12,
+                                     0.5, -0.5, 40.0, L.MEM_HOST), H.ctx)
+    assert rel(Y, ref) < 1e-11
+
+
+def test_filter_fp32(si8_24):
+    s, op, H = si8_24
+    from lynx_amd import _lib as L
+    X = s.random_block(4, 8)
+    ref = op.cheb_filter(X, s.veff, 0.6, -0.4, 45.0, 10)
+    H.set_veff(s.veff)
+    X32 = np.asfortranarray(X, np.float32)
+    Y = np.zeros_like(X32)
+    L.check(L.lib().lynx_cheb_filter(H.ctx, L.F32, L.vptr(X32), X.shape[0], L.vptr(Y), X.shape[0], 4, 10,
+                                     0.6, -0.4, 45.0, L.MEM_HOST), H.ctx)
+    assert rel(Y, ref) < 2e-5
+
+
+def _subspace_dist(A, B, dV):
+    # distance between the spans: || P_A - P_B || via orthonormal bases
+    qa, _ = np.linalg.qr(A)
+    qb, _ = np.linalg.qr(B)
+    s = np.linalg.svd(qa.conj().T @ qb, compute_uv=False)
+    return float(np.sqrt(max(0.0, 1.0 - s.min() ** 2)))
+
+
+def test_solve_real(si8_24):
+    s, op, H = si8_24
+    from lynx_amd import core
+    es = core.EigenSolver().setup(s.context(16), H)
+    X = s.random_block(16, 9)
+    lo, hi, _ = op.lanczos_bounds(s.veff)
+    lc = 0.5 * (lo + hi)
+    ref_psi, ref_eig, ref_lc = op.solve(X, s.veff, lc, lo, hi, 12)
+    psi, eig = es.solve(X, s.veff, lc, lo, hi, 12)
+    assert np.abs(eig - ref_eig).max() < 1e-9 * max(1.0, np.abs(ref_eig).max())
+    assert abs(es.lambda_cutoff() - ref_lc) < 1e-9
+    # orthonormality in the dV metric and same subspace
+    dV = s.a ** 3 / s.nd
+    G = dV * psi.T @ psi
+    assert np.abs(G - np.eye(16)).max() < 1e-10
+    assert _subspace_dist(psi, ref_psi, dV) < 1e-7
+
+
+def test_solve_repeated_converges(si8_24):
+    # repeated CheFSI passes converge; eigenvalues equal the oracle's pass by pass
+    s, op, H = si8_24
+    from lynx_amd import core
+    es = core.EigenSolver().setup(s.context(12), H)
+    X = s.random_block(12, 10)
+    lo, hi, _ = op.lanczos_bounds(s.veff)
+    lc = 0.5 * (lo + hi)
+    psi, rpsi = X, X
+    lcg = lcr = lc
+    for it in range(4):
+        psi, eig = es.solve(psi, s.veff, lcg, lo, hi, 15)
+        rpsi, reig, lcr = op.solve(rpsi, s.veff, lcr, lo, hi, 15)
+        lcg = es.lambda_cutoff()
+        assert np.abs(eig - reig).max() < 1e-8 * max(1.0, np.abs(reig).max()), it
+
+
+def test_solve_kpoint():
+    k = (0.25 * 2 * np.pi / A_SI, 0.25 * 2 * np.pi / A_SI, 0.0)
+    s = si8(20, kpt=k)
+    op, H = s.oracle_op(), s.hamiltonian()
+    from lynx_amd import core
+    es = core.EigenSolver().setup(s.context(10), H)
+    X = s.random_block(10, 11, complex_=True)
+    lo, hi, _ = op.lanczos_bounds(s.veff)
+    lc = 0.5 * (lo + hi)
+    ref_psi, ref_eig, _ = op.solve(X, s.veff, lc, lo, hi, 10)
+    psi, eig = es.solve_kpt(X, s.veff, lc, lo, hi, k, cheb_degree=10)
+    assert np.abs(eig - ref_eig).max() < 1e-9 * max(1.0, np.abs(ref_eig).max())
+
+
+def test_lanczos_real(si8_24):
+    s, op, H = si8_24
+    from lynx_amd import core
+    es = core.EigenSolver().setup(s.context(8), H)
+    mn, mx = es.lanczos_bounds(s.veff)
+    rmn, rmx, rit = op.lanczos_bounds(s.veff)
+    assert abs(mn - rmn) < 1e-8 * max(1, abs(rmn))
+    assert abs(mx - rmx) < 1e-8 * max(1, abs(rmx))
+    assert es.lanczos_iters == rit
+
+
+def test_lanczos_kpoint():
+    k = (0.5 * 2 * np.pi / A_SI, 0.0, 0.0)
+    s = si8(20, kpt=k)
+    op, H = s.oracle_op(), s.hamiltonian()
+    from lynx_amd import core
+    es = core.EigenSolver().setup(s.context(8), H)
+    mn, mx = es.lanczos_bounds_kpt(s.veff, k)
+    rmn, rmx, _ = op.lanczos_bounds(s.veff)
+    assert abs(mn - rmn) < 1e-8 * max(1, abs(rmn))
+    assert abs(mx - rmx) < 1e-8 * max(1, abs(rmx))
+
+
+def test_displaced_supercell_apply():
+    # Si64-shaped 2x2x2 cell with displaced atoms on a coarse grid
+    from systems import supercell
+    frac = displace(supercell(2), 2 * A_SI, 0.05, seed=1)
+    s = System(32, 2 * A_SI, frac)
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(4, 12)
+    ref = op.apply(X, s.veff)
+    got = H.apply(H.halo, X.T.copy(), s.veff).reshape(4, -1).T
+    assert rel(got, ref) < 1e-12
+
+
+def test_device_pointers_and_leading_dimension():
+    from lynx_amd import _lib as L
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    s = si8(20)
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(3, 13)
+    ref = op.apply(X, s.veff, c=0.2)
+    nd = s.nd
+    ld = nd + 37
+    H.set_veff(s.veff)
+    hx = np.zeros((3, ld))
+    hx[:, :nd] = X.T
+    dx = DeviceArray.from_numpy(hx)
+    dy = DeviceArray.from_numpy(np.full((3, ld), 7.0))
+    L.check(L.lib().lynx_h_apply(H.ctx, L.F64, dx.value, ld, dy.value, ld, 3, 0.2, L.MEM_DEVICE), H.ctx)
+    synchronize()
+    out = dy.download(np.zeros((3, ld)))
+    assert rel(out[:, :nd].T, ref) < 1e-12
+    assert np.all(out[:, nd:] == 7.0)    # padding untouched
+    # filter and solve on device pointers with padded leading dimension
+    Yd = DeviceArray.from_numpy(np.zeros((3, ld)))
+    L.check(L.lib().lynx_cheb_filter(H.ctx, L.F64, dx.value, ld, Yd.value, ld, 3, 9, 0.6, -0.4, 45.0,
+                                     L.MEM_DEVICE), H.ctx)
+    ref_f = op.cheb_filter(X, s.veff, 0.6, -0.4, 45.0, 9)
+    out = Yd.download(np.zeros((3, ld)))
+    assert rel(out[:, :nd].T, ref_f) < 1e-11
+
+
+def test_errors_are_reported():
+    from lynx_amd import _lib as L
+    s = si8(20)
+    H = s.hamiltonian()
+    X = s.random_block(2, 14)
+    Y = np.zeros_like(X)
+    # no Veff yet
+    st = L.lib().lynx_h_apply(H.ctx, L.F64, L.vptr(X), s.nd, L.vptr(Y), s.nd, 2, 0.0, L.MEM_HOST)
+    assert st == L.E_STATE
+    H.set_veff(s.veff)
+    # leading dimension too small
+    st = L.lib().lynx_h_apply(H.ctx, L.F64, L.vptr(X), s.nd - 1, L.vptr(Y), s.nd, 2, 0.0, L.MEM_HOST)
+    assert st == L.E_ARG
+    # non-SPD overlap: duplicated column -> Cholesky failure status
+    Z = np.asfortranarray(np.concatenate([X, X[:, :1]], axis=1))
+    st = L.lib().lynx_orthogonalize(H.ctx, L.F64, L.vptr(Z), s.nd, 3, L.MEM_HOST)
+    assert st == L.E_CHOLESKY
+    # non-finite potential
+    v = s.veff.copy()
+    v[5] = np.nan
+    st = L.lib().lynx_set_veff(H.ctx, L.dptr(v), L.MEM_HOST)
+    assert st == L.E_NONFINITE
