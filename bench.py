@@ -81,6 +81,10 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
     synchronize()
     elapsed = time.perf_counter() - t0
     L.check(lib.lynx_set_timing(ctx, 0), ctx)
+    pd = ctypes.c_int(); zc = ctypes.c_int(); nt = ctypes.c_int(16)
+    tms = (ctypes.c_double * 16)()
+    L.check(lib.lynx_get_stencil_config(ctx, dtype_code, NCOL, ctypes.byref(pd), ctypes.byref(zc), tms,
+                                        ctypes.byref(nt)), ctx)
     Y = np.zeros((NCOL, nd), dtype=Xh.dtype)
     Yd.download(Y)
     finite = bool(np.all(np.isfinite(Y)))
@@ -100,6 +104,8 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
         scatter_ms_per_step=kern_ms[3] / steps,
         stencil_gbs=B / (stencil_ms * 1e-3) / 1e9 if stencil_ms > 0 else None,
         lanczos=(lo.value, hi.value, it.value),
+        stencil_config={"points_div": pd.value, "z_chunks": zc.value,
+                        "tuner_ms": [round(tms[i], 4) for i in range(nt.value)]},
         finite=finite,
     )
     Xd.free(); Yd.free()
@@ -152,7 +158,8 @@ def main():
         variants["f32"] = {"value": round(r32["gbs"], 2), "ms_per_step": round(r32["ms_per_step"], 4),
                            "stencil_avg_launch_ms": round(r32["stencil_avg_launch_ms"], 5),
                            "roofline_frac": round(r32["stencil_gbs"] / HBM_PEAK_GBS, 4),
-                           "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3)}
+                           "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
+                           "launch_shape": r32["stencil_config"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -188,6 +195,7 @@ def main():
             "traffic": None,
             "kernel": "k_stencil_step<double> (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
+            "launch_shape": r64["stencil_config"],
         },
         "cpu_baseline": cpu,
         "variants": variants,

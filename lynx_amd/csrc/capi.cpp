@@ -97,7 +97,13 @@ struct lynx_ctx {
     DevBuf s1, s2, gwork, dvals, evals, info, partials;
     std::vector<double> hpart;
 
-    gpu::StepCfg cfg{};
+    // stencil launch shape: forced by lynx_set_stencil_config, else tuned
+    // per (grid, dtype, ncol) on first use and cached
+    gpu::StepCfg forced{};
+    bool has_forced = false;
+    bool autotune = true;
+    std::map<std::vector<long>, gpu::StepCfg> tuned;
+    std::map<std::vector<long>, std::vector<double>> tune_log;
 
     bool timing = false;
     double t_last[4] = {0, 0, 0, 0};
@@ -350,11 +356,67 @@ void upload_phases(lynx_ctx *c)
     c->nl32.phase = c->d_ph32.p;
 }
 
+// ------------------------------------------------------------ autotuner --
+// Candidate launch shapes of the fused stencil step: points per thread
+// (register ring depth x points) and number of z chunks per column.  Each is
+// timed on the caller's own data (plain H application into the destination
+// block, which the real step overwrites afterwards); the fastest is cached
+// for this grid shape, element type and column count.
+std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c)
+{
+    std::vector<gpu::StepCfg> v;
+    for (int div : {1, 2})
+        for (int nz : {1, 2, 3, 4}) {
+            if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
+            gpu::StepCfg k;
+            k.rp_div = div;
+            k.nzch = nz;
+            v.push_back(k);
+        }
+    return v;
+}
+
+const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, void *y, long ldy, int ncol)
+{
+    if (c->has_forced) return c->forced;
+    std::vector<long> key = {c->st.nx, c->st.ny, c->st.nz, e, ncol};
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) return it->second;
+    gpu::StepCfg best{};
+    if (c->autotune) {
+        const auto cands = step_candidates(c);
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        double tbest = 1e30;
+        std::vector<double> times;
+        for (const auto &k : cands) {
+            float ms = 0, tot = 0;
+            for (int rep = 0; rep < 3; ++rep) {
+                HIP_OK(hipEventRecord(e0, c->stream));
+                gpu::stencil_step(e, geom(c), x, ldx, nullptr, 0, y, ldy, veff_for(c, e), ncol, 1.0, 0.0, 0.0, k,
+                                  c->stream);
+                HIP_OK(hipEventRecord(e1, c->stream));
+                HIP_OK(hipEventSynchronize(e1));
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) tot += ms;
+            }
+            times.push_back(tot / 2);
+            if (tot / 2 < tbest) { tbest = tot / 2; best = k; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->tune_log[key] = times;
+    }
+    return c->tuned[key] = best;
+}
+
 // ------------------------------------------------------------- operator --
 // y = a * (H - shift) x + b * old, H = -1/2 lap + Veff + V_nl.
 void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long ldo, void *y, long ldy,
               int ncol, double a, double b, double shift)
 {
+    const gpu::StepCfg cfg = step_config(c, e, x, ldx, y, ldy, ncol);
     const bool nl = c->has_nl;
     const gpu::NlDev &nd = nl_for(c, e);
     if (nl) {
@@ -364,7 +426,7 @@ void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long
     }
     {
         Marker m(c, 1);
-        gpu::stencil_step(e, geom(c), x, ldx, old, ldo, y, ldy, veff_for(c, e), ncol, a, b, shift, c->cfg,
+        gpu::stencil_step(e, geom(c), x, ldx, old, ldo, y, ldy, veff_for(c, e), ncol, a, b, shift, cfg,
                           c->stream);
     }
     if (nl) {
@@ -1050,6 +1112,44 @@ lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int la
     for (int k = 0; k < 4; ++k) {
         if (times) times[k] = ctx->t_last[k];
         if (launches) launches[k] = ctx->n_last[k];
+    }
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int points_div, int z_chunks)
+{
+    if (!ctx) return LYNX_E_ARG;
+    if (points_div == 0 && z_chunks == 0) {
+        ctx->has_forced = false;
+        return LYNX_OK;
+    }
+    if ((points_div != 1 && points_div != 2) || z_chunks < 1) return LYNX_E_ARG;
+    ctx->forced.rp_div = points_div;
+    ctx->forced.nzch = z_chunks;
+    ctx->has_forced = true;
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *points_div, int *z_chunks,
+                                    double *tune_ms, int *n_tune)
+{
+    if (!ctx) return LYNX_E_ARG;
+    const gpu::StepCfg *k = nullptr;
+    std::vector<long> key = {ctx->st.nx, ctx->st.ny, ctx->st.nz, dtype, ncol};
+    if (ctx->has_forced) k = &ctx->forced;
+    else {
+        auto it = ctx->tuned.find(key);
+        if (it == ctx->tuned.end()) return LYNX_E_STATE;
+        k = &it->second;
+    }
+    if (points_div) *points_div = k->rp_div;
+    if (z_chunks) *z_chunks = k->nzch;
+    if (n_tune) {
+        auto it = ctx->tune_log.find(key);
+        const int n = it == ctx->tune_log.end() ? 0 : (int)it->second.size();
+        if (tune_ms)
+            for (int i = 0; i < std::min(n, *n_tune); ++i) tune_ms[i] = it->second[i];
+        *n_tune = n;
     }
     return LYNX_OK;
 }

@@ -115,7 +115,16 @@ struct StepArgs {
     A ph[3][2][2];
 };
 
-template <class S, int R, int RP, int NT>
+// Loads of one z-step, issued one step ahead of their use.
+template <class S, int RP, int HP>
+struct StepLoads {
+    S front[RP];                 // plane z+R of the owned points (ring input)
+    S old[RP];                   // Y_{k-2} at plane z
+    typename Tr<S>::A v[RP];     // Veff at plane z
+    S halo[HP];                  // halo-row values of plane z
+};
+
+template <class S, int R, int RP, int NT, int HP, int W>
 __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A> p)
 {
     using A = typename Tr<S>::A;
@@ -124,9 +133,13 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     extern __shared__ __align__(16) unsigned char smem_raw[];
     S *lds = reinterpret_cast<S *>(smem_raw);
 
+    // W: LDS row pitch, a compile-time constant >= nx + 2R so that every
+    // neighbour read is one base address plus an immediate offset
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const int W = nx + 2 * R;
-    const int plane = W * (p.tyr + 2 * R);
+    // each LDS plane buffer ends with one dummy slot that absorbs the writes
+    // of idle lanes and of elements without a ghost copy (no branches)
+    const int dummy = W * (p.tyr + 2 * R);
+    const int plane = dummy + 1;
 
     const int per = gridDim.x >> 3;
     const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -139,7 +152,7 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     const int rows = min(p.tyr, ny - y0);
     const int z0 = zc * p.zch;
     const int z1 = min(z0 + p.zch, nz);
-    const long nxny = (long)nx * ny;
+    const int nxny = nx * ny;
 
     const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
     const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
@@ -148,33 +161,86 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
 
     const int tid = threadIdx.x;
     const int npts = rows * nx;
-    int qx[RP], qy[RP], qoff[RP];
+    // owned points: plane offset (0 for idle lanes), LDS index and the LDS
+    // indices of its wrapped x-ghost copies (dummy when none)
+    unsigned qoff[RP];
+    int qli[RP], qgh[RP], qgl[RP];
     bool qv[RP];
 #pragma unroll
     for (int r = 0; r < RP; ++r) {
         const int q = tid + r * NT;
         qv[r] = q < npts;
         const int qq = qv[r] ? q : 0;
-        qy[r] = qq / nx;
-        qx[r] = qq - qy[r] * nx;
-        qoff[r] = (y0 + qy[r]) * nx + qx[r];
+        const int qy = qq / nx;
+        const int qx = qq - qy * nx;
+        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
+        const int li = (qy + R) * W + qx + R;
+        qli[r] = qv[r] ? li : dummy;
+        qgh[r] = (qv[r] && qx < R) ? li + nx : dummy;
+        qgl[r] = (qv[r] && qx >= nx - R) ? li - nx : dummy;
+    }
+    // halo slots: R rows below and R above the band, wrapped in y
+    const int nh = 2 * R * nx;
+    unsigned hsrc[HP];
+    int hli[HP], hgh[HP], hgl[HP], hys[HP];
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT;
+        const bool v = t < nh;
+        const int tt = v ? t : 0;
+        const int i = tt / nx;
+        const int xx = tt - i * nx;
+        const int yb = (i < R) ? (i - R) : (rows + i - R);
+        int yg = y0 + yb, ys = 0;
+        if (yg < 0) { yg += ny; ys = 1; }
+        else if (yg >= ny) { yg -= ny; ys = 2; }
+        hsrc[h] = v ? yg * nx + xx : 0;
+        const int li = (yb + R) * W + xx + R;
+        hli[h] = v ? li : dummy;
+        hgh[h] = (v && xx < R) ? li + nx : dummy;
+        hgl[h] = (v && xx >= nx - R) ? li - nx : dummy;
+        hys[h] = ys;
     }
 
-    S ring[RP][NR];
     auto load_plane = [&](int zz, S *dst) {
         int side = -1;
         if (zz < 0) { zz += nz; side = 0; }
         else if (zz >= nz) { zz -= nz; side = 1; }
-        const S *src = X + (long)zz * nxny;
+        const S *src = X + zz * nxny;
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            S v = qv[r] ? src[qoff[r]] : zero<S>();
+            S v = src[qoff[r]];
             if constexpr (C) {
                 if (side >= 0) v = phased(v, p.ph[2][side][0], p.ph[2][side][1]);
             }
             dst[r] = v;
         }
     };
+    // every load of plane zs (clamped into the chunk, so the look-ahead past
+    // the last plane stays in bounds)
+    auto issue = [&](int zs, StepLoads<S, RP, HP> &L) {
+        zs = min(zs, z1 - 1);
+        load_plane(zs + R, L.front);
+        const int zo = zs * nxny;
+        const S *Oz = O + zo;
+        const A *Vz = p.veff + zo;
+        const S *Xz = X + zo;
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            L.old[r] = has_old ? Oz[qoff[r]] : zero<S>();
+            L.v[r] = Vz[qoff[r]];
+        }
+#pragma unroll
+        for (int h = 0; h < HP; ++h) L.halo[h] = Xz[hsrc[h]];
+    };
+    // an element and its wrapped x-ghost copies into the LDS plane
+    auto put = [&](S *L, int li, int gh, int gl, S v) {
+        L[li] = v;
+        L[gh] = phased(v, p.ph[0][1][0], p.ph[0][1][1]);
+        L[gl] = phased(v, p.ph[0][0][0], p.ph[0][0][1]);
+    };
+
+    S ring[RP][NR];
 #pragma unroll
     for (int i = 0; i < 2 * R; ++i) {
         S tmp[RP];
@@ -182,78 +248,60 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
 #pragma unroll
         for (int r = 0; r < RP; ++r) ring[r][i] = tmp[r];
     }
-    S front[RP];
-    load_plane(z0 + R, front);
 
     const A diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
     int buf = 0;
-    for (int z = z0; z < z1; ++z) {
-#pragma unroll
-        for (int r = 0; r < RP; ++r) ring[r][2 * R] = front[r];
-        if (z + 1 < z1) load_plane(z + 1 + R, front);
-        const long zoff = (long)z * nxny;
-        S o[RP];
-        A vv[RP];
-#pragma unroll
-        for (int r = 0; r < RP; ++r) {
-            o[r] = (has_old && qv[r]) ? O[zoff + qoff[r]] : zero<S>();
-            vv[r] = qv[r] ? p.veff[zoff + qoff[r]] : A(0);
-        }
+    // one z-step: prefetch the next step first, then consume this one
+    auto step = [&](int z, StepLoads<S, RP, HP> &cur, StepLoads<S, RP, HP> &nxt) {
+        issue(z + 1, nxt);
         S *L = lds + buf * plane;
-        // own points of plane z, with wrapped x-ghost copies
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            if (qv[r]) {
-                const int li = (qy[r] + R) * W + qx[r] + R;
-                const S v = ring[r][R];
-                L[li] = v;
-                if (qx[r] < R) L[li + nx] = phased(v, p.ph[0][1][0], p.ph[0][1][1]);
-                if (qx[r] >= nx - R) L[li - nx] = phased(v, p.ph[0][0][0], p.ph[0][0][1]);
-            }
+            ring[r][2 * R] = cur.front[r];
+            put(L, qli[r], qgh[r], qgl[r], ring[r][R]);
         }
-        // R halo rows below and above the band (wrapped in y)
-        const int nh = 2 * R * nx;
-        for (int t = tid; t < nh; t += NT) {
-            const int i = t / nx;
-            const int xx = t - i * nx;
-            const int yb = (i < R) ? (i - R) : (rows + i - R);
-            int yg = y0 + yb;
-            int side = -1;
-            if (yg < 0) { yg += ny; side = 0; }
-            else if (yg >= ny) { yg -= ny; side = 1; }
-            S v = X[zoff + (long)yg * nx + xx];
+#pragma unroll
+        for (int h = 0; h < HP; ++h) {
+            S v = cur.halo[h];
             if constexpr (C) {
-                if (side >= 0) v = phased(v, p.ph[1][side][0], p.ph[1][side][1]);
+                const A pr = hys[h] == 1 ? p.ph[1][0][0] : (hys[h] == 2 ? p.ph[1][1][0] : A(1));
+                const A pi = hys[h] == 1 ? p.ph[1][0][1] : (hys[h] == 2 ? p.ph[1][1][1] : A(0));
+                v = cmul(v, pr, pi);
             }
-            const int li = (yb + R) * W + xx + R;
-            L[li] = v;
-            if (xx < R) L[li + nx] = phased(v, p.ph[0][1][0], p.ph[0][1][1]);
-            if (xx >= nx - R) L[li - nx] = phased(v, p.ph[0][0][0], p.ph[0][0][1]);
+            put(L, hli[h], hgh[h], hgl[h], v);
         }
         __syncthreads();
+        S *Yz = Y + z * nxny;
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            if (qv[r]) {
-                const S *c = L + (qy[r] + R) * W + qx[r] + R;
-                S lap = scl(diag, ring[r][R]);
+            const S *c = L + qli[r];
+            S lap = scl(diag, ring[r][R]);
 #pragma unroll
-                for (int k = 1; k <= R; ++k) {
-                    lap = fm(p.c[0][k], add(c[k], c[-k]), lap);
-                    lap = fm(p.c[1][k], add(c[k * W], c[-k * W]), lap);
-                    lap = fm(p.c[2][k], add(ring[r][R + k], ring[r][R - k]), lap);
-                }
-                const S h = fm(vv[r] + p.s, ring[r][R], scl(A(-0.5), lap));
-                S res = scl(p.a, h);
-                if (has_old) res = fm(p.b, o[r], res);
-                Y[zoff + qoff[r]] = res;
+            for (int k = 1; k <= R; ++k) {
+                lap = fm(p.c[0][k], add(c[k], c[-k]), lap);
+                lap = fm(p.c[1][k], add(c[k * W], c[-k * W]), lap);
+                lap = fm(p.c[2][k], add(ring[r][R + k], ring[r][R - k]), lap);
             }
+            const S hval = fm(cur.v[r] + p.s, ring[r][R], scl(A(-0.5), lap));
+            S res = scl(p.a, hval);
+            if (has_old) res = fm(p.b, cur.old[r], res);
+            if (qv[r]) Yz[qoff[r]] = res;
         }
 #pragma unroll
         for (int r = 0; r < RP; ++r)
 #pragma unroll
             for (int i = 0; i < 2 * R; ++i) ring[r][i] = ring[r][i + 1];
         buf ^= 1;
+    };
+
+    StepLoads<S, RP, HP> la, lb;
+    issue(z0, la);
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        step(z, la, lb);
+        step(z + 1, lb, la);
     }
+    if (z < z1) step(z, la, lb);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -571,6 +619,7 @@ void dispatch(int e, F &&f)
 }
 
 template <class S> constexpr int base_rp() { return int(32 / sizeof(S)); }
+constexpr int kHaloSlots = 6;   // halo elements per thread: rows up to 128 points
 
 template <class S, int RP>
 void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
@@ -584,7 +633,8 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
     const long pts = (long)NT * RP;
-    if (g.nx > pts) fail(8, "grid rows longer than one workgroup's points are not supported");
+    if (g.nx > pts || 2 * kMaxR * g.nx > NT * kHaloSlots || g.nx + 2 * kMaxR > 128)
+        fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
     p.tyr = (g.ny + nb - 1) / nb;
@@ -601,8 +651,16 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
         for (int sd = 0; sd < 2; ++sd)
             for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (A)g.phase[ax][sd][q];
     const int blocks = ((p.nwork + 7) / 8) * 8;
-    const size_t shm = 2 * sizeof(S) * (size_t)(g.nx + 2 * kMaxR) * (p.tyr + 2 * kMaxR);
-    hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT>), dim3(blocks), dim3(NT), shm, st, p);
+    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + 1);
+    const bool small_halo = 2 * kMaxR * g.nx <= NT * 3;
+    if (pitch == 64) {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, 3, 64>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, kHaloSlots, 64>), dim3(blocks), dim3(NT), shm, st, p);
+    } else {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, 3, 128>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, kHaloSlots, 128>), dim3(blocks), dim3(NT), shm, st, p);
+    }
 }
 
 }  // namespace

@@ -1,0 +1,146 @@
+// Thin C ABI over a subset of the reference sources, compiled from
+// /root/reference by oracle/ref_build.py into oracle/_ref/libref.so.
+// TEST INFRASTRUCTURE ONLY: used by tests/test_ref_subset.py to validate the
+// oracle's restatement (and the library's host set-up) against the
+// reference's own code for the leaf steps that build without MPI, LAPACK or
+// the rest of the program: FD stencil, lattice/grid, halo wrap, psp8 reader
+// and splines, k-points.
+#include "core/Domain.hpp"
+#include "core/FDGrid.hpp"
+#include "core/KPoints.hpp"
+#include "core/Lattice.hpp"
+#include "core/NumericalMethods.hpp"
+#include "atoms/Pseudopotential.hpp"
+#include "operators/FDStencil.hpp"
+#include "parallel/HaloExchange.hpp"
+
+#include <complex>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+using namespace lynx;
+
+namespace {
+Lattice make_lattice(const double *lv, int orth)
+{
+    Mat3 m;
+    for (int i = 0; i < 9; ++i) m.data[i] = lv[i];
+    return Lattice(m, orth ? CellType::Orthogonal : CellType::NonOrthogonal);
+}
+}  // namespace
+
+extern "C" {
+
+// FDStencil(order, grid, lattice) for a periodic grid.
+int ref_stencil(int order, int nx, int ny, int nz, const double *lv, int orth, double *w1, double *w2,
+                double *D1, double *D2, double *D2m, double *max_eig, double *h, double *dV)
+{
+    try {
+        Lattice lat = make_lattice(lv, orth);
+        FDGrid g(nx, ny, nz, lat, BCType::Periodic, BCType::Periodic, BCType::Periodic);
+        FDStencil st(order, g, lat);
+        const int m = order / 2 + 1;
+        std::memcpy(w1, st.weights_D1(), m * sizeof(double));
+        std::memcpy(w2, st.weights_D2(), m * sizeof(double));
+        const double *d1[3] = {st.D1_coeff_x(), st.D1_coeff_y(), st.D1_coeff_z()};
+        const double *d2[3] = {st.D2_coeff_x(), st.D2_coeff_y(), st.D2_coeff_z()};
+        const double *dm[3] = {st.D2_coeff_xy(), st.D2_coeff_xz(), st.D2_coeff_yz()};
+        for (int a = 0; a < 3; ++a) {
+            std::memcpy(D1 + a * m, d1[a], m * sizeof(double));
+            std::memcpy(D2 + a * m, d2[a], m * sizeof(double));
+            std::memcpy(D2m + a * m, dm[a], m * sizeof(double));
+        }
+        *max_eig = st.max_eigval_half_lap();
+        h[0] = g.dx(); h[1] = g.dy(); h[2] = g.dz();
+        *dV = g.dV();
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+// HaloExchange::execute / execute_kpt on ncol columns.
+int ref_halo(int nx, int ny, int nz, int fdn, int ncol, const double *x, double *xe, int cplx, const double *kpt,
+             const double *L)
+{
+    try {
+        Mat3 m;
+        m(0, 0) = L[0]; m(1, 1) = L[1]; m(2, 2) = L[2];
+        Lattice lat(m, CellType::Orthogonal);
+        FDGrid g(nx, ny, nz, lat, BCType::Periodic, BCType::Periodic, BCType::Periodic);
+        DomainVertices v{0, nx - 1, 0, ny - 1, 0, nz - 1};
+        Domain d(g, v);
+        HaloExchange h(d, fdn);
+        if (cplx) {
+            h.execute_kpt(reinterpret_cast<const std::complex<double> *>(x), reinterpret_cast<std::complex<double> *>(xe),
+                          ncol, Vec3(kpt[0], kpt[1], kpt[2]), Vec3(L[0], L[1], L[2]));
+        } else {
+            h.execute(x, xe, ncol);
+        }
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+// Pseudopotential::load_psp8: header values and the (l, p) radial tables.
+// sizes[0..4] = lmax, lloc, mmax, nproj_per_atom, n_tables; tables holds
+// for each nonlocal (l, p) in order: udv[mmax], udv_d[mmax]; gamma and rc as
+// flat arrays (gamma per table, rc per l).
+int ref_psp8(const char *path, int *sizes, double *zion, double *r, double *tables, double *gamma, double *rc,
+             int cap_tables)
+{
+    try {
+        Pseudopotential p;
+        p.load_psp8(path);
+        const int mmax = p.grid_size();
+        sizes[0] = p.lmax(); sizes[1] = p.lloc(); sizes[2] = mmax; sizes[3] = p.nproj_per_atom();
+        *zion = p.Zval();
+        std::memcpy(r, p.radial_grid().data(), mmax * sizeof(double));
+        int t = 0;
+        for (int l = 0; l <= p.lmax(); ++l) {
+            rc[l] = p.rc()[l];
+            if (l == p.lloc()) continue;
+            for (int q = 0; q < p.ppl()[l]; ++q) {
+                if (t >= cap_tables) return 2;
+                std::memcpy(tables + (2 * t) * mmax, p.UdV()[l][q].data(), mmax * sizeof(double));
+                std::memcpy(tables + (2 * t + 1) * mmax, p.UdV_spline_d()[l][q].data(), mmax * sizeof(double));
+                gamma[t] = p.Gamma()[l][q];
+                ++t;
+            }
+        }
+        sizes[4] = t;
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
+}
+
+// Cubic Hermite spline of NumericalMethods: derivatives and evaluation.
+void ref_spline(const double *x, const double *y, int n, const double *xi, int m, double *yd, double *yi)
+{
+    std::vector<double> vx(x, x + n), vy(y, y + n);
+    SplineData s = spline_setup(vx, vy);
+    std::memcpy(yd, s.dydx.data(), n * sizeof(double));
+    spline_eval_array(s, xi, yi, m);
+}
+
+// KPoints::generate; returns the number of reduced k-points.
+int ref_kpoints(int kx, int ky, int kz, const double *shift, const double *lv, int orth, double *kcart,
+                double *weights, int cap)
+{
+    Lattice lat = make_lattice(lv, orth);
+    KPoints k;
+    k.generate(kx, ky, kz, Vec3(shift[0], shift[1], shift[2]), lat);
+    const int n = k.Nkpts();
+    for (int i = 0; i < n && i < cap; ++i) {
+        kcart[3 * i] = k.kpts_cart()[i].x;
+        kcart[3 * i + 1] = k.kpts_cart()[i].y;
+        kcart[3 * i + 2] = k.kpts_cart()[i].z;
+        weights[i] = k.weights()[i];
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -258,3 +258,32 @@ def test_errors_are_reported():
     v[5] = np.nan
     st = L.lib().lynx_set_veff(H.ctx, L.dptr(v), L.MEM_HOST)
     assert st == L.E_NONFINITE
+
+
+@pytest.mark.parametrize("n", [(24, 24, 24), (60, 40, 26), (90, 30, 24)])
+def test_every_stencil_launch_shape(n):
+    # each launch shape the tuner may pick (points per thread, z chunks,
+    # halo slots, LDS pitch) gives the oracle's result, real and complex
+    from lynx_amd import _lib as L
+    k = (0.25 * 2 * np.pi / A_SI, 0.0, -0.125 * 2 * np.pi / A_SI)
+    s = System(n, A_SI, DIAMOND, kpt=k)
+    s.latvec = np.diag([A_SI * n[0] / 24, A_SI * n[1] / 24, A_SI * n[2] / 24])
+    op, H = s.oracle_op(), s.hamiltonian()
+    Xr = s.random_block(2, 21)
+    Xc = s.random_block(2, 22, complex_=True)
+    op.set_kpoint((0, 0, 0))
+    ref_r = op.apply(Xr, s.veff, c=0.1)
+    op.set_kpoint(k)
+    ref_c = op.apply(Xc, s.veff, c=0.1)
+    for div in (1, 2):
+        for nzc in (1, 2, 3, 4):
+            L.check(L.lib().lynx_set_stencil_config(H.ctx, div, nzc), H.ctx)
+            H.set_kpoint((0, 0, 0))
+            got = H.apply(H.halo, Xr.T.copy(), s.veff, 0.1).reshape(2, -1).T
+            assert rel(got, ref_r) < 1e-12, (div, nzc)
+            got = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k, 0.1).reshape(2, -1).T
+            assert rel(got, ref_c) < 1e-12, (div, nzc)
+            g32 = H._apply(Xr.T.copy(), s.veff, 0.1, np.float32).reshape(2, -1).T
+            H.set_kpoint((0, 0, 0))
+            assert rel(g32, ref_r) < 2e-6, (div, nzc)
+    L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0), H.ctx)

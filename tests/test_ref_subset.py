@@ -1,0 +1,124 @@
+"""Oracle restatement versus the reference's own code, for the leaf steps that
+build from the reference sources alone (oracle/_ref, compiled by
+oracle/ref_build.py in the container that holds the reference).  Skipped
+where that library is absent."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from systems import A_SI, ROOT, psp
+
+REF = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return ctypes.CDLL(REF)
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.mark.parametrize("order", [6, 12])
+@pytest.mark.parametrize("latvec,orth", [
+    (np.diag([10.26, 9.3, 12.1]), 1),
+    (0.5 * A_SI * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]), 0),
+    (np.array([[7.57, 0, 0], [0.4, 7.6, 0], [0.3, -0.2, 8.0]]), 0),
+])
+def test_stencil_matches_reference(ref, order, latvec, orth):
+    m = order // 2 + 1
+    lv = np.ascontiguousarray(latvec, float)
+    w1, w2 = np.zeros(m), np.zeros(m)
+    D1, D2, D2m = np.zeros(3 * m), np.zeros(3 * m), np.zeros(3 * m)
+    me, dV = ctypes.c_double(), ctypes.c_double()
+    h = np.zeros(3)
+    assert ref.ref_stencil(order, 20, 24, 28, P(lv), orth, P(w1), P(w2), P(D1), P(D2), P(D2m), ctypes.byref(me),
+                           P(h), ctypes.byref(dV)) == 0
+    lat = oracle.Lattice(lv, bool(orth))
+    g = oracle.Grid(20, 24, 28, lat)
+    st = oracle.stencil_coeffs(order, g)
+    rw1, rw2 = oracle.fd_weights(order)
+    assert np.array_equal(w1, rw1) and np.array_equal(w2, rw2)
+    assert np.allclose(D1.reshape(3, m), st.D1, rtol=1e-15, atol=0)
+    assert np.allclose(D2.reshape(3, m), st.D2, rtol=1e-13, atol=0)
+    assert np.allclose(D2m.reshape(3, m), st.D2m, rtol=1e-13, atol=1e-300)
+    assert abs(me.value - st.max_eig) <= 1e-13 * st.max_eig
+    assert np.allclose(h, g.h, rtol=1e-15) and abs(dV.value - g.dV) <= 1e-14 * g.dV
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_halo_matches_reference(ref, cplx):
+    n = (9, 8, 10)
+    L = np.array([9.0, 8.5, 11.0])
+    kpt = 2 * np.pi * np.array([0.25, -0.3333, 0.125]) / L
+    lat = oracle.Lattice(np.diag(L), True)
+    g = oracle.Grid(*n, lat)
+    op = oracle.Operator(g, 12, kpt=tuple(kpt) if cplx else (0, 0, 0))
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=g.nd * 2)
+    if cplx:
+        x = x + 1j * rng.normal(size=g.nd * 2)
+    mine = op.halo(x, complex_=bool(cplx))
+    out = np.zeros_like(mine)
+    xin = np.ascontiguousarray(x)
+    assert ref.ref_halo(n[0], n[1], n[2], 6, 2, P(xin), P(out), cplx, P(np.ascontiguousarray(kpt)), P(L)) == 0
+    assert np.abs(out - mine).max() <= 1e-15 * np.abs(out).max()
+
+
+def test_psp8_matches_reference(ref):
+    sizes = np.zeros(5, np.int32)
+    zion = ctypes.c_double()
+    r = np.zeros(600)
+    tabs = np.zeros(2 * 8 * 600)
+    gam = np.zeros(8)
+    rc = np.zeros(8)
+    assert ref.ref_psp8(psp("C.pot").encode(), P(sizes), ctypes.byref(zion), P(r), P(tabs), P(gam), P(rc), 8) == 0
+    lmax, lloc, mmax, nproj, nt = sizes
+    p = oracle.load_psp8(psp("C.pot"))
+    assert (lmax, lloc, mmax, nproj) == (p.lmax, p.lloc, p.mmax, p.nproj_per_atom) and zion.value == p.zion
+    assert np.array_equal(r[:mmax], p.r)
+    t = 0
+    for l in range(lmax + 1):
+        if l == lloc:
+            continue
+        assert rc[l] == p.rc[l]
+        for q in range(p.ppl[l]):
+            assert np.array_equal(tabs[2 * t * mmax:(2 * t + 1) * mmax], p.udv[(l, q)])
+            d = tabs[(2 * t + 1) * mmax:(2 * t + 2) * mmax]
+            assert np.abs(d - p.udv_d[(l, q)]).max() <= 1e-12 * np.abs(d).max()
+            assert gam[t] == p.gamma[(l, q)]
+            t += 1
+    assert t == nt
+
+
+def test_spline_matches_reference(ref):
+    x = np.cumsum(np.r_[0.0, 0.05 + 0.02 * np.sin(np.arange(79))])
+    y = np.exp(-x) * np.cos(3 * x)
+    xi = np.linspace(-0.1, x[-1] + 0.1, 333)
+    yd = np.zeros_like(x)
+    yi = np.zeros_like(xi)
+    ref.ref_spline(P(x), P(y), len(x), P(xi), len(xi), P(yd), P(yi))
+    myd = oracle.spline_deriv(x, y)
+    assert np.abs(yd - myd).max() <= 1e-13 * np.abs(yd).max()
+    assert np.abs(yi - oracle.spline_eval(x, y, myd, xi)).max() <= 1e-14
+
+
+@pytest.mark.parametrize("K,shift,latvec,orth", [
+    ((6, 6, 6), (0, 0, 0), 0.5 * A_SI * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]), 0),
+    ((2, 2, 2), (0.5, 0.5, 0.5), np.diag([A_SI] * 3), 1),
+    ((3, 4, 5), (0, 0.5, 0), np.diag([7.0, 8.0, 9.0]), 1),
+])
+def test_kpoints_match_reference(ref, K, shift, latvec, orth):
+    lv = np.ascontiguousarray(latvec, float)
+    kc = np.zeros(3 * 512)
+    w = np.zeros(512)
+    n = ref.ref_kpoints(K[0], K[1], K[2], P(np.array(shift, float)), P(lv), orth, P(kc), P(w), 512)
+    cart, red, nw = oracle.kpoints(K, shift, oracle.Lattice(lv, bool(orth)))
+    assert n == len(cart)
+    assert np.allclose(kc[:3 * n].reshape(n, 3), cart, rtol=0, atol=1e-14)
+    assert np.allclose(w[:n] / np.prod(K), nw, rtol=0, atol=1e-15)
