@@ -81,10 +81,10 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
     synchronize()
     elapsed = time.perf_counter() - t0
     L.check(lib.lynx_set_timing(ctx, 0), ctx)
-    pd = ctypes.c_int(); zc = ctypes.c_int(); nt = ctypes.c_int(16)
-    tms = (ctypes.c_double * 16)()
-    L.check(lib.lynx_get_stencil_config(ctx, dtype_code, NCOL, ctypes.byref(pd), ctypes.byref(zc), tms,
-                                        ctypes.byref(nt)), ctx)
+    thr = ctypes.c_int(); pd = ctypes.c_int(); zc = ctypes.c_int(); nt = ctypes.c_int(32)
+    tms = (ctypes.c_double * 32)()
+    L.check(lib.lynx_get_stencil_config(ctx, dtype_code, NCOL, ctypes.byref(thr), ctypes.byref(pd),
+                                        ctypes.byref(zc), tms, ctypes.byref(nt)), ctx)
     Y = np.zeros((NCOL, nd), dtype=Xh.dtype)
     Yd.download(Y)
     finite = bool(np.all(np.isfinite(Y)))
@@ -104,7 +104,7 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
         scatter_ms_per_step=kern_ms[3] / steps,
         stencil_gbs=B / (stencil_ms * 1e-3) / 1e9 if stencil_ms > 0 else None,
         lanczos=(lo.value, hi.value, it.value),
-        stencil_config={"points_div": pd.value, "z_chunks": zc.value,
+        stencil_config={"threads": thr.value, "points_div": pd.value, "z_chunks": zc.value,
                         "tuner_ms": [round(tms[i], 4) for i in range(nt.value)]},
         finite=finite,
     )

@@ -195,14 +195,15 @@ lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
 /* Launch shape of the fused stencil kernel.  By default the library times
  * the candidate shapes on the first call for each (grid, dtype, ncol) and
  * caches the fastest (its autotuner; no reference counterpart).
- * lynx_set_stencil_config forces a shape: points_div 1 or 2 (points per
- * thread = base / points_div), z_chunks >= 1; (0, 0) returns to tuning.
- * lynx_get_stencil_config reports the shape in use and, in tune_ms (capacity
- * *n_tune on input, count on output), the timed candidates in order
- * points_div 1..2 x z_chunks 1..4 (shapes too small for a grid skipped). */
-lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int points_div, int z_chunks);
-lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *points_div,
-                                    int *z_chunks, double *tune_ms, int *n_tune);
+ * lynx_set_stencil_config forces a shape: threads per workgroup 256 or 512,
+ * points_div 1 or 2 (points per thread = base / points_div), z_chunks >= 1;
+ * (0, 0, 0) returns to tuning.  lynx_get_stencil_config reports the shape in
+ * use and, in tune_ms (capacity *n_tune on input, count on output), the
+ * timed candidates in order threads 256..512 x points_div 1..2 x z_chunks
+ * 1..4 (shapes that do not fit a grid are skipped). */
+lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, int z_chunks);
+lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *threads,
+                                    int *points_div, int *z_chunks, double *tune_ms, int *n_tune);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,157 @@
+// lynx_chefsi.hpp -- C++ classes with the reference's EigenSolver and
+// Hamiltonian signatures, on top of the C ABI of liblynx_chefsi.so.
+//
+// A caller of LYNX's src/solvers/EigenSolver.hpp and
+// src/operators/Hamiltonian.hpp switches to the MI355X library by building
+// these objects instead: the argument lists, their meaning (column-major psi
+// with leading dimension ld, Nd_d grid points, eigvals ascending, Veff of
+// Nd_d doubles) and the side effect on lambda_cutoff() are the same.
+// Failures throw std::runtime_error, as the reference's solve does on a
+// Cholesky or eigensolver failure (EigenSolver.cpp:633-635, 667-669).
+#pragma once
+
+#include "lynx_chefsi.h"
+
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lynx_mi355x {
+
+using Complex = std::complex<double>;
+
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+};
+
+inline void check(lynx_status s, const lynx_ctx *c)
+{
+    if (s != LYNX_OK) {
+        std::string m = lynx_status_string(s);
+        if (c) m += std::string(": ") + lynx_last_error(c);
+        throw std::runtime_error(m);
+    }
+}
+
+// One operator H = -1/2 lap + Veff + V_nl on one grid.  setup() plays the
+// role of Hamiltonian::setup(stencil, domain, grid, halo, vnl)
+// (Hamiltonian.hpp:49-53) with the grid described by its lattice rows, the
+// point counts and the FD order.
+class Hamiltonian {
+public:
+    explicit Hamiltonian(int device_id = 0) { check(lynx_ctx_create(&ctx_, device_id), nullptr); }
+    ~Hamiltonian() { lynx_ctx_destroy(ctx_); }
+    Hamiltonian(const Hamiltonian &) = delete;
+    Hamiltonian &operator=(const Hamiltonian &) = delete;
+
+    void setup(int nx, int ny, int nz, const double latvec[9], bool orthogonal, int fd_order = 12)
+    {
+        check(lynx_set_grid(ctx_, nx, ny, nz, latvec, orthogonal ? LYNX_CELL_ORTHOGONAL : LYNX_CELL_NONORTHOGONAL,
+                            fd_order), ctx_);
+        nd_ = nx * ny * nz;
+    }
+    // Nonlocal projectors from psp8 files (Crystal::setup + NonlocalProjector::setup).
+    void set_atoms(const std::vector<std::string> &psp8, const std::vector<int> &type,
+                   const std::vector<double> &coords, bool fractional)
+    {
+        std::vector<const char *> p;
+        for (auto &s : psp8) p.push_back(s.c_str());
+        check(lynx_set_atoms(ctx_, (int)p.size(), p.data(), (int)type.size(), type.data(), coords.data(),
+                             fractional ? 1 : 0), ctx_);
+    }
+
+    // Hamiltonian::apply (Hamiltonian.hpp:64-65): psi and y packed (stride Nd).
+    void apply(const double *psi, const double *Veff, double *y, int ncol, double c = 0.0) const
+    {
+        set_veff(Veff);
+        check(lynx_h_apply(ctx_, LYNX_F64, psi, nd_, y, nd_, ncol, c, LYNX_MEM_HOST), ctx_);
+    }
+    // Hamiltonian::apply_kpt (Hamiltonian.hpp:78-80).
+    void apply_kpt(const Complex *psi, const double *Veff, Complex *y, int ncol, const Vec3 &kpt_cart,
+                   const Vec3 & /*cell_lengths*/, double c = 0.0) const
+    {
+        set_veff(Veff);
+        const double k[3] = {kpt_cart.x, kpt_cart.y, kpt_cart.z};
+        check(lynx_set_kpoint(ctx_, k), ctx_);
+        check(lynx_h_apply(ctx_, LYNX_C128, psi, nd_, y, nd_, ncol, c, LYNX_MEM_HOST), ctx_);
+    }
+
+    lynx_ctx *ctx() const { return ctx_; }
+    int Nd() const { return nd_; }
+
+    void set_veff(const double *Veff) const
+    {
+        if (Veff != last_veff_) {
+            check(lynx_set_veff(ctx_, Veff, LYNX_MEM_HOST), ctx_);
+            last_veff_ = Veff;
+        }
+    }
+    // Forget the cached potential pointer (call after changing its contents in place).
+    void invalidate_veff() const { last_veff_ = nullptr; }
+
+private:
+    lynx_ctx *ctx_ = nullptr;
+    int nd_ = 0;
+    mutable const double *last_veff_ = nullptr;
+};
+
+// EigenSolver with the reference's solve/solve_kpt/lanczos_bounds signatures
+// (EigenSolver.hpp:32-59).
+class EigenSolver {
+public:
+    void setup(const Hamiltonian &H) { H_ = &H; }
+
+    void solve(double *psi, double *eigvals, const double *Veff, int Nd_d, int Nband, double lambda_cutoff,
+               double eigval_min, double eigval_max, int cheb_degree = 20, int ld = 0)
+    {
+        need(Nd_d);
+        H_->set_veff(Veff);
+        check(lynx_chefsi_solve(H_->ctx(), LYNX_F64, psi, ld ? ld : Nd_d, Nband, eigvals, lambda_cutoff, eigval_min,
+                                eigval_max, cheb_degree, &lambda_cutoff_, LYNX_MEM_HOST), H_->ctx());
+    }
+
+    void solve_kpt(Complex *psi, double *eigvals, const double *Veff, int Nd_d, int Nband, double lambda_cutoff,
+                   double eigval_min, double eigval_max, const Vec3 &kpt_cart, const Vec3 & /*cell_lengths*/,
+                   int cheb_degree = 20, int ld = 0)
+    {
+        need(Nd_d);
+        H_->set_veff(Veff);
+        const double k[3] = {kpt_cart.x, kpt_cart.y, kpt_cart.z};
+        check(lynx_set_kpoint(H_->ctx(), k), H_->ctx());
+        check(lynx_chefsi_solve(H_->ctx(), LYNX_C128, psi, ld ? ld : Nd_d, Nband, eigvals, lambda_cutoff,
+                                eigval_min, eigval_max, cheb_degree, &lambda_cutoff_, LYNX_MEM_HOST), H_->ctx());
+    }
+
+    void lanczos_bounds(const double *Veff, int Nd_d, double &eigmin, double &eigmax, double tol = 1e-2,
+                        int max_iter = 1000)
+    {
+        need(Nd_d);
+        H_->set_veff(Veff);
+        check(lynx_lanczos_bounds(H_->ctx(), LYNX_F64, &eigmin, &eigmax, tol, max_iter, 1, nullptr), H_->ctx());
+    }
+
+    void lanczos_bounds_kpt(const double *Veff, int Nd_d, const Vec3 &kpt_cart, const Vec3 & /*cell_lengths*/,
+                            double &eigmin, double &eigmax, double tol = 1e-2, int max_iter = 1000)
+    {
+        need(Nd_d);
+        H_->set_veff(Veff);
+        const double k[3] = {kpt_cart.x, kpt_cart.y, kpt_cart.z};
+        check(lynx_set_kpoint(H_->ctx(), k), H_->ctx());
+        check(lynx_lanczos_bounds(H_->ctx(), LYNX_C128, &eigmin, &eigmax, tol, max_iter, 1, nullptr), H_->ctx());
+    }
+
+    double lambda_cutoff() const { return lambda_cutoff_; }
+    void set_lambda_cutoff(double v) { lambda_cutoff_ = v; }
+
+private:
+    void need(int Nd_d) const
+    {
+        if (!H_) throw std::runtime_error("EigenSolver::setup has not been called");
+        if (Nd_d != H_->Nd()) throw std::runtime_error("Nd_d differs from the Hamiltonian's grid");
+    }
+    const Hamiltonian *H_ = nullptr;
+    double lambda_cutoff_ = 0.0;
+};
+
+}  // namespace lynx_mi355x

@@ -365,14 +365,17 @@ void upload_phases(lynx_ctx *c)
 std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c)
 {
     std::vector<gpu::StepCfg> v;
-    for (int div : {1, 2})
-        for (int nz : {1, 2, 3, 4}) {
-            if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
-            gpu::StepCfg k;
-            k.rp_div = div;
-            k.nzch = nz;
-            v.push_back(k);
-        }
+    for (int nt : {256, 512})
+        for (int div : {1, 2})
+            for (int nz : {1, 2, 3, 4}) {
+                if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
+                if (2 * gpu::kMaxR * c->st.nx > nt * (nt >= 512 ? 3 : 6)) continue;
+                gpu::StepCfg k;
+                k.rp_div = div;
+                k.nzch = nz;
+                k.nt = nt;
+                v.push_back(k);
+            }
     return v;
 }
 
@@ -390,19 +393,21 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
         HIP_OK(hipEventCreate(&e1));
         double tbest = 1e30;
         std::vector<double> times;
+        // the timed step reads the destination as its Y_{k-2} stream (in
+        // place, as the recurrence does), so all three streams are measured
         for (const auto &k : cands) {
-            float ms = 0, tot = 0;
-            for (int rep = 0; rep < 3; ++rep) {
+            float ms = 0, tmin = 1e30f;
+            for (int rep = 0; rep < 4; ++rep) {
                 HIP_OK(hipEventRecord(e0, c->stream));
-                gpu::stencil_step(e, geom(c), x, ldx, nullptr, 0, y, ldy, veff_for(c, e), ncol, 1.0, 0.0, 0.0, k,
+                gpu::stencil_step(e, geom(c), x, ldx, y, ldy, y, ldy, veff_for(c, e), ncol, 1.0, 1e-300, 0.0, k,
                                   c->stream);
                 HIP_OK(hipEventRecord(e1, c->stream));
                 HIP_OK(hipEventSynchronize(e1));
                 HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0) tot += ms;
+                if (rep > 0) tmin = std::min(tmin, ms);
             }
-            times.push_back(tot / 2);
-            if (tot / 2 < tbest) { tbest = tot / 2; best = k; }
+            times.push_back(tmin);
+            if (tmin < tbest) { tbest = tmin; best = k; }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1116,22 +1121,24 @@ lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int la
     return LYNX_OK;
 }
 
-lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int points_div, int z_chunks)
+lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, int z_chunks)
 {
     if (!ctx) return LYNX_E_ARG;
-    if (points_div == 0 && z_chunks == 0) {
+    if (threads == 0 && points_div == 0 && z_chunks == 0) {
         ctx->has_forced = false;
         return LYNX_OK;
     }
-    if ((points_div != 1 && points_div != 2) || z_chunks < 1) return LYNX_E_ARG;
+    if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1)
+        return LYNX_E_ARG;
+    ctx->forced.nt = threads;
     ctx->forced.rp_div = points_div;
     ctx->forced.nzch = z_chunks;
     ctx->has_forced = true;
     return LYNX_OK;
 }
 
-lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *points_div, int *z_chunks,
-                                    double *tune_ms, int *n_tune)
+lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *threads, int *points_div,
+                                    int *z_chunks, double *tune_ms, int *n_tune)
 {
     if (!ctx) return LYNX_E_ARG;
     const gpu::StepCfg *k = nullptr;
@@ -1142,6 +1149,7 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
         if (it == ctx->tuned.end()) return LYNX_E_STATE;
         k = &it->second;
     }
+    if (threads) *threads = k->nt;
     if (points_div) *points_div = k->rp_div;
     if (z_chunks) *z_chunks = k->nzch;
     if (n_tune) {

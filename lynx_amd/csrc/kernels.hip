@@ -124,6 +124,19 @@ struct StepLoads {
     S halo[HP];                  // halo-row values of plane z
 };
 
+// element i of a uniform base pointer, addressed by a 32-bit byte offset so
+// the load can use a scalar base plus a vector offset
+template <class T>
+__device__ __forceinline__ T ld_at(const T *base, unsigned i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st_at(T *base, unsigned i, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(T)) = v;
+}
+
 template <class S, int R, int RP, int NT, int HP, int W>
 __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A> p)
 {
@@ -163,7 +176,7 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     const int npts = rows * nx;
     // owned points: plane offset (0 for idle lanes), LDS index and the LDS
     // indices of its wrapped x-ghost copies (dummy when none)
-    unsigned qoff[RP];
+    unsigned qoff[RP];   // element offsets within a plane
     int qli[RP], qgh[RP], qgl[RP];
     bool qv[RP];
 #pragma unroll
@@ -209,7 +222,7 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
         const S *src = X + zz * nxny;
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            S v = src[qoff[r]];
+            S v = ld_at(src, qoff[r]);
             if constexpr (C) {
                 if (side >= 0) v = phased(v, p.ph[2][side][0], p.ph[2][side][1]);
             }
@@ -227,11 +240,11 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
         const S *Xz = X + zo;
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            L.old[r] = has_old ? Oz[qoff[r]] : zero<S>();
-            L.v[r] = Vz[qoff[r]];
+            L.old[r] = has_old ? ld_at(Oz, qoff[r]) : zero<S>();
+            L.v[r] = ld_at(Vz, qoff[r]);
         }
 #pragma unroll
-        for (int h = 0; h < HP; ++h) L.halo[h] = Xz[hsrc[h]];
+        for (int h = 0; h < HP; ++h) L.halo[h] = ld_at(Xz, hsrc[h]);
     };
     // an element and its wrapped x-ghost copies into the LDS plane
     auto put = [&](S *L, int li, int gh, int gl, S v) {
@@ -285,7 +298,7 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
             const S hval = fm(cur.v[r] + p.s, ring[r][R], scl(A(-0.5), lap));
             S res = scl(p.a, hval);
             if (has_old) res = fm(p.b, cur.old[r], res);
-            if (qv[r]) Yz[qoff[r]] = res;
+            if (qv[r]) st_at(Yz, qoff[r], res);
         }
 #pragma unroll
         for (int r = 0; r < RP; ++r)
@@ -619,21 +632,21 @@ void dispatch(int e, F &&f)
 }
 
 template <class S> constexpr int base_rp() { return int(32 / sizeof(S)); }
-constexpr int kHaloSlots = 6;   // halo elements per thread: rows up to 128 points
 
-template <class S, int RP>
+template <class S, int RP, int NT>
 void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                     void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                     int nzch, hipStream_t st)
 {
     using A = typename Tr<S>::A;
-    constexpr int NT = 256;
+    constexpr int HS = NT >= 512 ? 2 : 3;      // halo slots: rows up to 64 (256) or 85 (512) points
+    constexpr int HB = NT >= 512 ? 3 : 6;      // rows up to 128 points
     StepArgs<A> p;
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const A *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
     const long pts = (long)NT * RP;
-    if (g.nx > pts || 2 * kMaxR * g.nx > NT * kHaloSlots || g.nx + 2 * kMaxR > 128)
+    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB || g.nx + 2 * kMaxR > 128)
         fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
@@ -653,13 +666,13 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
     const int blocks = ((p.nwork + 7) / 8) * 8;
     const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
     const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + 1);
-    const bool small_halo = 2 * kMaxR * g.nx <= NT * 3;
+    const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
     if (pitch == 64) {
-        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, 3, 64>), dim3(blocks), dim3(NT), shm, st, p);
-        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, kHaloSlots, 64>), dim3(blocks), dim3(NT), shm, st, p);
+        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, p);
     } else {
-        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, 3, 128>), dim3(blocks), dim3(NT), shm, st, p);
-        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, kHaloSlots, 128>), dim3(blocks), dim3(NT), shm, st, p);
+        if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, p);
     }
 }
 
@@ -680,10 +693,17 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         constexpr int B = base_rp<S>();
-        if (cfg.rp_div >= 2)
-            launch_step_rp<S, B / 2>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
-        else
-            launch_step_rp<S, B>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+        if (cfg.nt >= 512) {
+            if (cfg.rp_div >= 2)
+                launch_step_rp<S, B / 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+            else
+                launch_step_rp<S, B, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+        } else {
+            if (cfg.rp_div >= 2)
+                launch_step_rp<S, B / 2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+            else
+                launch_step_rp<S, B, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+        }
     });
 }
 

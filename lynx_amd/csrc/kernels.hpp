@@ -27,6 +27,7 @@ struct StencilGeom {
 struct StepCfg {
     int rp_div = 1;    // points per thread = base / rp_div
     int nzch = 1;      // number of z chunks per column
+    int nt = 256;      // threads per workgroup (256 or 512)
 };
 
 // out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)

@@ -151,3 +151,20 @@ def test_gpu_context_without_device_fails_cleanly():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
     assert int(r.stdout.strip().splitlines()[-1]) in (L.E_ARG, L.E_HIP)
+
+
+def test_cpp_header_builds_and_runs_host_calls(tmp_path):
+    # the C++ drop-in classes compile with g++, link the library, and run
+    # the host-side set-up through a host-only context
+    import subprocess
+    exe = tmp_path / "host_api"
+    lib_dir = os.path.join(ROOT, "lynx_amd")
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "host_api.cpp"), "-L", lib_dir, "-llynx_chefsi",
+           f"-Wl,-rpath,{lib_dir}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe), psp("C.pot")], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    nimg, ngpos, natom, nproj, threw = map(int, out.stdout.split())
+    assert (natom, nproj, threw) == (8, 64, 1) and nimg >= 8 and ngpos > 8 * 100
