@@ -46,13 +46,16 @@ def build_system():
     return System(N, A, frac)
 
 
-def run_gpu(sysm, dtype_code, steps, warmup, X64):
+def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
     from lynx_amd import _lib as L
     from lynx_amd.hipmem import DeviceArray, synchronize
     lib = L.lib()
     H = sysm.hamiltonian(0)
     H.set_veff(sysm.veff)
     ctx = H.ctx
+    if shape is not None:
+        vec, thr_, div_, zc_ = shape
+        L.check(lib.lynx_set_stencil_config(ctx, thr_, div_, zc_, vec), ctx)
     lo = ctypes.c_double(); hi = ctypes.c_double(); it = ctypes.c_int()
     L.check(lib.lynx_lanczos_bounds(ctx, L.F64, ctypes.byref(lo), ctypes.byref(hi), 0.0, LANCZOS_STEPS, 1,
                                     ctypes.byref(it)), ctx)
@@ -81,10 +84,10 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
     synchronize()
     elapsed = time.perf_counter() - t0
     L.check(lib.lynx_set_timing(ctx, 0), ctx)
-    thr = ctypes.c_int(); pd = ctypes.c_int(); zc = ctypes.c_int(); nt = ctypes.c_int(32)
-    tms = (ctypes.c_double * 32)()
+    thr = ctypes.c_int(); pd = ctypes.c_int(); zc = ctypes.c_int(); vr = ctypes.c_int(); nt = ctypes.c_int(64)
+    tms = (ctypes.c_double * 64)()
     L.check(lib.lynx_get_stencil_config(ctx, dtype_code, NCOL, ctypes.byref(thr), ctypes.byref(pd),
-                                        ctypes.byref(zc), tms, ctypes.byref(nt)), ctx)
+                                        ctypes.byref(zc), ctypes.byref(vr), tms, ctypes.byref(nt)), ctx)
     Y = np.zeros((NCOL, nd), dtype=Xh.dtype)
     Yd.download(Y)
     finite = bool(np.all(np.isfinite(Y)))
@@ -105,6 +108,7 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64):
         stencil_gbs=B / (stencil_ms * 1e-3) / 1e9 if stencil_ms > 0 else None,
         lanczos=(lo.value, hi.value, it.value),
         stencil_config={"threads": thr.value, "points_div": pd.value, "z_chunks": zc.value,
+                        "vector_runs": vr.value,
                         "tuner_ms": [round(tms[i], 4) for i in range(nt.value)]},
         finite=finite,
     )
@@ -134,7 +138,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--stencil-shape", default=None, metavar="VEC,THREADS,DIV,ZCHUNKS",
+                    help="force the FP64 stencil launch shape instead of the autotuned one (e.g. 1,512,2,1)")
     args = ap.parse_args()
+    shape = tuple(int(v) for v in args.stencil_shape.split(",")) if args.stencil_shape else None
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} (WORLD_SIZE {world}) is not run: the filter "
@@ -147,7 +154,7 @@ def main():
     rng = np.random.default_rng(2)
     X64 = np.asfortranarray(rng.uniform(-0.5, 0.5, (sysm.nd, NCOL)))
 
-    r64 = run_gpu(sysm, L.F64, args.steps, args.warmup, X64)
+    r64 = run_gpu(sysm, L.F64, args.steps, args.warmup, X64, shape)
     print(f"[bench] f64: {r64['ms_per_step']:.3f} ms/filter, {r64['gbs']:.0f} GB/s; stencil kernels "
           f"{r64['stencil_ms_per_step']:.3f} ms ({r64['stencil_gbs']:.0f} GB/s), gather "
           f"{r64['gather_ms_per_step']:.3f} ms, scatter {r64['scatter_ms_per_step']:.3f} ms", file=sys.stderr)
@@ -193,7 +200,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "k_stencil_step<double> (fused stencil + Veff + Chebyshev recurrence)",
+            "kernel": ("k_stencil_vec<double>" if r64["stencil_config"]["vector_runs"] else "k_stencil_step<double>")
+            + " (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
             "launch_shape": r64["stencil_config"],
         },

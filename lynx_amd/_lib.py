@@ -51,8 +51,8 @@ SIGNATURES = {
     "lynx_psp8_info": (_I, [ctypes.c_char_p, _DP, _IP, _IP, _IP, _DP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
-    "lynx_set_stencil_config": (_I, [_P, _I, _I, _I]),
-    "lynx_get_stencil_config": (_I, [_P, _I, _I, _IP, _IP, _IP, _DP, _IP]),
+    "lynx_set_stencil_config": (_I, [_P, _I, _I, _I, _I]),
+    "lynx_get_stencil_config": (_I, [_P, _I, _I, _IP, _IP, _IP, _IP, _DP, _IP]),
 }
 
 _lib = None

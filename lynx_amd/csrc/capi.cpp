@@ -158,7 +158,6 @@ void need_gpu_stencil(const lynx_ctx *c)
     need_device(c);
     need_grid(c);
     if (c->st.fdn != gpu::kMaxR) fail(LYNX_E_UNSUPPORTED, "GPU stencil kernels are built for FD order 12");
-    if (!c->lat.orth) fail(LYNX_E_UNSUPPORTED, "non-orthogonal cells are not supported by the GPU stencil yet");
     if (c->st.nx < gpu::kMaxR || c->st.ny < gpu::kMaxR || c->st.nz < gpu::kMaxR)
         fail(LYNX_E_ARG, "grid dimensions must be at least the stencil radius");
 }
@@ -179,6 +178,20 @@ gpu::StencilGeom geom(const lynx_ctx *c)
     g.nx = c->st.nx; g.ny = c->st.ny; g.nz = c->st.nz; g.R = c->st.fdn;
     for (int a = 0; a < 3; ++a)
         for (int k = 0; k <= gpu::kMaxR; ++k) g.c[a][k] = k <= c->st.fdn ? c->st.D2[a][k] : 0.0;
+    // factored form for the non-orthogonal kernel (FDStencil.cpp:59-101)
+    g.orth = c->lat.orth;
+    for (int k = 0; k <= gpu::kMaxR; ++k) {
+        g.w1[k] = k <= c->st.fdn ? c->st.w1[k] : 0.0;
+        g.w2[k] = k <= c->st.fdn ? c->st.w2[k] : 0.0;
+    }
+    const double *T = c->lat.lapcT;
+    for (int a = 0; a < 3; ++a) {
+        g.h[a] = c->st.h[a];
+        g.beta[a] = (c->lat.orth ? 1.0 : T[4 * a]) / (g.h[a] * g.h[a]);
+    }
+    g.alpha[0] = c->lat.orth ? 0.0 : 2.0 * T[1] / g.h[0];
+    g.alpha[1] = c->lat.orth ? 0.0 : 2.0 * T[2] / g.h[0];
+    g.alpha[2] = c->lat.orth ? 0.0 : 2.0 * T[5] / g.h[1];
     for (int a = 0; a < 3; ++a) {
         const double th = c->kpt[a] * c->lat.L[a];
         g.phase[a][0][0] = std::cos(th);
@@ -362,20 +375,23 @@ void upload_phases(lynx_ctx *c)
 // timed on the caller's own data (plain H application into the destination
 // block, which the real step overwrites afterwards); the fastest is cached
 // for this grid shape, element type and column count.
-std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c)
+std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c, int e)
 {
     std::vector<gpu::StepCfg> v;
-    for (int nt : {256, 512})
-        for (int div : {1, 2})
-            for (int nz : {1, 2, 3, 4}) {
-                if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
-                if (2 * gpu::kMaxR * c->st.nx > nt * (nt >= 512 ? 3 : 6)) continue;
-                gpu::StepCfg k;
-                k.rp_div = div;
-                k.nzch = nz;
-                k.nt = nt;
-                v.push_back(k);
-            }
+    for (int vec : {0, 1})
+        for (int nt : {256, 512})
+            for (int div : {1, 2})
+                for (int nz : {1, 2, 3, 4}) {
+                    gpu::StepCfg k;
+                    k.rp_div = div;
+                    k.nzch = nz;
+                    k.nt = nt;
+                    k.vec = vec;
+                    if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
+                    if (vec && !c->lat.orth) continue;   // the non-orthogonal kernel has one form
+                    if (!gpu::stencil_shape_fits(e, c->st.nx, k)) continue;
+                    v.push_back(k);
+                }
     return v;
 }
 
@@ -387,7 +403,7 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
     if (it != c->tuned.end()) return it->second;
     gpu::StepCfg best{};
     if (c->autotune) {
-        const auto cands = step_candidates(c);
+        const auto cands = step_candidates(c, e);
         hipEvent_t e0, e1;
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
@@ -1121,24 +1137,26 @@ lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int la
     return LYNX_OK;
 }
 
-lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, int z_chunks)
+lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, int z_chunks, int vector_runs)
 {
     if (!ctx) return LYNX_E_ARG;
     if (threads == 0 && points_div == 0 && z_chunks == 0) {
         ctx->has_forced = false;
         return LYNX_OK;
     }
-    if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1)
+    if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1 ||
+        (vector_runs != 0 && vector_runs != 1))
         return LYNX_E_ARG;
     ctx->forced.nt = threads;
     ctx->forced.rp_div = points_div;
     ctx->forced.nzch = z_chunks;
+    ctx->forced.vec = vector_runs;
     ctx->has_forced = true;
     return LYNX_OK;
 }
 
 lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *threads, int *points_div,
-                                    int *z_chunks, double *tune_ms, int *n_tune)
+                                    int *z_chunks, int *vector_runs, double *tune_ms, int *n_tune)
 {
     if (!ctx) return LYNX_E_ARG;
     const gpu::StepCfg *k = nullptr;
@@ -1152,6 +1170,7 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
     if (threads) *threads = k->nt;
     if (points_div) *points_div = k->rp_div;
     if (z_chunks) *z_chunks = k->nzch;
+    if (vector_runs) *vector_runs = k->vec;
     if (n_tune) {
         auto it = ctx->tune_log.find(key);
         const int n = it == ctx->tune_log.end() ? 0 : (int)it->second.size();

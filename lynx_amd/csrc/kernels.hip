@@ -149,10 +149,11 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     // W: LDS row pitch, a compile-time constant >= nx + 2R so that every
     // neighbour read is one base address plus an immediate offset
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    // each LDS plane buffer ends with one dummy slot that absorbs the writes
-    // of idle lanes and of elements without a ghost copy (no branches)
-    const int dummy = W * (p.tyr + 2 * R);
-    const int plane = dummy + 1;
+    // each LDS plane buffer ends with one dummy slot per lane that absorbs
+    // the writes of idle lanes and of elements without a ghost copy (no
+    // branches, and no two lanes of a wave writing the same address)
+    const int dummy = W * (p.tyr + 2 * R) + (int)threadIdx.x;
+    const int plane = W * (p.tyr + 2 * R) + NT;
 
     const int per = gridDim.x >> 3;
     const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -308,6 +309,444 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     };
 
     StepLoads<S, RP, HP> la, lb;
+    issue(z0, la);
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+        step(z, la, lb);
+        step(z + 1, lb, la);
+    }
+    if (z < z1) step(z, la, lb);
+}
+
+// --------------------------------------- fused stencil, non-orthogonal cell --
+// The non-orthogonal Laplacian adds three mixed terms to the orthogonal one
+// (Hamiltonian.cpp:176-267):
+//   sum_pq cxy[p] d1y[q] (x(+p,+q) - x(+p,-q) - x(-p,+q) + x(-p,-q))  (x,y)
+// and the same for (x,z) with cxz, d1z and (y,z) with cyz, d1z.  Each double
+// sum factors into two one-dimensional antisymmetric sums:
+//   xy: sum_p cxy[p] (g(x+p) - g(x-p)),  g = sum_q d1y[q] (x(y+q) - x(y-q))
+//   xz + yz: sum_q d1z[q] (h(z+q) - h(z-q)),
+//            h = sum_p cxz[p] (x(x+p) - x(x-p)) + cyz[p] (x(y+p) - x(y-p))
+// so a point costs 36 LDS reads and about 60 FMAs instead of 108 four-point
+// groups.  g and h are in-plane quantities: each plane passes through LDS
+// once; g goes back to LDS (with x ghosts) for its x-differences, and h
+// enters the z-difference by forward accumulation: plane f adds its
+// contributions to the 2R+1 pending outputs f-R..f+R held in registers, and
+// output f-R is complete after plane f (the potential term joins the
+// accumulator when the output's own plane passes, so no plane is kept).  A chunk of planes [z0, z1) thus
+// streams planes z0-R .. z1-1+R.  Two barriers per plane, single-buffered
+// LDS (plane + g plane).
+// Coefficients are passed factored, D2_a[k] = beta_a w2[k], mixed
+// c_ab[k] = alpha_ab w1[k] and D1_a[k] = w1[k] / h_a, so that the kernel
+// holds 24 scalars instead of 56 (they live in SGPRs).
+template <class A>
+struct NonorthArgs {
+    StepArgs<A> b;
+    A w1[kMaxR + 1];
+    A w2[kMaxR + 1];
+    A beta[3];
+    A axy_hy;            // alpha_xy / h_y
+    A axz, ayz;          // alpha_xz, alpha_yz
+    A ihz;               // 1 / h_z
+};
+
+template <class S, int R, int RP, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<S>::A> q)
+{
+    using A = typename Tr<S>::A;
+    constexpr bool C = Tr<S>::C;
+    constexpr int NA = 2 * R + 1;
+    const StepArgs<A> &p = q.b;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int dummy = W * (p.tyr + 2 * R) + (int)threadIdx.x;
+    const int plane = W * (p.tyr + 2 * R) + NT;
+    S *LX = reinterpret_cast<S *>(smem_raw);
+    S *LG = LX + plane;
+
+    const int per = gridDim.x >> 3;
+    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= p.nwork) return;
+    const int band = w % p.nbands;
+    const int rest = w / p.nbands;
+    const int zc = rest % p.nzch;
+    const int col = rest / p.nzch;
+    const int y0 = band * p.tyr;
+    const int rows = min(p.tyr, ny - y0);
+    const int z0 = zc * p.zch;
+    const int z1 = min(z0 + p.zch, nz);
+    const int nxny = nx * ny;
+
+    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
+    const bool has_old = p.b != A(0);
+
+    const int tid = threadIdx.x;
+    const int npts = rows * nx;
+    unsigned qoff[RP];
+    int qli[RP], qgh[RP], qgl[RP];
+    bool qv[RP];
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+        const int t = tid + r * NT;
+        qv[r] = t < npts;
+        const int tt = qv[r] ? t : 0;
+        const int qy = tt / nx;
+        const int qx = tt - qy * nx;
+        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
+        const int li = (qy + R) * W + qx + R;
+        qli[r] = qv[r] ? li : dummy;
+        qgh[r] = (qv[r] && qx < R) ? li + nx : dummy;
+        qgl[r] = (qv[r] && qx >= nx - R) ? li - nx : dummy;
+    }
+    const int nh = 2 * R * nx;
+    unsigned hsrc[HP];
+    int hli[HP], hgh[HP], hgl[HP], hys[HP];
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT;
+        const bool v = t < nh;
+        const int tt = v ? t : 0;
+        const int i = tt / nx;
+        const int xx = tt - i * nx;
+        const int yb = (i < R) ? (i - R) : (rows + i - R);
+        int yg = y0 + yb, ys = 0;
+        if (yg < 0) { yg += ny; ys = 1; }
+        else if (yg >= ny) { yg -= ny; ys = 2; }
+        hsrc[h] = v ? yg * nx + xx : 0;
+        const int li = (yb + R) * W + xx + R;
+        hli[h] = v ? li : dummy;
+        hgh[h] = (v && xx < R) ? li + nx : dummy;
+        hgl[h] = (v && xx >= nx - R) ? li - nx : dummy;
+        hys[h] = ys;
+    }
+
+    const int f_end = z1 - 1 + R;
+    // every load of plane f (owned points and halo rows, z-wrapped) and of
+    // the output plane f - R (Y_{k-2}, Veff), clamped into range
+    auto issue = [&](int f, StepLoads<S, RP, HP> &L) {
+        f = min(f, f_end);
+        int zz = f;
+        if (zz < 0) zz += nz;
+        else if (zz >= nz) zz -= nz;
+        const S *Xz = X + zz * nxny;
+#pragma unroll
+        for (int r = 0; r < RP; ++r) L.front[r] = ld_at(Xz, qoff[r]);
+#pragma unroll
+        for (int h = 0; h < HP; ++h) L.halo[h] = ld_at(Xz, hsrc[h]);
+        const int zo = max(f - R, z0) * nxny;
+        const int zv = min(max(f, z0), z1 - 1) * nxny;
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            L.old[r] = has_old ? ld_at(O + zo, qoff[r]) : zero<S>();
+            L.v[r] = ld_at(p.veff + zv, qoff[r]);
+        }
+    };
+    auto put = [&](S *L, int li, int gh, int gl, S v) {
+        L[li] = v;
+        L[gh] = phased(v, p.ph[0][1][0], p.ph[0][1][1]);
+        L[gl] = phased(v, p.ph[0][0][0], p.ph[0][0][1]);
+    };
+
+    // acc[j] accumulates -2 (H - s) x at output plane f - R + j; the output's
+    // own plane adds -2 (Veff + s) x there, so H x = -acc / 2 at the end
+    S acc[RP][NA];
+#pragma unroll
+    for (int r = 0; r < RP; ++r)
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[r][j] = zero<S>();
+    const A diag = (q.beta[0] + q.beta[1] + q.beta[2]) * q.w2[0];
+
+    auto step = [&](int f, StepLoads<S, RP, HP> &cur, StepLoads<S, RP, HP> &nxt) {
+        issue(f + 1, nxt);
+        // z-wrap phase of this plane (uniform)
+        A zr = A(1), zi = A(0);
+        if constexpr (C) {
+            if (f < 0) { zr = p.ph[2][0][0]; zi = p.ph[2][0][1]; }
+            else if (f >= nz) { zr = p.ph[2][1][0]; zi = p.ph[2][1][1]; }
+        }
+        S xf[RP];
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            xf[r] = phased(cur.front[r], zr, zi);
+            put(LX, qli[r], qgh[r], qgl[r], xf[r]);
+        }
+#pragma unroll
+        for (int h = 0; h < HP; ++h) {
+            S v = cur.halo[h];
+            if constexpr (C) {
+                const A pr = hys[h] == 1 ? p.ph[1][0][0] : (hys[h] == 2 ? p.ph[1][1][0] : A(1));
+                const A pi = hys[h] == 1 ? p.ph[1][0][1] : (hys[h] == 2 ? p.ph[1][1][1] : A(0));
+                v = cmul(cmul(v, pr, pi), zr, zi);
+            }
+            put(LX, hli[h], hgh[h], hgl[h], v);
+        }
+        __syncthreads();
+        S inpl[RP], hz[RP], xzs[RP];
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            const S *c = LX + qli[r];
+            S sx2 = zero<S>(), sy2 = zero<S>(), sx1 = zero<S>(), sy1 = zero<S>();
+#pragma unroll
+            for (int k = 1; k <= R; ++k) {
+                const S xp = c[k], xm = c[-k], yp = c[k * W], ym = c[-k * W];
+                sx2 = fm(q.w2[k], add(xp, xm), sx2);
+                sy2 = fm(q.w2[k], add(yp, ym), sy2);
+                sx1 = fm(q.w1[k], add(xp, scl(A(-1), xm)), sx1);
+                sy1 = fm(q.w1[k], add(yp, scl(A(-1), ym)), sy1);
+            }
+            // in-plane second differences, the potential term (-2 (V + s) x)
+            S a2 = fm(diag - A(2) * (cur.v[r] + p.s), xf[r], scl(q.beta[0], sx2));
+            inpl[r] = fm(q.beta[1], sy2, a2);
+            hz[r] = scl(q.ihz, fm(q.axz, sx1, scl(q.ayz, sy1)));
+            xzs[r] = scl(q.beta[2], xf[r]);
+            put(LG, qli[r], qgh[r], qgl[r], sy1);
+        }
+        __syncthreads();
+        const bool out = f - R >= z0;
+        S *Yz = Y + (f - R) * nxny;
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            const S *g = LG + qli[r];
+            S mxy = zero<S>();
+#pragma unroll
+            for (int k = 1; k <= R; ++k) mxy = fm(q.w1[k], add(g[k], scl(A(-1), g[-k])), mxy);
+            const S a2 = fm(q.axy_hy, mxy, inpl[r]);
+            // plane f contributes to outputs f-R+j, j = 0..2R (distance d = R-j)
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+                const int d = R - j;
+                if (d == 0) {
+                    acc[r][j] = add(acc[r][j], a2);
+                } else {
+                    const int ad = d > 0 ? d : -d;
+                    acc[r][j] = fm(q.w2[ad], xzs[r], acc[r][j]);
+                    acc[r][j] = fm(d > 0 ? q.w1[ad] : -q.w1[ad], hz[r], acc[r][j]);
+                }
+            }
+            if (out) {
+                S res = scl(A(-0.5) * p.a, acc[r][0]);
+                if (has_old) res = fm(p.b, cur.old[r], res);
+                if (qv[r]) st_at(Yz, qoff[r], res);
+            }
+#pragma unroll
+            for (int j = 0; j + 1 < NA; ++j) acc[r][j] = acc[r][j + 1];
+            acc[r][NA - 1] = zero<S>();
+        }
+    };
+
+    StepLoads<S, RP, HP> la, lb;
+    int f = z0 - R;
+    issue(f, la);
+    for (; f + 1 <= f_end; f += 2) {
+        step(f, la, lb);
+        step(f + 1, lb, la);
+    }
+    if (f <= f_end) step(f, la, lb);
+}
+
+// ------------------------------------------------ fused stencil, 16-B runs --
+// Same step as k_stencil_step for real element types, with each thread owning
+// runs of VW = 16/sizeof(S) consecutive x points (two doubles, four floats).
+// A run is loaded, stored and kept in LDS as one 16-byte vector; its x
+// neighbours come from 1 + 2*XO/VW aligned 16-byte LDS reads shared by the
+// run, its y neighbours from one 16-byte read per offset.  This cuts the LDS
+// instructions per point by about 2.5x and makes every global access a
+// dwordx4.  Requires nx % VW == 0 and 16-byte aligned columns.
+template <class S>
+struct alignas(16) V16 {
+    S v[16 / sizeof(S)];
+};
+
+template <class S>
+__device__ __forceinline__ V16<S> vzero()
+{
+    V16<S> r;
+#pragma unroll
+    for (int e = 0; e < int(16 / sizeof(S)); ++e) r.v[e] = S(0);
+    return r;
+}
+
+template <class S>
+__device__ __forceinline__ V16<S> vld(const S *base, unsigned i)
+{
+    return *reinterpret_cast<const V16<S> *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(S));
+}
+
+template <class S, int RV, int HP>
+struct VecLoads {
+    V16<S> front[RV];
+    V16<S> old[RV];
+    V16<S> v[RV];
+    V16<S> halo[HP];
+};
+
+template <class S, int R, int RV, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
+{
+    constexpr int VW = int(16 / sizeof(S));
+    constexpr int XO = (R + VW - 1) / VW * VW;   // LDS column of x = 0, vector aligned
+    constexpr int NXV = 1 + 2 * XO / VW;         // 16-byte reads covering x-R .. x+VW-1+R
+    constexpr int NR = 2 * R + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    S *lds = reinterpret_cast<S *>(smem_raw);
+
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int base = W * (p.tyr + 2 * R);
+    const int plane = base + NT * VW;            // + one dummy run per lane
+
+    const int per = gridDim.x >> 3;
+    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= p.nwork) return;
+    const int band = w % p.nbands;
+    const int rest = w / p.nbands;
+    const int zc = rest % p.nzch;
+    const int col = rest / p.nzch;
+    const int y0 = band * p.tyr;
+    const int rows = min(p.tyr, ny - y0);
+    const int z0 = zc * p.zch;
+    const int z1 = min(z0 + p.zch, nz);
+    const int nxny = nx * ny;
+
+    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
+    const bool has_old = p.b != S(0);
+
+    const int tid = threadIdx.x;
+    const int dummy = base + tid * VW;
+    const int nvec = rows * nx / VW;
+    // owned runs: plane offset (element index), LDS index and x of the first
+    // element (kNoX for lanes without a run, whose stores go to the dummy)
+    constexpr int kNoX = -(1 << 20);
+    unsigned qoff[RV];
+    int qli[RV], qx0[RV];
+    bool qv[RV];
+#pragma unroll
+    for (int r = 0; r < RV; ++r) {
+        const int q = tid + r * NT;
+        qv[r] = q < nvec;
+        const int e0 = (qv[r] ? q : 0) * VW;
+        const int qy = e0 / nx;
+        const int qx = e0 - qy * nx;
+        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
+        qli[r] = qv[r] ? (qy + R) * W + XO + qx : dummy;
+        qx0[r] = qv[r] ? qx : kNoX;
+    }
+    const int nhv = 2 * R * nx / VW;
+    unsigned hsrc[HP];
+    int hli[HP], hx0[HP];
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT;
+        const bool v = t < nhv;
+        const int e0 = (v ? t : 0) * VW;
+        const int i = e0 / nx;
+        const int xx = e0 - i * nx;
+        const int yb = (i < R) ? (i - R) : (rows + i - R);
+        int yg = y0 + yb;
+        if (yg < 0) yg += ny;
+        else if (yg >= ny) yg -= ny;
+        hsrc[h] = v ? yg * nx + xx : 0;
+        hli[h] = v ? (yb + R) * W + XO + xx : dummy;
+        hx0[h] = v ? xx : kNoX;
+    }
+
+    auto load_plane = [&](int zz, V16<S> *dst) {
+        if (zz < 0) zz += nz;
+        else if (zz >= nz) zz -= nz;
+        const S *src = X + zz * nxny;
+#pragma unroll
+        for (int r = 0; r < RV; ++r) dst[r] = vld(src, qoff[r]);
+    };
+    auto issue = [&](int zs, VecLoads<S, RV, HP> &L) {
+        zs = min(zs, z1 - 1);
+        load_plane(zs + R, L.front);
+        const int zo = zs * nxny;
+#pragma unroll
+        for (int r = 0; r < RV; ++r) {
+            L.old[r] = has_old ? vld(O + zo, qoff[r]) : vzero<S>();
+            L.v[r] = vld(p.veff + zo, qoff[r]);
+        }
+#pragma unroll
+        for (int h = 0; h < HP; ++h) L.halo[h] = vld(X + zo, hsrc[h]);
+    };
+    // a run and the wrapped copies of its elements that lie within R of an
+    // x face (x + nx to the right of the row, x - nx to the left)
+    auto put = [&](S *L, int li, int x0, const V16<S> &v) {
+        *reinterpret_cast<V16<S> *>(L + li) = v;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const int xe = x0 + e;
+            L[unsigned(xe) < unsigned(R) ? li + e + nx : dummy + e] = v.v[e];
+            L[unsigned(xe - (nx - R)) < unsigned(R) ? li + e - nx : dummy + e] = v.v[e];
+        }
+    };
+
+    V16<S> ring[RV][NR];
+#pragma unroll
+    for (int i = 0; i < 2 * R; ++i) {
+        V16<S> tmp[RV];
+        load_plane(z0 - R + i, tmp);
+#pragma unroll
+        for (int r = 0; r < RV; ++r) ring[r][i] = tmp[r];
+    }
+
+    const S diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
+    int buf = 0;
+    auto step = [&](int z, VecLoads<S, RV, HP> &cur, VecLoads<S, RV, HP> &nxt) {
+        issue(z + 1, nxt);
+        S *L = lds + buf * plane;
+#pragma unroll
+        for (int r = 0; r < RV; ++r) {
+            ring[r][2 * R] = cur.front[r];
+            put(L, qli[r], qx0[r], ring[r][R]);
+        }
+#pragma unroll
+        for (int h = 0; h < HP; ++h) put(L, hli[h], hx0[h], cur.halo[h]);
+        __syncthreads();
+        S *Yz = Y + z * nxny;
+#pragma unroll
+        for (int r = 0; r < RV; ++r) {
+            S xr[NXV * VW];
+#pragma unroll
+            for (int j = 0; j < NXV; ++j) {
+                const V16<S> t = *reinterpret_cast<const V16<S> *>(L + qli[r] - XO + j * VW);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) xr[j * VW + e] = t.v[e];
+            }
+            S lap[VW];
+#pragma unroll
+            for (int e = 0; e < VW; ++e) lap[e] = diag * ring[r][R].v[e];
+#pragma unroll
+            for (int k = 1; k <= R; ++k) {
+                const V16<S> up = *reinterpret_cast<const V16<S> *>(L + qli[r] + k * W);
+                const V16<S> dn = *reinterpret_cast<const V16<S> *>(L + qli[r] - k * W);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    lap[e] = fm(p.c[0][k], add(xr[XO + e + k], xr[XO + e - k]), lap[e]);
+                    lap[e] = fm(p.c[1][k], add(up.v[e], dn.v[e]), lap[e]);
+                    lap[e] = fm(p.c[2][k], add(ring[r][R + k].v[e], ring[r][R - k].v[e]), lap[e]);
+                }
+            }
+            V16<S> res;
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                const S hval = fm(cur.v[r].v[e] + p.s, ring[r][R].v[e], S(-0.5) * lap[e]);
+                res.v[e] = p.a * hval;
+                if (has_old) res.v[e] = fm(p.b, cur.old[r].v[e], res.v[e]);
+            }
+            if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+        }
+#pragma unroll
+        for (int r = 0; r < RV; ++r)
+#pragma unroll
+            for (int i = 0; i < 2 * R; ++i) ring[r][i] = ring[r][i + 1];
+        buf ^= 1;
+    };
+
+    VecLoads<S, RV, HP> la, lb;
     issue(z0, la);
     int z = z0;
     for (; z + 1 < z1; z += 2) {
@@ -665,7 +1104,7 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
             for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (A)g.phase[ax][sd][q];
     const int blocks = ((p.nwork + 7) / 8) * 8;
     const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
-    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + 1);
+    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
     if (pitch == 64) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, p);
@@ -674,6 +1113,108 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
         if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, p);
         else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, p);
     }
+}
+
+template <class S, int RP, int NT>
+void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo, void *out,
+                         long ldout, const void *veff, int ncol, double a, double b, double s, int nzch,
+                         hipStream_t st)
+{
+    using A = typename Tr<S>::A;
+    constexpr int HS = NT >= 512 ? 2 : 3;
+    constexpr int HB = NT >= 512 ? 3 : 6;
+    const long pts = (long)NT * RP;
+    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB || g.nx + 2 * kMaxR > 128)
+        fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
+    NonorthArgs<A> q;
+    StepArgs<A> &p = q.b;
+    p.x = x; p.old = old; p.out = out; p.veff = static_cast<const A *>(veff);
+    p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
+    p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
+    int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
+    nb = std::max(1, std::min(nb, g.ny));
+    p.tyr = (g.ny + nb - 1) / nb;
+    while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+    p.nbands = (g.ny + p.tyr - 1) / p.tyr;
+    nzch = std::max(1, std::min(nzch, g.nz));
+    p.zch = (g.nz + nzch - 1) / nzch;
+    p.nzch = (g.nz + p.zch - 1) / p.zch;
+    p.nwork = p.nbands * p.nzch * ncol;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k <= kMaxR; ++k) p.c[ax][k] = (A)g.c[ax][k];
+    for (int k = 0; k <= kMaxR; ++k) {
+        q.w1[k] = (A)g.w1[k];
+        q.w2[k] = (A)g.w2[k];
+    }
+    for (int ax = 0; ax < 3; ++ax) q.beta[ax] = (A)g.beta[ax];
+    q.axy_hy = (A)(g.alpha[0] / g.h[1]);
+    q.axz = (A)g.alpha[1];
+    q.ayz = (A)g.alpha[2];
+    q.ihz = (A)(1.0 / g.h[2]);
+    p.a = (A)a; p.b = (A)b; p.s = (A)s;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int sd = 0; sd < 2; ++sd)
+            for (int k = 0; k < 2; ++k) p.ph[ax][sd][k] = (A)g.phase[ax][sd][k];
+    const int blocks = ((p.nwork + 7) / 8) * 8;
+    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
+    const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
+    if (pitch == 64) {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, q);
+        else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, q);
+    } else {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, q);
+        else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, q);
+    }
+}
+
+template <class S, int RV, int NT>
+bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
+                     void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
+                     int nzch, hipStream_t st)
+{
+    constexpr int VW = int(16 / sizeof(S));
+    constexpr int XO = (kMaxR + VW - 1) / VW * VW;
+    constexpr int HS = NT >= 512 ? 1 : 2;      // halo runs per thread
+    constexpr int HB = NT >= 512 ? 2 : 3;
+    auto aligned = [](const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; };
+    if (g.nx % VW || (ldx % VW) || (old && ldo % VW) || (ldout % VW) || !aligned(x) || (old && !aligned(old)) ||
+        !aligned(out) || !aligned(veff))
+        return false;
+    const long pts = (long)NT * RV * VW;
+    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB * VW || g.nx + 2 * XO > 128) return false;
+    static_assert(2 * kMaxR * (128 - 2 * XO) <= NT * HB * VW, "halo slots cover every row that fits the pitch");
+    StepArgs<S> p;
+    p.x = x; p.old = old; p.out = out; p.veff = static_cast<const S *>(veff);
+    p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
+    p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
+    int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
+    nb = std::max(1, std::min(nb, g.ny));
+    p.tyr = (g.ny + nb - 1) / nb;
+    while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+    p.nbands = (g.ny + p.tyr - 1) / p.tyr;
+    nzch = std::max(1, std::min(nzch, g.nz));
+    p.zch = (g.nz + nzch - 1) / nzch;
+    p.nzch = (g.nz + p.zch - 1) / p.zch;
+    p.nwork = p.nbands * p.nzch * ncol;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int k = 0; k <= kMaxR; ++k) p.c[ax][k] = (S)g.c[ax][k];
+    p.a = (S)a; p.b = (S)b; p.s = (S)s;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int sd = 0; sd < 2; ++sd)
+            for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (S)g.phase[ax][sd][q];
+    const int blocks = ((p.nwork + 7) / 8) * 8;
+    const int pitch = g.nx + 2 * XO <= 64 ? 64 : 128;
+    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT * VW);
+    const bool small_halo = 2 * kMaxR * g.nx <= NT * HS * VW;
+    if (pitch == 64) {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, p);
+    } else {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, p);
+        else hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, p);
+    }
+    return true;
 }
 
 }  // namespace
@@ -686,6 +1227,30 @@ int stencil_points_per_thread(int e, const StepCfg &cfg)
     return rp;
 }
 
+bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
+{
+    bool ok = false;
+    dispatch(e, [&](auto t) {
+        using S = decltype(t);
+        const int nt = cfg.nt >= 512 ? 512 : 256;
+        if (cfg.vec) {
+            if constexpr (Tr<S>::C) {
+                ok = false;
+            } else {
+                constexpr int VW = int(16 / sizeof(S));
+                constexpr int XO = (kMaxR + VW - 1) / VW * VW;
+                const int hb = nt >= 512 ? 2 : 3;
+                const long pts = (long)nt * (cfg.rp_div >= 2 ? 1 : 2) * VW;
+                ok = nx % VW == 0 && nx <= pts && 2 * kMaxR * nx <= nt * hb * VW && nx + 2 * XO <= 128;
+            }
+        } else {
+            const long pts = (long)nt * (base_rp<S>() / std::max(1, cfg.rp_div));
+            ok = nx <= pts && 2 * kMaxR * nx <= nt * (nt >= 512 ? 3 : 6) && nx + 2 * kMaxR <= 128;
+        }
+    });
+    return ok;
+}
+
 void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                   void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                   const StepCfg &cfg, hipStream_t st)
@@ -693,6 +1258,33 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         constexpr int B = base_rp<S>();
+        if (!g.orth) {
+            // non-orthogonal cells: 2R+1 accumulators per point, so half the
+            // orthogonal kernel's points per thread
+            constexpr int B2 = B / 2 > 0 ? B / 2 : 1;
+            constexpr int B4 = B / 4 > 0 ? B / 4 : 1;
+            if (cfg.nt >= 512) {
+                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                else launch_step_nonorth<S, B2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+            } else {
+                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                else launch_step_nonorth<S, B2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+            }
+            return;
+        }
+        if constexpr (!Tr<S>::C) {
+            // real types: 16-byte runs; base RV = 2 runs, RV = 1 for rp_div 2
+            if (cfg.vec) {
+                bool ok;
+                if (cfg.nt >= 512)
+                    ok = cfg.rp_div >= 2 ? launch_step_vec<S, 1, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st)
+                                         : launch_step_vec<S, 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                else
+                    ok = cfg.rp_div >= 2 ? launch_step_vec<S, 1, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st)
+                                         : launch_step_vec<S, 2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                if (ok) return;
+            }
+        }
         if (cfg.nt >= 512) {
             if (cfg.rp_div >= 2)
                 launch_step_rp<S, B / 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);

@@ -20,6 +20,11 @@ struct StencilGeom {
     int nx, ny, nz, R;
     double c[3][kMaxR + 1];          // D2 coefficients per axis
     double phase[3][2][2];           // [axis][0: e^{-i th}, 1: e^{+i th}][re, im]
+    // non-orthogonal cells: D2_a[k] = beta_a w2[k], mixed coefficient
+    // c_ab[k] = alpha_ab w1[k] (ab = xy, xz, yz), D1_a[k] = w1[k] / h_a
+    bool orth = true;
+    double w1[kMaxR + 1], w2[kMaxR + 1];
+    double beta[3], alpha[3], h[3];
 };
 
 // Launch shape of the fused stencil step; chosen per grid shape by the
@@ -28,6 +33,7 @@ struct StepCfg {
     int rp_div = 1;    // points per thread = base / rp_div
     int nzch = 1;      // number of z chunks per column
     int nt = 256;      // threads per workgroup (256 or 512)
+    int vec = 1;       // real types: threads own 16-byte x runs (falls back when unaligned)
 };
 
 // out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)
@@ -35,6 +41,9 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
                   void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                   const StepCfg &cfg, hipStream_t st);
 int stencil_points_per_thread(int e, const StepCfg &cfg);
+// True when the shape can run the grid's x rows (halo slots and LDS pitch);
+// for vec shapes, also when nx is a multiple of the 16-byte run.
+bool stencil_shape_fits(int e, int nx, const StepCfg &cfg);
 
 // Nonlocal projector data in device layout.
 //   points: all (image, sphere point) pairs grouped by physical atom;

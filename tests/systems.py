@@ -40,6 +40,25 @@ def displace(frac, a, sigma, seed):
     return frac + rng.normal(0.0, sigma, frac.shape) / a
 
 
+FCC_PRIMITIVE = 0.5 * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]])
+SI2 = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]])
+
+
+def gaussian_veff_cell(n, latvec, frac, depth=1.2, width=1.6, offset=0.3):
+    """Gaussian wells at every atom of a general cell (rows of latvec are
+    the lattice vectors), nearest periodic image in fractional coordinates."""
+    nx, ny, nz = n
+    k, j, i = np.meshgrid(np.arange(nz) / nz, np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    g = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1)
+    V = np.full(g.shape[0], offset)
+    for f in np.asarray(frac):
+        df = g - f
+        df -= np.round(df)
+        d = df @ latvec
+        V -= depth * np.exp(-(d * d).sum(axis=1) / (2 * width ** 2))
+    return V
+
+
 def gaussian_veff(n, a, frac, depth=1.2, width=1.6, offset=0.3):
     """Effective potential with a Gaussian well at every atom (periodic
     minimum image), a stand-in for the atomic-superposition Veff."""
@@ -62,25 +81,30 @@ class System:
     """One cubic cell: grid, atoms, Veff, and both the oracle operator and
     (optionally) the GPU Hamiltonian built from the same inputs."""
 
-    def __init__(self, n, a, frac, psp_name="C.pot", order=12, with_nonlocal=True, kpt=(0.0, 0.0, 0.0)):
+    def __init__(self, n, a, frac, psp_name="C.pot", order=12, with_nonlocal=True, kpt=(0.0, 0.0, 0.0),
+                 latvec=None):
         if isinstance(n, int):
             n = (n, n, n)
         self.n = tuple(n)
         self.a = a
         self.frac = np.asarray(frac, float)
         self.order = order
-        self.latvec = np.diag([a, a, a]).astype(float)
+        self.orth = latvec is None
+        self.latvec = np.diag([a, a, a]).astype(float) if latvec is None else np.asarray(latvec, float)
         self.psp_files = [psp(psp_name)]
         self.types = np.zeros(len(self.frac), np.int32)
         self.with_nonlocal = with_nonlocal and len(self.frac) > 0
-        self.veff = gaussian_veff(self.n, a, self.frac)
+        if self.orth:
+            self.veff = gaussian_veff(self.n, a, self.frac)
+        else:
+            self.veff = gaussian_veff_cell(self.n, self.latvec, self.frac)
         self.nd = int(np.prod(self.n))
         self.kpt = tuple(kpt)
 
     # -- oracle ---------------------------------------------------------------
     def oracle_op(self):
         import oracle
-        lat = oracle.Lattice(self.latvec, True)
+        lat = oracle.Lattice(self.latvec, self.orth)
         g = oracle.Grid(*self.n, lat)
         if self.with_nonlocal:
             ps = [oracle.load_psp8(f) for f in self.psp_files]
@@ -90,7 +114,7 @@ class System:
     # -- GPU library -------------------------------------------------------------
     def hamiltonian(self, device_id=0):
         from lynx_amd import core
-        lat = core.Lattice(self.latvec, core.CellType.Orthogonal)
+        lat = core.Lattice(self.latvec, core.CellType.Orthogonal if self.orth else core.CellType.NonOrthogonal)
         grid = core.FDGrid(*self.n, lat)
         st = core.FDStencil(self.order, grid, lat)
         dom = core.Domain(grid)
@@ -119,3 +143,8 @@ class System:
 
 def si8(n=30, **kw):
     return System(n, A_SI, DIAMOND, **kw)
+
+
+def si2_fcc(n=24, **kw):
+    """Si2 in the FCC primitive cell (non-orthogonal), a = 10.26 Bohr."""
+    return System(n, A_SI, SI2, latvec=A_SI * FCC_PRIMITIVE, **kw)

@@ -275,15 +275,15 @@ def test_every_stencil_launch_shape(n):
     ref_r = op.apply(Xr, s.veff, c=0.1)
     op.set_kpoint(k)
     ref_c = op.apply(Xc, s.veff, c=0.1)
-    shapes = [(t, d, z) for t in (256, 512) for d in (1, 2) for z in (1, 2, 3, 4)]
-    for nt, div, nzc in shapes:
-        L.check(L.lib().lynx_set_stencil_config(H.ctx, nt, div, nzc), H.ctx)
+    shapes = [(v, t, d, z) for v in (0, 1) for t in (256, 512) for d in (1, 2) for z in (1, 2, 3, 4)]
+    for shape in shapes:
+        L.check(L.lib().lynx_set_stencil_config(H.ctx, *shape[1:], shape[0]), H.ctx)
         H.set_kpoint((0, 0, 0))
         got = H.apply(H.halo, Xr.T.copy(), s.veff, 0.1).reshape(2, -1).T
-        assert rel(got, ref_r) < 1e-12, (nt, div, nzc)
+        assert rel(got, ref_r) < 1e-12, shape
         got = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k, 0.1).reshape(2, -1).T
-        assert rel(got, ref_c) < 1e-12, (nt, div, nzc)
+        assert rel(got, ref_c) < 1e-12, shape
         H.set_kpoint((0, 0, 0))
         g32 = H._apply(Xr.T.copy(), s.veff, 0.1, np.float32).reshape(2, -1).T
-        assert rel(g32, ref_r) < 2e-6, (nt, div, nzc)
-    L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0), H.ctx)
+        assert rel(g32, ref_r) < 2e-6, shape
+    L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0, 0), H.ctx)

@@ -161,6 +161,93 @@ def test_plane_wave_symbol_with_bloch_phase():
     assert np.abs(y - lam * psi).max() < 1e-10 * abs(lam)
 
 
+FCC = 0.5 * A_SI * np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 1.0], [1.0, 1.0, 0.0]])
+
+
+def _nonorth_symbol(st, phi):
+    # FD symbol of the non-orthogonal operator on exp(i sum_a phi_a n_a):
+    # second-difference terms give cosines; each mixed term
+    # sum_pq c[p] d1[q] (x++ - x+- - x-+ + x--) gives -4 sum_p c[p] sin(p phi_a)
+    # * sum_q d1[q] sin(q phi_b)
+    R = st.fdn
+    s = 0.0
+    for a in range(3):
+        s += st.D2[a, 0] + 2 * sum(st.D2[a, p] * math.cos(p * phi[a]) for p in range(1, R + 1))
+    sn = lambda c, f: sum(c[p] * math.sin(p * f) for p in range(1, R + 1))  # noqa: E731
+    s -= 4 * sn(st.D2m[0], phi[0]) * sn(st.D1[1], phi[1])
+    s -= 4 * sn(st.D2m[1], phi[0]) * sn(st.D1[2], phi[2])
+    s -= 4 * sn(st.D2m[2], phi[1]) * sn(st.D1[2], phi[2])
+    return -0.5 * s
+
+
+def test_nonorth_plane_wave_symbol_with_bloch_phase():
+    # FCC primitive cell, unequal grid dimensions so a swapped axis shows:
+    # a Bloch plane wave is an eigenvector with the stencil symbol as its
+    # eigenvalue (closed form, independent of the real-space loops)
+    lat = oracle.Lattice(FCC, False)
+    n = (12, 14, 16)
+    g = oracle.Grid(*n, lat)
+    L = lat.lengths
+    kpt = np.array([0.25, -0.125, 0.375]) * 2 * math.pi / L   # theta_a = kpt_a L_a
+    op = oracle.Operator(g, 12, kpt=tuple(kpt))
+    m = np.array([1, -2, 3])
+    phi = (kpt * L + 2 * math.pi * m) / np.array(n)
+    k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    idx = np.stack([i, j, k], axis=-1).reshape(-1, 3)
+    psi = np.exp(1j * idx @ phi)
+    y = op.apply(psi[:, None], np.zeros(g.nd), nonlocal_=False, complex_=True)[:, 0]
+    lam = _nonorth_symbol(op.st, phi)
+    assert np.abs(y - lam * psi).max() < 1e-10 * abs(lam)
+    assert abs(op.st.D2m[0, 1]) > 0 and abs(op.st.D2m[2, 1]) > 0
+
+
+@pytest.mark.parametrize("m", [(1, 0, 0), (0, 1, -1), (1, 1, 1), (2, -1, 1)])
+def test_nonorth_laplacian_continuum_limit(m):
+    # -1/2 lap cos(G.r) = 1/2 |G|^2 cos(G.r) for G = 2 pi sum_a m_a b_a; at
+    # long wavelength FD12 is exact to ~1e-7, so the lapc_T weights, the
+    # factor 2 of the mixed terms and their 1/h scaling are all pinned
+    lat = oracle.Lattice(FCC, False)
+    n = (30, 32, 34)
+    g = oracle.Grid(*n, lat)
+    op = oracle.Operator(g, 12)
+    B = np.linalg.inv(FCC).T                      # rows b_a with a_i . b_j = delta_ij
+    G = 2 * math.pi * np.asarray(m, float) @ B
+    k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    frac = np.stack([i / n[0], j / n[1], k / n[2]], axis=-1).reshape(-1, 3)
+    r = frac @ FCC
+    psi = np.cos(r @ G)
+    y = op.apply(psi[:, None], np.zeros(g.nd), nonlocal_=False)[:, 0]
+    want = 0.5 * (G @ G) * psi
+    assert np.abs(y - want).max() < 2e-6 * 0.5 * (G @ G)
+
+
+def test_nonorth_sphere_membership_brute_force():
+    # every (atom, image, point) with Cartesian distance <= rc, found by brute
+    # force over +-1 images in an FCC primitive cell
+    lat = oracle.Lattice(FCC, False)
+    n = 12
+    g = oracle.Grid(n, n, n, lat)
+    ps = [oracle.load_psp8(psp("C.pot"))]
+    frac = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]])
+    op = oracle.setup_system(g, ps, [0, 0], frac, 12)
+    rc = ps[0].rc_max
+    L = lat.lengths
+    got = {(im.atom, tuple(np.round(im.shift / L).astype(int)), int(p)) for im in op.images for p in im.gpos}
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    fr = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) / n
+    want = set()
+    rng = range(-2, 3)
+    for a, f in enumerate(frac):
+        for sx in rng:
+            for sy in rng:
+                for sz in rng:
+                    sh = np.array([sx, sy, sz])
+                    d = np.linalg.norm((fr - (f + sh)) @ FCC, axis=1)
+                    for p in np.nonzero(d <= rc)[0]:
+                        want.add((a, (sx, sy, sz), int(p)))
+    assert len(want) > 0 and got == want
+
+
 # ---------------------------------------------------------------- psp8 and splines
 
 
