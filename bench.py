@@ -40,6 +40,24 @@ def filter_bytes(nd, ncol, degree, esize):
     return esize * nd * ncol * (3 * degree - 1) + 8 * nd * degree
 
 
+def profiled_traffic(shape, kernel_prefix):
+    """HBM bytes per launch of the stencil kernel from a committed rocprofv3
+    summary (profiles/*.md: FETCH_SIZE and WRITE_SIZE runs, calibrated on a
+    device copy) for the same launch shape, or None."""
+    import glob
+    key = "%d,%d,%d,%d" % (shape["vector_runs"], shape["threads"], shape["points_div"], shape["z_chunks"])
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        fields = {}
+        with open(path) as f:
+            for line in f:
+                if ":" in line and line.split(":", 1)[0] in ("traffic_bytes_per_launch", "kernel", "shape"):
+                    k, v = line.split(":", 1)
+                    fields[k] = v.strip()
+        if fields.get("shape") == key and fields.get("kernel", "").startswith(kernel_prefix):
+            return int(fields["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def build_system():
     from systems import System, displace, supercell
     frac = displace(supercell(2), A, 0.05, seed=1)
@@ -174,6 +192,9 @@ def main():
         cpu = cpu_baseline(sysm, X64, 0.5 * (lo + hi), lo, hi)
 
     achieved = r64["stencil_gbs"]
+    kname = {0: "k_stencil_step<double", 1: "k_stencil_vec<double", 2: "k_stencil_vacc<double"}[
+        r64["stencil_config"]["vector_runs"]]
+    traffic, traffic_src = profiled_traffic(r64["stencil_config"], kname)
     line = {
         "metric": METRIC,
         "value": round(r64["gbs"], 2),
@@ -199,8 +220,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": ("k_stencil_vec<double>" if r64["stencil_config"]["vector_runs"] else "k_stencil_step<double>")
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "kernel": {0: "k_stencil_step<double>", 1: "k_stencil_vec<double>",
+                       2: "k_stencil_vacc<double>"}[r64["stencil_config"]["vector_runs"]]
             + " (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
             "launch_shape": r64["stencil_config"],

@@ -378,7 +378,7 @@ void upload_phases(lynx_ctx *c)
 std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c, int e)
 {
     std::vector<gpu::StepCfg> v;
-    for (int vec : {0, 1})
+    for (int vec : {0, 1, 2})
         for (int nt : {256, 512})
             for (int div : {1, 2})
                 for (int nz : {1, 2, 3, 4}) {
@@ -1052,11 +1052,20 @@ lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, doubl
         void *vjp1 = ctx->lz[2].get(n * es);
         double *part = (double *)ctx->partials.get(gpu::kPartialBlocks * sizeof(double));
         (void)part;
-        // Lanczos.hpp:104-110: glibc rand() start vector, normalised on the host
+        // Lanczos.hpp:104-110: glibc rand() start vector, normalised on the
+        // host.  The complex entries are Complex(rand(), rand()) there, whose
+        // two calls g++ evaluates imaginary part first (Lanczos.hpp:64-66).
         const long nr = cplx ? 2 * n : n;
         std::vector<double> v(nr);
         std::srand(seed);
-        for (long i = 0; i < nr; ++i) v[i] = (double)std::rand() / RAND_MAX;
+        if (cplx) {
+            for (long i = 0; i < n; ++i) {
+                v[2 * i + 1] = (double)std::rand() / RAND_MAX;
+                v[2 * i] = (double)std::rand() / RAND_MAX;
+            }
+        } else {
+            for (long i = 0; i < n; ++i) v[i] = (double)std::rand() / RAND_MAX;
+        }
         double s = 0;
         if (cplx) for (long i = 0; i < n; ++i) s += v[2 * i] * v[2 * i] + v[2 * i + 1] * v[2 * i + 1];
         else for (long i = 0; i < n; ++i) s += v[i] * v[i];
@@ -1145,7 +1154,7 @@ lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, 
         return LYNX_OK;
     }
     if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1 ||
-        (vector_runs != 0 && vector_runs != 1))
+        vector_runs < 0 || vector_runs > 2)
         return LYNX_E_ARG;
     ctx->forced.nt = threads;
     ctx->forced.rp_div = points_div;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 namespace lynx {
 namespace gpu {
@@ -137,8 +138,17 @@ __device__ __forceinline__ void st_at(T *base, unsigned i, T v)
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(T)) = v;
 }
 
+// XCD-aware work order: the hardware hands workgroups to the 8 XCDs round
+// robin, so workgroup b runs on XCD b % 8; mapping it to work item
+// (b % 8) * per + b / 8 gives each XCD a contiguous range of work items.
+__device__ __forceinline__ int xcd_work_item()
+{
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 template <class S, int R, int RP, int NT, int HP, int W>
-__global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A> p)
+__device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>::A> &p, const int w)
 {
     using A = typename Tr<S>::A;
     constexpr bool C = Tr<S>::C;
@@ -155,9 +165,6 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     const int dummy = W * (p.tyr + 2 * R) + (int)threadIdx.x;
     const int plane = W * (p.tyr + 2 * R) + NT;
 
-    const int per = gridDim.x >> 3;
-    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (w >= p.nwork) return;
     const int band = w % p.nbands;
     const int rest = w / p.nbands;
     const int zc = rest % p.nzch;
@@ -318,6 +325,14 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
     if (z < z1) step(z, la, lb);
 }
 
+template <class S, int R, int RP, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A> p)
+{
+    const int w = xcd_work_item();
+    if (w >= p.nwork) return;
+    stencil_step_body<S, R, RP, NT, HP, W>(p, w);
+}
+
 // --------------------------------------- fused stencil, non-orthogonal cell --
 // The non-orthogonal Laplacian adds three mixed terms to the orthogonal one
 // (Hamiltonian.cpp:176-267):
@@ -351,7 +366,7 @@ struct NonorthArgs {
 };
 
 template <class S, int R, int RP, int NT, int HP, int W>
-__global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<S>::A> q)
+__device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename Tr<S>::A> &q, const int w)
 {
     using A = typename Tr<S>::A;
     constexpr bool C = Tr<S>::C;
@@ -364,9 +379,6 @@ __global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<
     S *LX = reinterpret_cast<S *>(smem_raw);
     S *LG = LX + plane;
 
-    const int per = gridDim.x >> 3;
-    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (w >= p.nwork) return;
     const int band = w % p.nbands;
     const int rest = w / p.nbands;
     const int zc = rest % p.nzch;
@@ -488,12 +500,19 @@ __global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<
         for (int r = 0; r < RP; ++r) {
             const S *c = LX + qli[r];
             S sx2 = zero<S>(), sy2 = zero<S>(), sx1 = zero<S>(), sy1 = zero<S>();
+            // x then y neighbours; the scheduling barrier keeps the compiler
+            // from hoisting all 24 LDS reads at once (register pressure)
 #pragma unroll
             for (int k = 1; k <= R; ++k) {
-                const S xp = c[k], xm = c[-k], yp = c[k * W], ym = c[-k * W];
+                const S xp = c[k], xm = c[-k];
                 sx2 = fm(q.w2[k], add(xp, xm), sx2);
-                sy2 = fm(q.w2[k], add(yp, ym), sy2);
                 sx1 = fm(q.w1[k], add(xp, scl(A(-1), xm)), sx1);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 1; k <= R; ++k) {
+                const S yp = c[k * W], ym = c[-k * W];
+                sy2 = fm(q.w2[k], add(yp, ym), sy2);
                 sy1 = fm(q.w1[k], add(yp, scl(A(-1), ym)), sy1);
             }
             // in-plane second differences, the potential term (-2 (V + s) x)
@@ -546,6 +565,14 @@ __global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<
     if (f <= f_end) step(f, la, lb);
 }
 
+template <class S, int R, int RP, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<S>::A> q)
+{
+    const int w = xcd_work_item();
+    if (w >= q.b.nwork) return;
+    stencil_nonorth_body<S, R, RP, NT, HP, W>(q, w);
+}
+
 // ------------------------------------------------ fused stencil, 16-B runs --
 // Same step as k_stencil_step for real element types, with each thread owning
 // runs of VW = 16/sizeof(S) consecutive x points (two doubles, four floats).
@@ -583,7 +610,7 @@ struct VecLoads {
 };
 
 template <class S, int R, int RV, int NT, int HP, int W>
-__global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
+__device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const int w)
 {
     constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (R + VW - 1) / VW * VW;   // LDS column of x = 0, vector aligned
@@ -596,9 +623,6 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
     const int base = W * (p.tyr + 2 * R);
     const int plane = base + NT * VW;            // + one dummy run per lane
 
-    const int per = gridDim.x >> 3;
-    const int w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (w >= p.nwork) return;
     const int band = w % p.nbands;
     const int rest = w / p.nbands;
     const int zc = rest % p.nzch;
@@ -754,6 +778,217 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
         step(z + 1, lb, la);
     }
     if (z < z1) step(z, la, lb);
+}
+
+template <class S, int R, int RV, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
+{
+    const int w = xcd_work_item();
+    if (w >= p.nwork) return;
+    stencil_vec_body<S, R, RV, NT, HP, W>(p, w);
+}
+
+// The same 16-byte-run step with the z-direction accumulated forward
+// instead of kept as a ring of planes: the LDS plane is the newest plane f,
+// whose in-plane terms go to output f and whose z-neighbour terms go to the
+// 2R pending outputs f-R..f+R (2R+1 accumulators per run, as many registers
+// as the ring).  Output f-R is complete after plane f.  The halo rows of
+// plane f are then read in the same step in which the neighbouring bands
+// read those rows as their own newest plane, so they come from L2 instead of
+// being re-fetched R steps later.  Planes outside the chunk only feed the
+// z-terms and skip LDS.
+template <class S, int R, int RV, int NT, int HP, int W>
+__device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const int w)
+{
+    constexpr int VW = int(16 / sizeof(S));
+    constexpr int XO = (R + VW - 1) / VW * VW;
+    constexpr int NXV = 1 + 2 * XO / VW;
+    constexpr int NA = 2 * R + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    S *lds = reinterpret_cast<S *>(smem_raw);
+
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int base = W * (p.tyr + 2 * R);
+    const int plane = base + NT * VW;
+
+    const int band = w % p.nbands;
+    const int rest = w / p.nbands;
+    const int zc = rest % p.nzch;
+    const int col = rest / p.nzch;
+    const int y0 = band * p.tyr;
+    const int rows = min(p.tyr, ny - y0);
+    const int z0 = zc * p.zch;
+    const int z1 = min(z0 + p.zch, nz);
+    const int nxny = nx * ny;
+
+    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
+    const bool has_old = p.b != S(0);
+
+    const int tid = threadIdx.x;
+    const int dummy = base + tid * VW;
+    const int nvec = rows * nx / VW;
+    constexpr int kNoX = -(1 << 20);
+    unsigned qoff[RV];
+    int qli[RV], qx0[RV];
+    bool qv[RV];
+#pragma unroll
+    for (int r = 0; r < RV; ++r) {
+        const int q = tid + r * NT;
+        qv[r] = q < nvec;
+        const int e0 = (qv[r] ? q : 0) * VW;
+        const int qy = e0 / nx;
+        const int qx = e0 - qy * nx;
+        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
+        qli[r] = qv[r] ? (qy + R) * W + XO + qx : dummy;
+        qx0[r] = qv[r] ? qx : kNoX;
+    }
+    const int nhv = 2 * R * nx / VW;
+    unsigned hsrc[HP];
+    int hli[HP], hx0[HP];
+#pragma unroll
+    for (int h = 0; h < HP; ++h) {
+        const int t = tid + h * NT;
+        const bool v = t < nhv;
+        const int e0 = (v ? t : 0) * VW;
+        const int i = e0 / nx;
+        const int xx = e0 - i * nx;
+        const int yb = (i < R) ? (i - R) : (rows + i - R);
+        int yg = y0 + yb;
+        if (yg < 0) yg += ny;
+        else if (yg >= ny) yg -= ny;
+        hsrc[h] = v ? yg * nx + xx : 0;
+        hli[h] = v ? (yb + R) * W + XO + xx : dummy;
+        hx0[h] = v ? xx : kNoX;
+    }
+
+    const int f_end = z1 - 1 + R;
+    // loads for plane f: its owned runs (z-wrapped), and, for planes of the
+    // chunk, its halo runs and Veff; Y_{k-2} of output plane f - R
+    auto issue = [&](int f, VecLoads<S, RV, HP> &L) {
+        f = min(f, f_end);
+        int zz = f;
+        if (zz < 0) zz += nz;
+        else if (zz >= nz) zz -= nz;
+        const S *Xz = X + zz * nxny;
+#pragma unroll
+        for (int r = 0; r < RV; ++r) L.front[r] = vld(Xz, qoff[r]);
+        if (f >= z0 && f < z1) {
+#pragma unroll
+            for (int h = 0; h < HP; ++h) L.halo[h] = vld(Xz, hsrc[h]);
+#pragma unroll
+            for (int r = 0; r < RV; ++r) L.v[r] = vld(p.veff + zz * nxny, qoff[r]);
+        }
+        if (has_old && f - R >= z0) {
+            const int zo = (f - R) * nxny;
+#pragma unroll
+            for (int r = 0; r < RV; ++r) L.old[r] = vld(O + zo, qoff[r]);
+        }
+    };
+    auto put = [&](S *L, int li, int x0, const V16<S> &v) {
+        *reinterpret_cast<V16<S> *>(L + li) = v;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const int xe = x0 + e;
+            L[unsigned(xe) < unsigned(R) ? li + e + nx : dummy + e] = v.v[e];
+            L[unsigned(xe - (nx - R)) < unsigned(R) ? li + e - nx : dummy + e] = v.v[e];
+        }
+    };
+
+    // acc[j]: -2 (H - s) x at output plane f - R + j, so H x = -acc / 2
+    V16<S> acc[RV][NA];
+#pragma unroll
+    for (int r = 0; r < RV; ++r)
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[r][j] = vzero<S>();
+
+    const S diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
+    int buf = 0;
+    auto step = [&](int f, VecLoads<S, RV, HP> &cur, VecLoads<S, RV, HP> &nxt) {
+        issue(f + 1, nxt);
+        if (f >= z0 && f < z1) {
+            S *L = lds + buf * plane;
+#pragma unroll
+            for (int r = 0; r < RV; ++r) put(L, qli[r], qx0[r], cur.front[r]);
+#pragma unroll
+            for (int h = 0; h < HP; ++h) put(L, hli[h], hx0[h], cur.halo[h]);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < RV; ++r) {
+                S xr[NXV * VW];
+#pragma unroll
+                for (int j = 0; j < NXV; ++j) {
+                    const V16<S> t = *reinterpret_cast<const V16<S> *>(L + qli[r] - XO + j * VW);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) xr[j * VW + e] = t.v[e];
+                }
+                S lap[VW];
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    lap[e] = fm(diag - S(2) * (cur.v[r].v[e] + p.s), cur.front[r].v[e], acc[r][R].v[e]);
+#pragma unroll
+                for (int k = 1; k <= R; ++k) {
+                    const V16<S> up = *reinterpret_cast<const V16<S> *>(L + qli[r] + k * W);
+                    const V16<S> dn = *reinterpret_cast<const V16<S> *>(L + qli[r] - k * W);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) {
+                        lap[e] = fm(p.c[0][k], add(xr[XO + e + k], xr[XO + e - k]), lap[e]);
+                        lap[e] = fm(p.c[1][k], add(up.v[e], dn.v[e]), lap[e]);
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < VW; ++e) acc[r][R].v[e] = lap[e];
+            }
+            buf ^= 1;
+        }
+        // z-neighbour terms of plane f for the outputs within R of it
+#pragma unroll
+        for (int r = 0; r < RV; ++r)
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+                if (j == R) continue;
+                const int d = j < R ? R - j : j - R;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) acc[r][j].v[e] = fm(p.c[2][d], cur.front[r].v[e], acc[r][j].v[e]);
+            }
+        if (f - R >= z0) {
+            S *Yz = Y + (f - R) * nxny;
+#pragma unroll
+            for (int r = 0; r < RV; ++r) {
+                V16<S> res;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    res.v[e] = (S(-0.5) * p.a) * acc[r][0].v[e];
+                    if (has_old) res.v[e] = fm(p.b, cur.old[r].v[e], res.v[e]);
+                }
+                if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RV; ++r) {
+#pragma unroll
+            for (int j = 0; j + 1 < NA; ++j) acc[r][j] = acc[r][j + 1];
+            acc[r][NA - 1] = vzero<S>();
+        }
+    };
+
+    VecLoads<S, RV, HP> la, lb;
+    int f = z0 - R;
+    issue(f, la);
+    for (; f + 1 <= f_end; f += 2) {
+        step(f, la, lb);
+        step(f + 1, lb, la);
+    }
+    if (f <= f_end) step(f, la, lb);
+}
+
+template <class S, int R, int RV, int NT, int HP, int W>
+__global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
+{
+    const int w = xcd_work_item();
+    if (w >= p.nwork) return;
+    stencil_vacc_body<S, R, RV, NT, HP, W>(p, w);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -1168,7 +1403,7 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     }
 }
 
-template <class S, int RV, int NT>
+template <class S, int RV, int NT, bool ACC>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                      int nzch, hipStream_t st)
@@ -1207,12 +1442,23 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     const int pitch = g.nx + 2 * XO <= 64 ? 64 : 128;
     const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT * VW);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS * VW;
-    if (pitch == 64) {
-        if (small_halo) hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, p);
-        else hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, p);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), shm, st, p); };
+    if constexpr (ACC) {
+        if (pitch == 64) {
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64>);
+        } else {
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128>);
+        }
     } else {
-        if (small_halo) hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, p);
-        else hipLaunchKernelGGL((k_stencil_vec<S, kMaxR, RV, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, p);
+        if (pitch == 64) {
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 64>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 64>);
+        } else {
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 128>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 128>);
+        }
     }
     return true;
 }
@@ -1259,10 +1505,11 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
         using S = decltype(t);
         constexpr int B = base_rp<S>();
         if (!g.orth) {
-            // non-orthogonal cells: 2R+1 accumulators per point, so half the
-            // orthogonal kernel's points per thread
-            constexpr int B2 = B / 2 > 0 ? B / 2 : 1;
-            constexpr int B4 = B / 4 > 0 ? B / 4 : 1;
+            // non-orthogonal cells: 2R+1 accumulators per point, so a quarter
+            // of the orthogonal kernel's points per thread (8 B per thread for
+            // points_div 1, half that for 2, at least one point)
+            constexpr int B2 = B / 4 > 0 ? B / 4 : 1;
+            constexpr int B4 = B / 8 > 0 ? B / 8 : 1;
             if (cfg.nt >= 512) {
                 if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
                 else launch_step_nonorth<S, B2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
@@ -1276,12 +1523,23 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
             // real types: 16-byte runs; base RV = 2 runs, RV = 1 for rp_div 2
             if (cfg.vec) {
                 bool ok;
-                if (cfg.nt >= 512)
-                    ok = cfg.rp_div >= 2 ? launch_step_vec<S, 1, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st)
-                                         : launch_step_vec<S, 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
-                else
-                    ok = cfg.rp_div >= 2 ? launch_step_vec<S, 1, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st)
-                                         : launch_step_vec<S, 2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                auto run = [&](auto rv, auto nt, auto acc) {
+                    return launch_step_vec<S, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value>(
+                        g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                };
+                using I1 = std::integral_constant<int, 1>;
+                using I2 = std::integral_constant<int, 2>;
+                using N256 = std::integral_constant<int, 256>;
+                using N512 = std::integral_constant<int, 512>;
+                if (cfg.vec == 2) {
+                    using T = std::true_type;
+                    if (cfg.nt >= 512) ok = cfg.rp_div >= 2 ? run(I1{}, N512{}, T{}) : run(I2{}, N512{}, T{});
+                    else ok = cfg.rp_div >= 2 ? run(I1{}, N256{}, T{}) : run(I2{}, N256{}, T{});
+                } else {
+                    using F = std::false_type;
+                    if (cfg.nt >= 512) ok = cfg.rp_div >= 2 ? run(I1{}, N512{}, F{}) : run(I2{}, N512{}, F{});
+                    else ok = cfg.rp_div >= 2 ? run(I1{}, N256{}, F{}) : run(I2{}, N256{}, F{});
+                }
                 if (ok) return;
             }
         }

@@ -33,7 +33,8 @@ struct StepCfg {
     int rp_div = 1;    // points per thread = base / rp_div
     int nzch = 1;      // number of z chunks per column
     int nt = 256;      // threads per workgroup (256 or 512)
-    int vec = 1;       // real types: threads own 16-byte x runs (falls back when unaligned)
+    int vec = 1;       // real types: threads own 16-byte x runs (falls back when unaligned);
+                       // 2: the same with z-terms accumulated forward
 };
 
 // out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)

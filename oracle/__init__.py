@@ -704,6 +704,14 @@ def rand_uniform(seed, lo, hi, n):
     return out
 
 
+def lanczos_start(complex_, n, seed=1):
+    """Lanczos.hpp random_init after srand(seed), before normalisation:
+    n doubles, or n complex entries whose imaginary part is drawn first."""
+    out = np.empty(2 * n if complex_ else n, np.float64)
+    lib().orc_lanczos_start(int(bool(complex_)), ctypes.c_long(n), ctypes.c_uint(seed), _p(out))
+    return out[0::2] + 1j * out[1::2] if complex_ else out
+
+
 def setup_system(grid: Grid, psps, types, frac, order=12, kpt=(0.0, 0.0, 0.0)):
     """Build the oracle operator for atoms at fractional coordinates."""
     pos = positions_noncart(grid.lattice, frac)

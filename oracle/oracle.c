@@ -498,13 +498,41 @@ void orc_lanczos_real(const orc_op *op, const double *veff, double tol, int max_
 #undef PROJ_TERM
 }
 
+/* Complex start entry of Lanczos.hpp random_init: Complex(rand()/RAND_MAX,
+ * rand()/RAND_MAX).  The order of the two calls is unspecified in C++; g++
+ * (the reference's compiler here) evaluates the imaginary argument first,
+ * which the reference subset confirms, so the first draw is the imaginary
+ * part. */
+static zd urand_cplx(void)
+{
+    const double im = urand();
+    const double re = urand();
+    return re + I * im;
+}
+
+/* The start vector before normalisation, for checking against the
+ * reference's own random_init; out holds n doubles or n (re, im) pairs. */
+void orc_lanczos_start(int complex_, long n, unsigned seed, double *out)
+{
+    srand(seed);
+    if (complex_) {
+        for (long i = 0; i < n; ++i) {
+            const zd z = urand_cplx();
+            out[2 * i] = creal(z);
+            out[2 * i + 1] = cimag(z);
+        }
+    } else {
+        for (long i = 0; i < n; ++i) out[i] = urand();
+    }
+}
+
 void orc_lanczos_cplx(const orc_op *op, const double *veff, double tol, int max_iter,
                       unsigned seed, int with_nonlocal, double *eig_min, double *eig_max,
                       int *iters)
 {
 #define PROJ_TERM(u, v) (conj(u) * (v))
     LANCZOS_BODY(zd, orc_h_apply_cplx, dot_cplx, nrm_cplx,
-                 (urand() + I * urand()), ((-1.0 + 2.0 * urand()) + I * (-1.0 + 2.0 * urand())))
+                 urand_cplx(), ((-1.0 + 2.0 * urand()) + I * (-1.0 + 2.0 * urand())))
 #undef PROJ_TERM
 }
 

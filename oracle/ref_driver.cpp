@@ -4,7 +4,8 @@
 // oracle's restatement (and the library's host set-up) against the
 // reference's own code for the leaf steps that build without MPI, LAPACK or
 // the rest of the program: FD stencil, lattice/grid, halo wrap, psp8 reader
-// and splines, k-points.
+// and splines, k-points, and the start vector of the Lanczos bound
+// estimate (only its header-inline random fill is instantiated).
 #include "core/Domain.hpp"
 #include "core/FDGrid.hpp"
 #include "core/KPoints.hpp"
@@ -13,6 +14,7 @@
 #include "atoms/Pseudopotential.hpp"
 #include "operators/FDStencil.hpp"
 #include "parallel/HaloExchange.hpp"
+#include "solvers/Lanczos.hpp"
 
 #include <complex>
 #include <cstring>
@@ -141,6 +143,21 @@ int ref_kpoints(int kx, int ky, int kz, const double *shift, const double *lv, i
         weights[i] = k.weights()[i];
     }
     return n;
+}
+
+// Lanczos start vector before normalisation (Lanczos.hpp random_init after
+// srand(seed)), as this compiler orders the two rand() calls of the complex
+// case; out holds n doubles (real) or n interleaved (re, im) pairs.
+void ref_lanczos_start(int complex_, int n, unsigned seed, double *out)
+{
+    std::srand(seed);
+    if (complex_) {
+        std::vector<std::complex<double>> v(n);
+        lanczos_detail::random_init(v.data(), n);
+        std::memcpy(out, v.data(), sizeof(double) * 2 * n);
+    } else {
+        lanczos_detail::random_init(out, n);
+    }
 }
 
 }  // extern "C"

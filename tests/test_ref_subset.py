@@ -122,3 +122,15 @@ def test_kpoints_match_reference(ref, K, shift, latvec, orth):
     assert n == len(cart)
     assert np.allclose(kc[:3 * n].reshape(n, 3), cart, rtol=0, atol=1e-14)
     assert np.allclose(w[:n] / np.prod(K), nw, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_lanczos_start_vector_matches_reference(ref, cplx):
+    # the reference's random_init as g++ compiles it: for complex entries the
+    # two rand() calls of Complex(rand(), rand()) run imaginary part first
+    n = 257
+    out = np.zeros(2 * n if cplx else n)
+    ref.ref_lanczos_start(cplx, n, 1, P(out))
+    want = out[0::2] + 1j * out[1::2] if cplx else out
+    got = oracle.lanczos_start(cplx, n, 1)
+    assert np.array_equal(got, want)
