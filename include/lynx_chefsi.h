@@ -212,6 +212,33 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
                                     int *points_div, int *z_chunks, int *vector_runs, double *tune_ms,
                                     int *n_tune);
 
+/* ---- Batched filter ------------------------------------------------------
+ * A batch filters many independent units together: unit u applies the
+ * operator of context ctxs[u] (grid, atoms and Veff set) at k-point
+ * kpts[3u..3u+2] (lattice-aligned basis, as lynx_set_kpoint; NULL or ignored
+ * for real dtypes) to its own block.  A context may serve several units
+ * (the k-points of one structure); contexts with different grids mix
+ * freely (a heterogeneous batch).  Each Chebyshev step is one launch per
+ * kernel for all units, and the whole filter is replayed from a HIP graph
+ * captured on the first call with the same buffers and degree.  The
+ * reference filters k-points and structures one after another
+ * (SCF.cpp:391-411 calling EigenSolver::solve_kpt, EigenSolver.cpp:834-842
+ * per k-point); this is the batched form of lynx_cheb_filter. */
+typedef struct lynx_batch lynx_batch;
+lynx_status lynx_batch_create(lynx_batch **batch, int nunit, lynx_ctx *const *ctxs, const double *kpts);
+lynx_status lynx_batch_destroy(lynx_batch *batch);
+const char *lynx_batch_last_error(const lynx_batch *batch);
+/* Per-unit blocks: x[u], y[u] with leading dimensions ldx[u], ldy[u] (0 or
+ * NULL: Nd of the unit), ncol[u] columns, and per-unit filter bounds. */
+lynx_status lynx_batch_filter(lynx_batch *batch, int dtype, void *const *x, const long *ldx, void *const *y,
+                              const long *ldy, const int *ncol, int degree, const double *lambda_cutoff,
+                              const double *eig_min, const double *eig_max, int mem);
+/* 1 (default): capture the filter into a HIP graph and replay it; 0: launch eagerly. */
+lynx_status lynx_batch_set_graph(lynx_batch *batch, int enable);
+/* Device time of the last lynx_batch_filter (HIP events around the graph
+ * launch or the eager launches) and the number of kernel launches it holds. */
+lynx_status lynx_batch_last_timing(const lynx_batch *batch, double *ms, int *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,13 @@ SIGNATURES = {
     "lynx_psp8_info": (_I, [ctypes.c_char_p, _DP, _IP, _IP, _IP, _DP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
+    "lynx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), _DP]),
+    "lynx_batch_destroy": (_I, [_P]),
+    "lynx_batch_last_error": (ctypes.c_char_p, [_P]),
+    "lynx_batch_filter": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_L), ctypes.POINTER(_P),
+                               ctypes.POINTER(_L), _IP, _I, _DP, _DP, _DP, _I]),
+    "lynx_batch_set_graph": (_I, [_P, _I]),
+    "lynx_batch_last_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_stencil_config": (_I, [_P, _I, _I, _I, _I]),
     "lynx_get_stencil_config": (_I, [_P, _I, _I, _IP, _IP, _IP, _IP, _DP, _IP]),
 }

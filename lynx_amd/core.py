@@ -341,3 +341,79 @@ class EigenSolver:
     def set_lambda_cutoff(self, v): self._lambda_cutoff = float(v)
     def Nband_global(self): return self._nband
     def is_band_parallel(self): return False
+
+
+class BatchFilter:
+    """Chebyshev filter of many independent units in one pass: unit i is
+    ``(H, kpt)`` with a set-up Hamiltonian (its grid, atoms and Veff) and a
+    k-point in the lattice-aligned basis (``None`` for Gamma).  The k-points
+    of one structure share one Hamiltonian; structures with different grids
+    mix freely.  Each Chebyshev step is one launch per kernel for all units,
+    replayed from a HIP graph (lynx_batch_filter)."""
+
+    def __init__(self, units, graph=True):
+        self.units = list(units)
+        n = len(self.units)
+        ctxs = (ctypes.c_void_p * n)(*[H.ctx.value for H, _ in self.units])
+        k = np.zeros(3 * n)
+        for i, (_, kpt) in enumerate(self.units):
+            if kpt is not None:
+                k[3 * i:3 * i + 3] = kpt
+        self._b = ctypes.c_void_p()
+        st = L.lib().lynx_batch_create(ctypes.byref(self._b), n, ctxs, L.dptr(k))
+        if st != L.OK:
+            raise L.LynxError(st, "lynx_batch_create failed: %s" % L.lib().lynx_status_string(st).decode())
+        self._check(L.lib().lynx_batch_set_graph(self._b, 1 if graph else 0))
+
+    def _check(self, st):
+        if st != L.OK:
+            raise L.LynxError(st, L.lib().lynx_batch_last_error(self._b).decode())
+
+    def filter(self, blocks, degree, lambda_cutoff, eig_min, eig_max, dtype=None):
+        """blocks: list of (Nd, ncol) arrays (column-major data); returns the
+        filtered blocks in the same dtype (float64, complex128, float32 or
+        complex64)."""
+        n = len(self.units)
+        if dtype is None:
+            dtype = blocks[0].dtype
+        xs = [np.asfortranarray(b, dtype=dtype) for b in blocks]
+        ys = [np.zeros_like(x) for x in xs]
+        xp = (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs])
+        yp = (ctypes.c_void_p * n)(*[y.ctypes.data for y in ys])
+        ld = (ctypes.c_long * n)(*[x.shape[0] for x in xs])
+        nc = np.array([x.shape[1] for x in xs], np.int32)
+        lc = np.broadcast_to(np.asarray(lambda_cutoff, float), (n,)).copy()
+        lo = np.broadcast_to(np.asarray(eig_min, float), (n,)).copy()
+        hi = np.broadcast_to(np.asarray(eig_max, float), (n,)).copy()
+        self._check(L.lib().lynx_batch_filter(self._b, L.dtype_code(dtype), xp, ld, yp, ld, L.iptr(nc), int(degree),
+                                              L.dptr(lc), L.dptr(lo), L.dptr(hi), L.MEM_HOST))
+        return ys
+
+    def filter_device(self, dtype, xptrs, yptrs, lds, ncols, degree, lambda_cutoff, eig_min, eig_max):
+        """Device-resident variant: lists of device addresses and sizes."""
+        n = len(self.units)
+        xp = (ctypes.c_void_p * n)(*xptrs)
+        yp = (ctypes.c_void_p * n)(*yptrs)
+        ld = (ctypes.c_long * n)(*lds)
+        nc = np.asarray(ncols, np.int32)
+        lc = np.broadcast_to(np.asarray(lambda_cutoff, float), (n,)).copy()
+        lo = np.broadcast_to(np.asarray(eig_min, float), (n,)).copy()
+        hi = np.broadcast_to(np.asarray(eig_max, float), (n,)).copy()
+        self._check(L.lib().lynx_batch_filter(self._b, L.dtype_code(dtype), xp, ld, yp, ld, L.iptr(nc), int(degree),
+                                              L.dptr(lc), L.dptr(lo), L.dptr(hi), L.MEM_DEVICE))
+
+    def last_timing(self):
+        ms = ctypes.c_double(); nl = ctypes.c_int()
+        self._check(L.lib().lynx_batch_last_timing(self._b, ctypes.byref(ms), ctypes.byref(nl)))
+        return ms.value, nl.value
+
+    def close(self):
+        if self._b:
+            L.lib().lynx_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

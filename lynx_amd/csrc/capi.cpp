@@ -172,7 +172,12 @@ void check_dtype(int dtype)
     if (dtype < 0 || dtype > 3) fail(LYNX_E_ARG, "unknown dtype");
 }
 
-gpu::StencilGeom geom(const lynx_ctx *c)
+gpu::StencilGeom geom_at(const lynx_ctx *c, const double *kpt);
+
+gpu::StencilGeom geom(const lynx_ctx *c) { return geom_at(c, c->kpt); }
+
+// Stencil description of the context's grid with the Bloch phases of kpt.
+gpu::StencilGeom geom_at(const lynx_ctx *c, const double *kpt)
 {
     gpu::StencilGeom g{};
     g.nx = c->st.nx; g.ny = c->st.ny; g.nz = c->st.nz; g.R = c->st.fdn;
@@ -193,7 +198,7 @@ gpu::StencilGeom geom(const lynx_ctx *c)
     g.alpha[1] = c->lat.orth ? 0.0 : 2.0 * T[2] / g.h[0];
     g.alpha[2] = c->lat.orth ? 0.0 : 2.0 * T[5] / g.h[1];
     for (int a = 0; a < 3; ++a) {
-        const double th = c->kpt[a] * c->lat.L[a];
+        const double th = kpt[a] * c->lat.L[a];
         g.phase[a][0][0] = std::cos(th);
         g.phase[a][0][1] = -std::sin(th);
         g.phase[a][1][0] = std::cos(th);
@@ -1198,3 +1203,263 @@ lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------- batches --
+// Many independent filter units (k-points of one structure, several
+// structures with their own grids) filtered together: every Chebyshev step
+// is one launch per kernel for all units (gather, stencil, scatter per
+// group of like units), and the whole filter is captured once into a HIP
+// graph and replayed.
+struct lynx_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    struct Unit {
+        lynx_ctx *ctx;
+        double kpt[3];
+        DevBuf ph64, ph32;     // image phases e^{-i k.R} of this unit's k-point
+        DevBuf work, dx, dy;   // recurrence work block; staging for host buffers
+    };
+    std::vector<Unit> units;
+    // plan and graph of the last call; rebuilt when buffers, shapes or dtype change
+    std::vector<long> key;
+    gpu::BatchPlan *plan = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_degree = 0;
+    bool use_graph = true;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double last_ms = 0;
+    int last_launches = 0;
+    ~lynx_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        gpu::batch_plan_destroy(plan);
+        for (auto &u : units) {
+            u.ph64.release(); u.ph32.release(); u.work.release(); u.dx.release(); u.dy.release();
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+template <class F>
+lynx_status bguarded(lynx_batch *b, F &&f)
+{
+    try {
+        HIP_OK(hipSetDevice(b->device));
+        f();
+        return LYNX_OK;
+    } catch (const Error &e) {
+        b->err = e.msg;
+        return e.status;
+    } catch (const std::exception &e) {
+        b->err = e.what();
+        return LYNX_E_HIP;
+    }
+}
+}  // namespace
+
+lynx_status lynx_batch_create(lynx_batch **out, int nunit, lynx_ctx *const *ctxs, const double *kpts)
+{
+    if (!out || nunit <= 0 || !ctxs) return LYNX_E_ARG;
+    *out = nullptr;
+    auto *b = new lynx_batch();
+    b->device = ctxs[0] ? ctxs[0]->device : -1;
+    const lynx_status st = bguarded(b, [&] {
+        for (int u = 0; u < nunit; ++u) {
+            lynx_ctx *c = ctxs[u];
+            if (!c) fail(LYNX_E_ARG, "null context in batch");
+            need_gpu_stencil(c);
+            need_veff(c);
+            if (c->device != b->device) fail(LYNX_E_ARG, "batch contexts must share one device");
+        }
+        HIP_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&b->e0));
+        HIP_OK(hipEventCreate(&b->e1));
+        b->units.resize(nunit);
+        for (int u = 0; u < nunit; ++u) {
+            lynx_batch::Unit &U = b->units[u];
+            U.ctx = ctxs[u];
+            for (int a = 0; a < 3; ++a) U.kpt[a] = kpts ? kpts[3 * u + a] : 0.0;
+            const NonlocalHost &h = U.ctx->nlh;
+            std::vector<double> ph(2 * std::max(1, h.n_img));
+            for (int i = 0; i < h.n_img; ++i) {
+                const double *sh = &h.img_shift[3 * i];
+                const double th = -(U.kpt[0] * sh[0] + U.kpt[1] * sh[1] + U.kpt[2] * sh[2]);
+                ph[2 * i] = std::cos(th);
+                ph[2 * i + 1] = std::sin(th);
+            }
+            upload(U.ph64, ph, b->stream);
+            upload(U.ph32, std::vector<float>(ph.begin(), ph.end()), b->stream);
+        }
+        HIP_OK(hipStreamSynchronize(b->stream));
+    });
+    if (st != LYNX_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return LYNX_OK;
+}
+
+lynx_status lynx_batch_destroy(lynx_batch *b)
+{
+    if (!b) return LYNX_E_ARG;
+    if (b->device >= 0) (void)hipSetDevice(b->device);
+    delete b;
+    return LYNX_OK;
+}
+
+const char *lynx_batch_last_error(const lynx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+lynx_status lynx_batch_set_graph(lynx_batch *b, int enable)
+{
+    if (!b) return LYNX_E_ARG;
+    b->use_graph = enable != 0;
+    return LYNX_OK;
+}
+
+lynx_status lynx_batch_last_timing(const lynx_batch *b, double *ms, int *launches)
+{
+    if (!b) return LYNX_E_ARG;
+    if (ms) *ms = b->last_ms;
+    if (launches) *launches = b->last_launches;
+    return LYNX_OK;
+}
+
+lynx_status lynx_batch_filter(lynx_batch *b, int dtype, void *const *x, const long *ldx, void *const *y,
+                              const long *ldy, const int *ncol, int degree, const double *lambda_c,
+                              const double *eig_min, const double *eig_max, int mem)
+{
+    if (!b) return LYNX_E_ARG;
+    return bguarded(b, [&] {
+        check_dtype(dtype);
+        const int n = (int)b->units.size();
+        if (!x || !y || !ncol || !lambda_c || !eig_min || !eig_max) fail(LYNX_E_ARG, "null batch argument");
+        if (degree < 1) fail(LYNX_E_ARG, "degree must be >= 1");
+        const size_t es = gpu::elem_size(dtype);
+        const bool single = gpu::elem_single(dtype);
+        const bool cplx = gpu::elem_complex(dtype);
+        // the contexts' own set-up work (Veff, projectors) must be complete
+        for (auto &U : b->units) HIP_OK(hipStreamSynchronize(U.ctx->stream));
+        std::vector<gpu::BatchUnit> bu(n);
+        std::vector<long> key = {dtype, degree % 2, mem};
+        for (int u = 0; u < n; ++u) {
+            lynx_batch::Unit &U = b->units[u];
+            lynx_ctx *c = U.ctx;
+            const long nd = c->nd;
+            if (ncol[u] <= 0 || !x[u] || !y[u]) fail(LYNX_E_ARG, "bad block of batch unit " + std::to_string(u));
+            const long lx = (ldx && ldx[u]) ? ldx[u] : nd, ly = (ldy && ldy[u]) ? ldy[u] : nd;
+            if (lx < nd || ly < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+            const long ldw = padded_ld(nd);
+            void *w = U.work.get((size_t)ldw * ncol[u] * es);
+            const void *X = x[u];
+            void *Y = y[u];
+            long lX = lx, lY = ly;
+            if (mem != LYNX_MEM_DEVICE) {
+                void *dx = U.dx.get((size_t)nd * ncol[u] * es);
+                Y = U.dy.get((size_t)nd * ncol[u] * es);
+                HIP_OK(hipMemcpy2DAsync(dx, nd * es, x[u], lx * es, nd * es, ncol[u], hipMemcpyHostToDevice,
+                                        b->stream));
+                X = dx;
+                lX = lY = nd;
+            }
+            gpu::BatchUnit &B = bu[u];
+            const double k0[3] = {0, 0, 0};
+            B.g = geom_at(c, cplx ? U.kpt : k0);
+            B.veff = single ? c->veff32.p : c->veff64.p;
+            if (c->has_nl) {
+                B.nl = single ? c->nl32 : c->nl64;
+                B.nl.phase = single ? U.ph32.p : U.ph64.p;
+            }
+            // odd steps write D1, even steps D2; the last step lands in Y
+            void *D1 = (degree % 2 == 1) ? Y : w;
+            void *D2 = (degree % 2 == 1) ? w : Y;
+            B.buf[0] = X; B.buf[1] = D1; B.buf[2] = D2;
+            B.ld[0] = lX;
+            B.ld[1] = (degree % 2 == 1) ? lY : ldw;
+            B.ld[2] = (degree % 2 == 1) ? ldw : lY;
+            B.ncol = ncol[u];
+            key.push_back(reinterpret_cast<long>(X));
+            key.push_back(reinterpret_cast<long>(Y));
+            key.push_back(lX);
+            key.push_back(lY);
+            key.push_back(ncol[u]);
+        }
+        if (key != b->key || !b->plan) {
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            gpu::batch_plan_destroy(b->plan);
+            b->plan = nullptr;
+            b->plan = gpu::batch_plan_create(dtype, bu.data(), n, b->stream);
+            b->key = key;
+        }
+        // filter scalars of every step and unit (EigenSolver.cpp:558-619)
+        std::vector<double> coef((size_t)degree * n * 3);
+        for (int u = 0; u < n; ++u) {
+            const double ehalf = (eig_max[u] - lambda_c[u]) / 2.0;
+            const double cen = (eig_max[u] + lambda_c[u]) / 2.0;
+            if (!(ehalf > 0)) fail(LYNX_E_ARG, "eig_max must exceed lambda_cutoff (unit " + std::to_string(u) + ")");
+            const double sigma1 = ehalf / (eig_min[u] - cen);
+            double sigma = sigma1;
+            coef[u * 3 + 0] = sigma / ehalf;
+            coef[u * 3 + 1] = 0.0;
+            coef[u * 3 + 2] = -cen;
+            for (int k = 2; k <= degree; ++k) {
+                const double sn = 1.0 / (2.0 / sigma1 - sigma);
+                double *cf = &coef[((size_t)(k - 1) * n + u) * 3];
+                cf[0] = 2.0 * sn / ehalf;
+                cf[1] = -sigma * sn;
+                cf[2] = -cen;
+                sigma = sn;
+            }
+        }
+        gpu::batch_set_coef(b->plan, coef.data(), degree, b->stream);
+        auto record = [&] {
+            gpu::batch_step(b->plan, 0, 0, 3, 1, b->stream);
+            int prev = 1;
+            for (int k = 2; k <= degree; ++k) {
+                const int dst = (k % 2 == 1) ? 1 : 2;
+                const int old = (k == 2) ? 0 : dst;
+                gpu::batch_step(b->plan, k - 1, prev, old, dst, b->stream);
+                prev = dst;
+            }
+        };
+        HIP_OK(hipEventRecord(b->e0, b->stream));
+        if (b->use_graph) {
+            if (!b->graph || b->graph_degree != degree) {
+                if (b->graph) (void)hipGraphExecDestroy(b->graph);
+                b->graph = nullptr;
+                hipGraph_t g = nullptr;
+                HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                record();
+                HIP_OK(hipStreamEndCapture(b->stream, &g));
+                const hipError_t r = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_OK(r);
+                b->graph_degree = degree;
+            }
+            HIP_OK(hipGraphLaunch(b->graph, b->stream));
+        } else {
+            record();
+        }
+        HIP_OK(hipEventRecord(b->e1, b->stream));
+        if (mem != LYNX_MEM_DEVICE) {
+            for (int u = 0; u < n; ++u) {
+                lynx_batch::Unit &U = b->units[u];
+                const long nd = U.ctx->nd;
+                const long ly = (ldy && ldy[u]) ? ldy[u] : nd;
+                HIP_OK(hipMemcpy2DAsync(y[u], ly * es, U.dy.p, nd * es, nd * es, ncol[u], hipMemcpyDeviceToHost,
+                                        b->stream));
+            }
+        }
+        HIP_OK(hipStreamSynchronize(b->stream));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, b->e0, b->e1));
+        b->last_ms = ms;
+        b->last_launches = degree * gpu::batch_launches_per_step(b->plan);
+    });
+}

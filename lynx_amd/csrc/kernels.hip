@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <type_traits>
+#include <vector>
+#include <string>
 
 namespace lynx {
 namespace gpu {
@@ -116,6 +118,24 @@ struct StepArgs {
     A ph[3][2][2];
 };
 
+// Per-launch operands of a step (buffers and recurrence scalars), kept
+// apart from the operator record so that a batch can read the record from
+// device memory while these arrive as kernel arguments.
+template <class A>
+struct StepIO {
+    const void *x;
+    const void *old;
+    void *out;
+    long ldx, ldo, ldout;
+    A a, b, s;
+};
+
+template <class A>
+__host__ __device__ inline StepIO<A> step_io(const StepArgs<A> &p)
+{
+    return StepIO<A>{p.x, p.old, p.out, p.ldx, p.ldo, p.ldout, p.a, p.b, p.s};
+}
+
 // Loads of one z-step, issued one step ahead of their use.
 template <class S, int RP, int HP>
 struct StepLoads {
@@ -148,7 +168,7 @@ __device__ __forceinline__ int xcd_work_item()
 }
 
 template <class S, int R, int RP, int NT, int HP, int W>
-__device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>::A> &p, const int w)
+__device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>::A> &p, const StepIO<typename Tr<S>::A> io, const int w)
 {
     using A = typename Tr<S>::A;
     constexpr bool C = Tr<S>::C;
@@ -175,10 +195,10 @@ __device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>:
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
-    const bool has_old = p.b != A(0);
+    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const bool has_old = io.b != A(0);
 
     const int tid = threadIdx.x;
     const int npts = rows * nx;
@@ -303,9 +323,9 @@ __device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>:
                 lap = fm(p.c[1][k], add(c[k * W], c[-k * W]), lap);
                 lap = fm(p.c[2][k], add(ring[r][R + k], ring[r][R - k]), lap);
             }
-            const S hval = fm(cur.v[r] + p.s, ring[r][R], scl(A(-0.5), lap));
-            S res = scl(p.a, hval);
-            if (has_old) res = fm(p.b, cur.old[r], res);
+            const S hval = fm(cur.v[r] + io.s, ring[r][R], scl(A(-0.5), lap));
+            S res = scl(io.a, hval);
+            if (has_old) res = fm(io.b, cur.old[r], res);
             if (qv[r]) st_at(Yz, qoff[r], res);
         }
 #pragma unroll
@@ -330,7 +350,7 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_step_body<S, R, RP, NT, HP, W>(p, w);
+    stencil_step_body<S, R, RP, NT, HP, W>(p, step_io(p), w);
 }
 
 // --------------------------------------- fused stencil, non-orthogonal cell --
@@ -366,7 +386,7 @@ struct NonorthArgs {
 };
 
 template <class S, int R, int RP, int NT, int HP, int W>
-__device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename Tr<S>::A> &q, const int w)
+__device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename Tr<S>::A> &q, const StepIO<typename Tr<S>::A> io, const int w)
 {
     using A = typename Tr<S>::A;
     constexpr bool C = Tr<S>::C;
@@ -389,10 +409,10 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
-    const bool has_old = p.b != A(0);
+    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const bool has_old = io.b != A(0);
 
     const int tid = threadIdx.x;
     const int npts = rows * nx;
@@ -516,7 +536,7 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
                 sy1 = fm(q.w1[k], add(yp, scl(A(-1), ym)), sy1);
             }
             // in-plane second differences, the potential term (-2 (V + s) x)
-            S a2 = fm(diag - A(2) * (cur.v[r] + p.s), xf[r], scl(q.beta[0], sx2));
+            S a2 = fm(diag - A(2) * (cur.v[r] + io.s), xf[r], scl(q.beta[0], sx2));
             inpl[r] = fm(q.beta[1], sy2, a2);
             hz[r] = scl(q.ihz, fm(q.axz, sx1, scl(q.ayz, sy1)));
             xzs[r] = scl(q.beta[2], xf[r]);
@@ -545,8 +565,8 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
                 }
             }
             if (out) {
-                S res = scl(A(-0.5) * p.a, acc[r][0]);
-                if (has_old) res = fm(p.b, cur.old[r], res);
+                S res = scl(A(-0.5) * io.a, acc[r][0]);
+                if (has_old) res = fm(io.b, cur.old[r], res);
                 if (qv[r]) st_at(Yz, qoff[r], res);
             }
 #pragma unroll
@@ -570,7 +590,7 @@ __global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<
 {
     const int w = xcd_work_item();
     if (w >= q.b.nwork) return;
-    stencil_nonorth_body<S, R, RP, NT, HP, W>(q, w);
+    stencil_nonorth_body<S, R, RP, NT, HP, W>(q, step_io(q.b), w);
 }
 
 // ------------------------------------------------ fused stencil, 16-B runs --
@@ -1003,14 +1023,13 @@ struct NlArgs {
 };
 
 template <class S, int NPM>
-__global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, const S *__restrict__ X,
-                                                   long ldx, S *__restrict__ beta, int ncol)
+__device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ X,
+                                               long ldx, S *__restrict__ beta, int ncol, int at, int cgroup)
 {
     using A = typename Tr<S>::A;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int at = blockIdx.x;
-    const int n = blockIdx.y * 4 + wave;
+    const int n = cgroup * 4 + wave;
     if (n >= ncol) return;
     const int pb = p.atom_pt_off[at], pe = p.atom_pt_off[at + 1];
     const S *Xn = X + (long)n * ldx;
@@ -1045,20 +1064,27 @@ __global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, 
         if (lane == j) bn[j] = scl(p.gamma[at * NPM + j], acc[j]);
 }
 
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, const S *__restrict__ X,
+                                                   long ldx, S *__restrict__ beta, int ncol)
+{
+    nl_gather_body<S, NPM>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
+}
+
 // -------------------------------------------------------- nonlocal scatter --
 // Owner-computes: one lane per covered grid point, one wave per column; sums
 // chi . beta over every (image, point) entry at that grid point, so no two
 // lanes write the same address and the result is deterministic.
 template <class S, int NPM>
-__global__ void __launch_bounds__(256) k_nl_scatter(NlArgs<typename Tr<S>::A> p,
-                                                    const S *__restrict__ beta, S *__restrict__ Y,
-                                                    long ldy, int ncol, typename Tr<S>::A a)
+__device__ __forceinline__ void nl_scatter_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ beta,
+                                                S *__restrict__ Y, long ldy, int ncol, typename Tr<S>::A a,
+                                                int cblock, int cgroup)
 {
     using A = typename Tr<S>::A;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int u = blockIdx.x * 64 + lane;
-    const int n = blockIdx.y * 4 + wave;
+    const int u = cblock * 64 + lane;
+    const int n = cgroup * 4 + wave;
     if (u >= p.n_cov || n >= ncol) return;
     const S *bn = beta + (long)n * p.n_atom * NPM;
     S acc = zero<S>();
@@ -1077,6 +1103,84 @@ __global__ void __launch_bounds__(256) k_nl_scatter(NlArgs<typename Tr<S>::A> p,
     }
     const long gi = (long)n * ldy + p.cov_gpos[u];
     Y[gi] = fm(a, acc, Y[gi]);
+}
+
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_scatter(NlArgs<typename Tr<S>::A> p,
+                                                    const S *__restrict__ beta, S *__restrict__ Y,
+                                                    long ldy, int ncol, typename Tr<S>::A a)
+{
+    nl_scatter_body<S, NPM>(p, beta, Y, ldy, ncol, a, blockIdx.x, blockIdx.y);
+}
+
+// ------------------------------------------------------- batched launches --
+// A batch holds many independent units (k-points of one structure, or
+// different structures with their own grids): each unit has its own
+// operator record, buffers and filter coefficients, and one launch of each
+// kernel covers every unit.  Work items of a unit are contiguous, so the
+// XCD-aware order keeps a unit's bands on one XCD; a table maps each work
+// item to its unit.
+template <class A>
+struct BatchRec {
+    NonorthArgs<A> q;           // operator of the unit; pointers and a, b, s set per step
+    const void *buf[4];         // X, D1, D2, none
+    long ld[4];
+    int item0;                  // first stencil work item
+    int g_item0, s_item0;       // first gather / scatter block
+    int ncol;
+    long beta_off;              // first beta element of the unit
+    NlArgs<A> nl;
+};
+
+template <class S, int R, int RP, int NT, int HP, int W, bool NONORTH>
+__global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
+                                                      const int *__restrict__ item_unit, int nitems,
+                                                      const typename Tr<S>::A *__restrict__ coef, int ix,
+                                                      int iold, int iout)
+{
+    using A = typename Tr<S>::A;
+    const int w = xcd_work_item();
+    if (w >= nitems) return;
+    const int u = item_unit[w];
+    const BatchRec<A> &r = recs[u];
+    const StepIO<A> io{r.buf[ix], r.buf[iold], const_cast<void *>(r.buf[iout]), r.ld[ix], r.ld[iold], r.ld[iout],
+                       coef[3 * u], coef[3 * u + 1], coef[3 * u + 2]};
+    if constexpr (NONORTH) stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
+    else stencil_step_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
+}
+
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
+                                                         const int *__restrict__ item_unit, int nitems,
+                                                         S *__restrict__ beta, int ix)
+{
+    using A = typename Tr<S>::A;
+    const int w = blockIdx.x;
+    if (w >= nitems) return;
+    const int u = item_unit[w];
+    const BatchRec<A> &r = recs[u];
+    const NlArgs<A> nl = r.nl;
+    const int lb = w - r.g_item0;
+    nl_gather_body<S, NPM>(nl, reinterpret_cast<const S *>(r.buf[ix]), r.ld[ix], beta + r.beta_off, r.ncol,
+                           lb % nl.n_atom, lb / nl.n_atom);
+}
+
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_scatter_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
+                                                          const int *__restrict__ item_unit, int nitems,
+                                                          const S *__restrict__ beta,
+                                                          const typename Tr<S>::A *__restrict__ coef, int iout)
+{
+    using A = typename Tr<S>::A;
+    const int w = blockIdx.x;
+    if (w >= nitems) return;
+    const int u = item_unit[w];
+    const BatchRec<A> &r = recs[u];
+    const NlArgs<A> nl = r.nl;
+    const int lb = w - r.s_item0;
+    const int ncb = (nl.n_cov + 63) / 64;
+    nl_scatter_body<S, NPM>(nl, beta + r.beta_off, reinterpret_cast<S *>(const_cast<void *>(r.buf[iout])),
+                            r.ld[iout], r.ncol, coef[3 * u], lb % ncb, lb / ncb);
 }
 
 // ---------------------------------------------------------------- GEMMs --
@@ -1601,6 +1705,271 @@ void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int
         else scatter_npm<S, 32>(nl, beta, y, ldy, ncol, a, st);
     });
 }
+
+// ------------------------------------------------------------ batch plan --
+struct BatchPlan {
+    int e = 0, n = 0;
+    void *recs = nullptr;
+    struct SGroup {
+        bool orth = true;
+        int W = 64;
+        int *items = nullptr;
+        int nitems = 0;
+        int max_tyr = 0;
+        bool small_halo = true;   // 2R nx <= 3 NT for every unit: 3 halo slots per thread, else 6
+    };
+    struct NGroup {
+        int npm = 8;
+        int *gitems = nullptr, *sitems = nullptr;
+        int ngitems = 0, nsitems = 0;
+    };
+    std::vector<SGroup> sg;
+    std::vector<NGroup> ng;
+    void *beta = nullptr;
+    void *coef = nullptr;
+    int coef_steps = 0;
+};
+
+namespace {
+void hip_check(hipError_t r, const char *what)
+{
+    if (r != hipSuccess) fail(2, std::string(what) + ": " + hipGetErrorString(r));
+}
+
+template <class T>
+T *dev_upload(const std::vector<T> &v, hipStream_t st)
+{
+    T *d = nullptr;
+    hip_check(hipMalloc(reinterpret_cast<void **>(&d), std::max<size_t>(1, v.size() * sizeof(T))), "hipMalloc");
+    if (!v.empty()) hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "copy");
+    return d;
+}
+
+// points per thread of the batched stencil shapes (256 threads per group)
+template <class S> constexpr int batch_rp_orth() { return base_rp<S>() / 2 > 0 ? base_rp<S>() / 2 : 1; }
+template <class S> constexpr int batch_rp_nonorth() { return base_rp<S>() / 4 > 0 ? base_rp<S>() / 4 : 1; }
+constexpr int kBatchNT = 256;
+
+template <class S>
+void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
+{
+    using A = typename Tr<S>::A;
+    std::vector<BatchRec<A>> recs(n);
+    // stencil groups by (orth, pitch), nonlocal groups by npm
+    std::vector<int> sgi(n), ngi(n, -1);
+    for (int u = 0; u < n; ++u) {
+        const StencilGeom &g = units[u].g;
+        const int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+        if (g.nx + 2 * kMaxR > 128) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
+        const bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
+        int k = -1;
+        for (size_t i = 0; i < P->sg.size(); ++i)
+            if (P->sg[i].orth == g.orth && P->sg[i].W == W && P->sg[i].small_halo == sh) k = (int)i;
+        if (k < 0) {
+            P->sg.push_back({g.orth, W, nullptr, 0, 0, sh});
+            k = (int)P->sg.size() - 1;
+        }
+        sgi[u] = k;
+        if (units[u].nl.n_atom > 0 && units[u].nl.n_pts > 0) {
+            int j = -1;
+            for (size_t i = 0; i < P->ng.size(); ++i)
+                if (P->ng[i].npm == units[u].nl.npm) j = (int)i;
+            if (j < 0) {
+                P->ng.push_back({units[u].nl.npm, nullptr, nullptr, 0, 0});
+                j = (int)P->ng.size() - 1;
+            }
+            ngi[u] = j;
+        }
+    }
+    std::vector<std::vector<int>> sitems(P->sg.size()), gitems(P->ng.size()), citems(P->ng.size());
+    long beta_off = 0;
+    for (int u = 0; u < n; ++u) {
+        const BatchUnit &U = units[u];
+        const StencilGeom &g = U.g;
+        BatchRec<A> &r = recs[u];
+        NonorthArgs<A> &q = r.q;
+        StepArgs<A> &p = q.b;
+        const long pts = (long)kBatchNT * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
+        if (g.nx > pts || 2 * kMaxR * g.nx > kBatchNT * 6)
+            fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
+        p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = U.ncol;
+        int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
+        nb = std::max(1, std::min(nb, g.ny));
+        p.tyr = (g.ny + nb - 1) / nb;
+        while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+        p.nbands = (g.ny + p.tyr - 1) / p.tyr;
+        p.zch = g.nz;
+        p.nzch = 1;
+        p.nwork = p.nbands * U.ncol;
+        p.veff = static_cast<const A *>(U.veff);
+        for (int ax = 0; ax < 3; ++ax)
+            for (int k = 0; k <= kMaxR; ++k) p.c[ax][k] = (A)g.c[ax][k];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int sd = 0; sd < 2; ++sd)
+                for (int k = 0; k < 2; ++k) p.ph[ax][sd][k] = (A)g.phase[ax][sd][k];
+        for (int k = 0; k <= kMaxR; ++k) {
+            q.w1[k] = (A)g.w1[k];
+            q.w2[k] = (A)g.w2[k];
+        }
+        for (int ax = 0; ax < 3; ++ax) q.beta[ax] = (A)g.beta[ax];
+        q.axy_hy = (A)(g.alpha[0] / g.h[1]);
+        q.axz = (A)g.alpha[1];
+        q.ayz = (A)g.alpha[2];
+        q.ihz = (A)(1.0 / g.h[2]);
+        for (int i = 0; i < 3; ++i) {
+            r.buf[i] = U.buf[i];
+            r.ld[i] = U.ld[i];
+        }
+        r.buf[3] = nullptr;
+        r.ld[3] = 0;
+        r.ncol = U.ncol;
+        P->sg[sgi[u]].max_tyr = std::max(P->sg[sgi[u]].max_tyr, p.tyr);
+        auto &si = sitems[sgi[u]];
+        r.item0 = (int)si.size();
+        for (int w = 0; w < p.nwork; ++w) si.push_back(u);
+        const NlDev &nl = U.nl;
+        r.nl = NlArgs<A>{nl.atom_pt_off, nl.pt_gpos, nl.pt_img, nl.pt_atom, nl.cov_off, nl.cov_gpos, nl.ent_pt,
+                         (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
+        r.beta_off = beta_off;
+        r.g_item0 = r.s_item0 = 0;
+        if (ngi[u] >= 0) {
+            const int cg = (U.ncol + 3) / 4;
+            auto &gi = gitems[ngi[u]];
+            r.g_item0 = (int)gi.size();
+            for (int w = 0; w < nl.n_atom * cg; ++w) gi.push_back(u);
+            auto &ci = citems[ngi[u]];
+            r.s_item0 = (int)ci.size();
+            for (int w = 0; w < ((nl.n_cov + 63) / 64) * cg; ++w) ci.push_back(u);
+            beta_off += (long)U.ncol * nl.n_atom * nl.npm;
+        }
+    }
+    P->recs = dev_upload(recs, st);
+    for (size_t i = 0; i < P->sg.size(); ++i) {
+        P->sg[i].nitems = (int)sitems[i].size();
+        P->sg[i].items = dev_upload(sitems[i], st);
+    }
+    for (size_t i = 0; i < P->ng.size(); ++i) {
+        P->ng[i].ngitems = (int)gitems[i].size();
+        P->ng[i].gitems = dev_upload(gitems[i], st);
+        P->ng[i].nsitems = (int)citems[i].size();
+        P->ng[i].sitems = dev_upload(citems[i], st);
+    }
+    hip_check(hipMalloc(&P->beta, std::max<size_t>(1, beta_off * sizeof(S))), "hipMalloc");
+    hip_check(hipStreamSynchronize(st), "batch upload");
+}
+
+template <class S, int W, int HP>
+void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const void *coef, int ix, int iold,
+                          int iout, hipStream_t st)
+{
+    using A = typename Tr<S>::A;
+    const int blocks = ((G.nitems + 7) / 8) * 8;
+    const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
+    const A *cf = static_cast<const A *>(coef);
+    const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
+    if (G.orth)
+        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, false>), dim3(blocks),
+                           dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+    else
+        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, true>), dim3(blocks),
+                           dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+}
+
+template <class S, int NPM>
+void launch_nl_batch(const BatchPlan *P, const BatchPlan::NGroup &G, const void *coef, int ix, int iout,
+                     bool gather, hipStream_t st)
+{
+    using A = typename Tr<S>::A;
+    const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
+    if (gather) {
+        if (G.ngitems > 0)
+            hipLaunchKernelGGL((k_nl_gather_batch<S, NPM>), dim3(G.ngitems), dim3(256), 0, st, recs, G.gitems,
+                               G.ngitems, static_cast<S *>(P->beta), ix);
+    } else if (G.nsitems > 0) {
+        hipLaunchKernelGGL((k_nl_scatter_batch<S, NPM>), dim3(G.nsitems), dim3(256), 0, st, recs, G.sitems,
+                           G.nsitems, static_cast<const S *>(P->beta), static_cast<const A *>(coef), iout);
+    }
+}
+}  // namespace
+
+BatchPlan *batch_plan_create(int e, const BatchUnit *units, int n, hipStream_t st)
+{
+    auto *P = new BatchPlan();
+    P->e = e;
+    P->n = n;
+    try {
+        dispatch(e, [&](auto t) { build_batch<decltype(t)>(P, units, n, st); });
+    } catch (...) {
+        batch_plan_destroy(P);
+        throw;
+    }
+    return P;
+}
+
+void batch_plan_destroy(BatchPlan *P)
+{
+    if (!P) return;
+    (void)hipFree(P->recs);
+    for (auto &g : P->sg) (void)hipFree(g.items);
+    for (auto &g : P->ng) {
+        (void)hipFree(g.gitems);
+        (void)hipFree(g.sitems);
+    }
+    (void)hipFree(P->beta);
+    (void)hipFree(P->coef);
+    delete P;
+}
+
+void batch_set_coef(BatchPlan *P, const double *coef, int steps, hipStream_t st)
+{
+    const size_t cnt = (size_t)steps * P->n * 3;
+    const bool single = elem_single(P->e);
+    const size_t bytes = cnt * (single ? sizeof(float) : sizeof(double));
+    if (steps > P->coef_steps) {
+        (void)hipFree(P->coef);
+        P->coef = nullptr;
+        hip_check(hipMalloc(&P->coef, bytes), "hipMalloc");
+        P->coef_steps = steps;
+    }
+    if (single) {
+        std::vector<float> f(coef, coef + cnt);
+        hip_check(hipMemcpyAsync(P->coef, f.data(), bytes, hipMemcpyHostToDevice, st), "copy");
+        hip_check(hipStreamSynchronize(st), "copy");
+    } else {
+        hip_check(hipMemcpyAsync(P->coef, coef, bytes, hipMemcpyHostToDevice, st), "copy");
+    }
+}
+
+void batch_step(BatchPlan *P, int step, int ix, int iold, int iout, hipStream_t st)
+{
+    if (step >= P->coef_steps) fail(1, "batch step beyond the uploaded filter coefficients");
+    dispatch(P->e, [&](auto t) {
+        using S = decltype(t);
+        using A = typename Tr<S>::A;
+        const void *cf = static_cast<const A *>(P->coef) + (size_t)step * P->n * 3;
+        for (const auto &G : P->ng) {
+            if (G.npm == 8) launch_nl_batch<S, 8>(P, G, cf, ix, iout, true, st);
+            else if (G.npm == 16) launch_nl_batch<S, 16>(P, G, cf, ix, iout, true, st);
+            else launch_nl_batch<S, 32>(P, G, cf, ix, iout, true, st);
+        }
+        for (const auto &G : P->sg) {
+            if (G.W == 64) {
+                if (G.small_halo) launch_stencil_batch<S, 64, 3>(P, G, cf, ix, iold, iout, st);
+                else launch_stencil_batch<S, 64, 6>(P, G, cf, ix, iold, iout, st);
+            } else {
+                if (G.small_halo) launch_stencil_batch<S, 128, 3>(P, G, cf, ix, iold, iout, st);
+                else launch_stencil_batch<S, 128, 6>(P, G, cf, ix, iold, iout, st);
+            }
+        }
+        for (const auto &G : P->ng) {
+            if (G.npm == 8) launch_nl_batch<S, 8>(P, G, cf, ix, iout, false, st);
+            else if (G.npm == 16) launch_nl_batch<S, 16>(P, G, cf, ix, iout, false, st);
+            else launch_nl_batch<S, 32>(P, G, cf, ix, iout, false, st);
+        }
+    });
+}
+
+int batch_launches_per_step(const BatchPlan *P) { return (int)(P->sg.size() + 2 * P->ng.size()); }
 
 static int gram_splits(int nd, int n)
 {

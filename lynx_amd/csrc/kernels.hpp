@@ -72,6 +72,30 @@ void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int 
 void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
                 hipStream_t st);
 
+// ---- batched filter: many independent units, one launch per kernel ----
+// One unit: an operator (grid, coefficients, Bloch phases, Veff, nonlocal
+// data with this unit's image phases) and its three column blocks (input X
+// and the two work blocks D1, D2 of the recurrence).
+struct BatchUnit {
+    StencilGeom g;
+    const void *veff = nullptr;        // device, in the batch's precision
+    NlDev nl;                          // n_atom == 0: no nonlocal part
+    const void *buf[3] = {nullptr, nullptr, nullptr};
+    long ld[3] = {0, 0, 0};
+    int ncol = 0;
+};
+
+// Device tables of a batch for one element type (opaque to callers).
+struct BatchPlan;
+BatchPlan *batch_plan_create(int e, const BatchUnit *units, int n, hipStream_t st);
+void batch_plan_destroy(BatchPlan *p);
+// Filter scalars of every step: coef[(step * n + u) * 3 + {0, 1, 2}] = (a, b, s).
+void batch_set_coef(BatchPlan *p, const double *coef, int steps, hipStream_t st);
+// One recurrence step for every unit: buf[iout] = a (H - shift) buf[ix] + b buf[iold]
+// (iold = 3: no old term).  Launches gather, stencil and scatter per group.
+void batch_step(BatchPlan *p, int step, int ix, int iold, int iout, hipStream_t st);
+int batch_launches_per_step(const BatchPlan *p);
+
 // C (n x n, ld n) = alpha * A^H B for A, B (nd x n); split-K partials in work.
 size_t gram_work_bytes(int e, int nd, int n);
 void gram(int e, const void *A, long lda, const void *B, long ldb, int nd, int n, double alpha,

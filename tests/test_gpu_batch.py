@@ -1,0 +1,107 @@
+"""Batched filter (lynx_batch_*): many units in one launch per kernel per
+Chebyshev step, replayed from a HIP graph.  Every unit's result is checked
+against the oracle's filter of that unit alone (FP64 1e-11, FP32 2e-5), and
+graph replays against eager launches of the same plan (bitwise)."""
+import numpy as np
+import pytest
+
+import oracle
+from systems import A_SI, DIAMOND, System, displace, si2_fcc, si8
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def kvec(s, red):
+    L = np.sqrt((s.latvec ** 2).sum(axis=1))
+    return tuple(2 * np.pi * np.asarray(red) / L)
+
+
+def _bounds(i):
+    return 0.3 + 0.05 * i, -0.6 - 0.02 * i, 60.0 + 3.0 * i
+
+
+@pytest.fixture(scope="module")
+def fcc_kbatch():
+    # k-points of one FCC Si2 cell sharing one Hamiltonian
+    from lynx_amd import core
+    s = si2_fcc(16)
+    H = s.hamiltonian()
+    H.set_veff(s.veff)
+    reds = [(0.0, 0.0, 0.0), (0.25, 0.0, 0.0), (0.25, 0.25, 0.0), (0.5, 0.25, -0.25), (-0.125, 0.375, 0.5)]
+    ks = [kvec(s, r) for r in reds]
+    return s, H, ks, core
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_kpoint_batch_matches_per_kpoint_oracle(fcc_kbatch, dtype, tol):
+    s, H, ks, core = fcc_kbatch
+    bf = core.BatchFilter([(H, k) for k in ks])
+    blocks = [s.random_block(3 + (i % 2), 50 + i, complex_=True) for i in range(len(ks))]
+    deg = 8
+    lc, lo, hi = zip(*[_bounds(i) for i in range(len(ks))])
+    got = bf.filter(blocks, deg, lc, lo, hi, dtype=dtype)
+    op = s.oracle_op()
+    for i, k in enumerate(ks):
+        op.set_kpoint(k)
+        ref = op.cheb_filter(blocks[i], s.veff, lc[i], lo[i], hi[i], deg)
+        assert rel(got[i], ref) < tol, (i, k)
+    bf.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 2e-5)])
+def test_heterogeneous_batch_matches_oracle(dtype, tol):
+    # orthogonal and non-orthogonal cells, different grids and band counts,
+    # one without projectors
+    from lynx_amd import core
+    systems = [si8(20), si2_fcc(18), System((20, 22, 24), A_SI, displace(DIAMOND, A_SI, 0.05, seed=4)),
+               si8(16, with_nonlocal=False)]
+    Hs = []
+    for s in systems:
+        H = s.hamiltonian()
+        H.set_veff(s.veff)
+        Hs.append(H)
+    bf = core.BatchFilter([(H, None) for H in Hs])
+    blocks = [s.random_block(2 + i, 60 + i) for i, s in enumerate(systems)]
+    deg = 7
+    lc, lo, hi = zip(*[_bounds(i) for i in range(len(systems))])
+    got = bf.filter(blocks, deg, lc, lo, hi, dtype=dtype)
+    for i, s in enumerate(systems):
+        ref = s.oracle_op().cheb_filter(blocks[i], s.veff, lc[i], lo[i], hi[i], deg)
+        assert rel(got[i], ref) < tol, i
+    bf.close()
+
+
+def test_graph_replays_match_eager_launches(fcc_kbatch):
+    # the same plan run eagerly and from the captured graph, several calls
+    # in a row with different inputs and bounds and other GPU work between
+    s, H, ks, core = fcc_kbatch
+    units = [(H, k) for k in ks[:3]]
+    g = core.BatchFilter(units, graph=True)
+    e = core.BatchFilter(units, graph=False)
+    for rep in range(3):
+        blocks = [s.random_block(4, 70 + 10 * rep + i, complex_=True) for i in range(3)]
+        lc, lo, hi = zip(*[_bounds(i + rep) for i in range(3)])
+        yg = g.filter(blocks, 9, lc, lo, hi, dtype=np.complex128)
+        H.apply_kpt(H.halo, blocks[0].T.copy(), s.veff, ks[1])      # unrelated work on the context's stream
+        ye = e.filter(blocks, 9, lc, lo, hi, dtype=np.complex128)
+        for a, b in zip(yg, ye):
+            assert np.array_equal(a, b), rep
+    ms, launches = g.last_timing()
+    assert launches == 9 * 3 and ms > 0
+    g.close()
+    e.close()
+
+
+def test_batch_errors():
+    from lynx_amd import core, _lib as L
+    s = si8(16)
+    H = s.hamiltonian()
+    H.set_veff(s.veff)
+    bf = core.BatchFilter([(H, None)])
+    with pytest.raises(L.LynxError):
+        bf.filter([s.random_block(2, 1)], 5, [50.0], [-1.0], [40.0])   # eig_max below lambda_cutoff
+    bf.close()
