@@ -134,6 +134,167 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
     return res
 
 
+def _bounds(ctx, lib, L, dtype_code, steps):
+    lo = ctypes.c_double(); hi = ctypes.c_double(); it = ctypes.c_int()
+    L.check(lib.lynx_lanczos_bounds(ctx, dtype_code, ctypes.byref(lo), ctypes.byref(hi), 0.0, steps, 1,
+                                    ctypes.byref(it)), ctx)
+    return lo.value, hi.value
+
+
+def _batch_bytes(units, esize, vsize, degree):
+    """Algorithmic bytes of one batched filter: s*P*(3m-1) plus one Veff read
+    per unit and step."""
+    return sum(esize * nd * nc * (3 * degree - 1) + vsize * nd * degree for nd, nc in units)
+
+
+def _time_batch(bf, dtype, xs, ys, lds, ncols, lc, lo, hi, steps, warmup):
+    from lynx_amd.hipmem import synchronize
+    args = (dtype, [x.value for x in xs], [y.value for y in ys], lds, ncols, DEGREE, lc, lo, hi)
+    for _ in range(warmup):
+        bf.filter_device(*args)
+    synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bf.filter_device(*args)
+        dev.append(bf.last_timing()[0])
+    synchronize()
+    return 1e3 * (time.perf_counter() - t0) / steps, float(np.median(dev)), bf.last_timing()[1]
+
+
+def run_kpoint_batch(steps, warmup):
+    """Batched k-points: Si2 FCC primitive cell (a = 10.26 Bohr), 24^3 grid,
+    6x6x6 Monkhorst-Pack grid reduced by time reversal to 112 k-points,
+    12 complex64 orbitals per k-point, all filtered in one graph-replayed
+    launch per kernel and step; against the same filter per k-point in
+    complex128 (the FP64 complex path) and in complex64, one call each."""
+    from systems import si2_fcc
+    from lynx_amd import _lib as L, core
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    lib = L.lib()
+    s = si2_fcc(24)
+    H = s.hamiltonian(0)
+    H.set_veff(s.veff)
+    lv = np.ascontiguousarray(s.latvec, float)
+    n = ctypes.c_int()
+    sh = np.zeros(3)
+    L.check(lib.lynx_kpoints(6, 6, 6, L.dptr(sh), L.dptr(lv), L.CELL_NONORTHOGONAL, None, None, None, 0,
+                             ctypes.byref(n)))
+    kc = np.zeros(3 * n.value)
+    L.check(lib.lynx_kpoints(6, 6, 6, L.dptr(sh), L.dptr(lv), L.CELL_NONORTHOGONAL, L.dptr(kc), None, None,
+                             n.value, ctypes.byref(n)))
+    ks = kc.reshape(-1, 3)
+    nk, nb, nd = len(ks), 12, s.nd
+    H.set_kpoint(ks[0])
+    lo, hi = _bounds(H.ctx, lib, L, L.C128, LANCZOS_STEPS)   # bounds at k-point 0 (SCF.cpp:883-935)
+    lc = lo + 0.25 * (hi - lo)
+    rng = np.random.default_rng(5)
+    X = [np.asfortranarray(rng.uniform(-0.5, 0.5, (nd, nb)) + 1j * rng.uniform(-0.5, 0.5, (nd, nb)))
+         for _ in range(nk)]
+    x64 = [DeviceArray.from_numpy(x.T.astype(np.complex64)) for x in X]
+    y64 = [DeviceArray(nd * nb * 8) for _ in range(nk)]
+    bf = core.BatchFilter([(H, k) for k in ks])
+    wall, dev, launches = _time_batch(bf, np.complex64, x64, y64, [nd] * nk, [nb] * nk, [lc] * nk, [lo] * nk,
+                                      [hi] * nk, steps, warmup)
+    B = _batch_bytes([(nd, nb)] * nk, 8, 4, DEGREE)
+    # the same work one k-point per call (complex64, then complex128)
+    per = {}
+    for code, xs, es in ((L.C64, x64, 8), (L.C128, None, 16)):
+        if xs is None:
+            xs = [DeviceArray.from_numpy(x.T.copy()) for x in X]
+        ys = [DeviceArray(nd * nb * es) for _ in range(nk)]
+        for rep in range(2):
+            synchronize()
+            t0 = time.perf_counter()
+            for i in range(nk):
+                H.set_kpoint(ks[i])
+                L.check(lib.lynx_cheb_filter(H.ctx, code, xs[i].value, nd, ys[i].value, nd, nb, DEGREE, lc, lo,
+                                             hi, L.MEM_DEVICE), H.ctx)
+            synchronize()
+        per[code] = (1e3 * (time.perf_counter() - t0), ys, xs)
+    # complex64 batch against the complex128 per-k-point filter
+    errs = []
+    for i in range(0, nk, 7):
+        a = np.zeros((nb, nd), np.complex64); y64[i].download(a)
+        b = np.zeros((nb, nd), np.complex128); per[L.C128][1][i].download(b)
+        errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    bf.close()
+    return {"workload": "Si2 FCC primitive cell a=10.26 Bohr, 24^3 grid, 6x6x6 MP -> %d k-points, %d complex64 "
+                        "orbitals per k-point, degree %d, bounds from %d Lanczos steps at k-point 0"
+                        % (nk, nb, DEGREE, LANCZOS_STEPS),
+            "ms_per_filter": round(wall, 4), "device_ms_per_filter": round(dev, 4),
+            "launches_per_filter": launches, "GBps": round(B / (dev * 1e-3) / 1e9, 1),
+            "roofline_frac": round(B / (dev * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "per_kpoint_c64_ms": round(per[L.C64][0], 3), "per_kpoint_c128_ms": round(per[L.C128][0], 3),
+            "speedup_vs_per_kpoint_c128": round(per[L.C128][0] / wall, 2),
+            "max_rel_err_c64_batch_vs_c128": max(errs)}
+
+
+def run_hetero_batch(steps, warmup):
+    """Heterogeneous batch: 32 displaced Si8 frames (30^3, 24 orbitals), 16
+    Si16 frames (30x30x60, 40), 8 BaTiO3 frames (28^3, 30; La/Zr/O psp8
+    stand-ins), 8 H2O frames (50^3, 8; O psp8, H in Veff only); sigma = 0.05
+    Bohr (seed 4 + frame).  Filtered together in FP64 and FP32; per-class
+    FP32-vs-FP64 error of the filtered blocks."""
+    from systems import System, A_SI, DIAMOND, displace, si16, batio3, h2o
+    from lynx_amd import _lib as L, core
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    lib = L.lib()
+    frames = [("Si8", System(30, A_SI, displace(DIAMOND, A_SI, 0.05, seed=4 + i)), 24) for i in range(32)]
+    frames += [("Si16", si16(36 + i), 40) for i in range(16)]
+    frames += [("BaTiO3", batio3(52 + i), 30) for i in range(8)]
+    frames += [("H2O", h2o(60 + i), 8) for i in range(8)]
+    t0 = time.perf_counter()
+    Hs, bnds = [], []
+    for _, s, nb in frames:
+        H = s.hamiltonian(0)
+        H.set_veff(s.veff)
+        Hs.append(H)
+        lo, hi = _bounds(H.ctx, lib, L, L.F64, LANCZOS_STEPS)
+        bnds.append((lo + 0.25 * (hi - lo), lo, hi))
+    setup_s = time.perf_counter() - t0
+    rng = np.random.default_rng(6)
+    X = [rng.uniform(-0.5, 0.5, (nb, s.nd)) for _, s, nb in frames]
+    lc, lo, hi = (list(v) for v in zip(*bnds))
+    res = {}
+    outs = {}
+    for dt, code, es in ((np.float64, L.F64, 8), (np.float32, L.F32, 4)):
+        xs = [DeviceArray.from_numpy(x.astype(dt)) for x in X]
+        ys = [DeviceArray(x.size * es) for x in X]
+        bf = core.BatchFilter([(H, None) for H in Hs])
+        wall, dev, launches = _time_batch(bf, dt, xs, ys, [s.nd for _, s, _ in frames], [nb for _, _, nb in frames],
+                                          lc, lo, hi, steps, warmup)
+        B = _batch_bytes([(s.nd, nb) for _, s, nb in frames], es, es, DEGREE)
+        res[code] = {"ms_per_filter": round(wall, 4), "device_ms_per_filter": round(dev, 4),
+                     "GBps": round(B / (dev * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(B / (dev * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "launches_per_filter": launches}
+        outs[code] = [y.download(np.zeros(x.shape, dt)) for y, x in zip(ys, X)]
+        bf.close()
+        for d in xs + ys:
+            d.free()
+    # one frame per call (FP64), the unbatched baseline
+    xs = [DeviceArray.from_numpy(x) for x in X]
+    ys = [DeviceArray(x.size * 8) for x in X]
+    synchronize()
+    t0 = time.perf_counter()
+    for i, (_, s, nb) in enumerate(frames):
+        L.check(lib.lynx_cheb_filter(Hs[i].ctx, L.F64, xs[i].value, s.nd, ys[i].value, s.nd, nb, DEGREE, lc[i], lo[i],
+                                     hi[i], L.MEM_DEVICE), Hs[i].ctx)
+    synchronize()
+    unbatched = 1e3 * (time.perf_counter() - t0)
+    err = {}
+    for i, (cls, _, _) in enumerate(frames):
+        a, b = outs[L.F32][i].astype(np.float64), outs[L.F64][i]
+        err[cls] = max(err.get(cls, 0.0), float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    P = sum(s.nd * nb for _, s, nb in frames)
+    return {"workload": "32 Si8 (30^3, 24) + 16 Si16 (30x30x60, 40) + 8 BaTiO3 (28^3, 30) + 8 H2O (50^3, 8), "
+                        "%d point-columns, degree %d" % (P, DEGREE),
+            "f64": res[L.F64], "f32": res[L.F32], "unbatched_f64_ms": round(unbatched, 3),
+            "speedup_f32_batch_vs_unbatched_f64": round(unbatched / res[L.F32]["ms_per_filter"], 2),
+            "max_rel_err_f32_vs_f64": err, "setup_s": round(setup_s, 2)}
+
+
 def cpu_baseline(sysm, X64, lam, lo, hi):
     """The oracle (single-threaded C restatement) on a bounded sample:
     the same grid, atoms and degree, 48 of the 160 columns."""
@@ -156,6 +317,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batches", action="store_true", help="skip the k-point and heterogeneous batch legs")
     ap.add_argument("--stencil-shape", default=None, metavar="VEC,THREADS,DIV,ZCHUNKS",
                     help="force the FP64 stencil launch shape instead of the autotuned one (e.g. 1,512,2,1)")
     args = ap.parse_args()
@@ -185,6 +347,12 @@ def main():
                            "roofline_frac": round(r32["stencil_gbs"] / HBM_PEAK_GBS, 4),
                            "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
                            "launch_shape": r32["stencil_config"]}
+
+    if not args.no_batches:
+        variants["c64_kpoint_batch"] = run_kpoint_batch(args.steps, args.warmup)
+        print(f"[bench] k-point batch: {variants['c64_kpoint_batch']}", file=sys.stderr)
+        variants["hetero_batch"] = run_hetero_batch(args.steps, args.warmup)
+        print(f"[bench] hetero batch: {variants['hetero_batch']}", file=sys.stderr)
 
     cpu = None
     if not args.no_cpu_baseline:

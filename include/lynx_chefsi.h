@@ -181,6 +181,13 @@ lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, doubl
 /* FDStencil::compute_weights (FDStencil.cpp:23-44): w1, w2 of fd_order/2+1. */
 lynx_status lynx_fd_weights(int fd_order, double *w1, double *w2);
 /* psp8 header summary: zion, lmax, lloc, nproj_per_atom, rc_max. */
+/* Monkhorst-Pack k-point grid k1 x k2 x k3 with shift, reduced by time
+ * reversal (replaces KPoints::generate, src/core/KPoints.cpp:26-90).
+ * kpt_cart: 3 per point, 2 pi k_red / L per lattice direction (the basis
+ * lynx_set_kpoint takes); kpt_red: reduced coordinates; weights sum to 1.
+ * *nkpt receives the count; pass capacity 0 and NULL arrays to query it. */
+lynx_status lynx_kpoints(int k1, int k2, int k3, const double shift[3], const double latvec[9], int cell,
+                         double *kpt_cart, double *kpt_red, double *weights, int capacity, int *nkpt);
 lynx_status lynx_psp8_info(const char *path, double *zion, int *lmax, int *lloc,
                            int *nproj_per_atom, double *rc_max);
 

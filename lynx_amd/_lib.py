@@ -49,6 +49,7 @@ SIGNATURES = {
     "lynx_lanczos_bounds": (_I, [_P, _I, _DP, _DP, _D, _I, ctypes.c_uint, _IP]),
     "lynx_fd_weights": (_I, [_I, _DP, _DP]),
     "lynx_psp8_info": (_I, [ctypes.c_char_p, _DP, _IP, _IP, _IP, _DP]),
+    "lynx_kpoints": (_I, [_I, _I, _I, _DP, _DP, _I, _DP, _DP, _DP, _I, _IP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
     "lynx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), _DP]),

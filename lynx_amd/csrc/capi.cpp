@@ -430,6 +430,13 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
             times.push_back(tmin);
             if (tmin < tbest) { tbest = tmin; best = k; }
         }
+        // among shapes within 1.5% of the fastest, take the one with the
+        // fewest z chunks (each chunk re-reads 2R planes): timing noise
+        // should not pick the shape with more HBM traffic
+        for (size_t i = 0; i < cands.size(); ++i)
+            if (times[i] <= 1.015 * tbest && cands[i].vec == best.vec && cands[i].nt == best.nt &&
+                cands[i].rp_div == best.rp_div && cands[i].nzch < best.nzch)
+                best = cands[i];
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c->tune_log[key] = times;
@@ -1118,6 +1125,29 @@ lynx_status lynx_fd_weights(int fd_order, double *w1, double *w2)
             if (w1) w1[i] = a[i];
             if (w2) w2[i] = b[i];
         }
+        return LYNX_OK;
+    } catch (const Error &e) {
+        return e.status;
+    }
+}
+
+lynx_status lynx_kpoints(int k1, int k2, int k3, const double shift[3], const double latvec[9], int cell,
+                         double *kpt_cart, double *kpt_red, double *weights, int capacity, int *nkpt)
+{
+    try {
+        if (!latvec || !nkpt) fail(LYNX_E_ARG, "null argument");
+        if (cell != LYNX_CELL_ORTHOGONAL && cell != LYNX_CELL_NONORTHOGONAL) fail(LYNX_E_ARG, "unknown cell type");
+        Lattice lat;
+        lattice_init(lat, latvec, cell == LYNX_CELL_ORTHOGONAL);
+        const double zero[3] = {0, 0, 0};
+        KPointSet ks;
+        kpoints_generate(k1, k2, k3, shift ? shift : zero, lat, ks);
+        const int n = (int)ks.weight.size();
+        *nkpt = n;
+        if (capacity < n) return (kpt_cart || kpt_red || weights) ? LYNX_E_ARG : LYNX_OK;
+        if (kpt_cart) std::copy(ks.cart.begin(), ks.cart.end(), kpt_cart);
+        if (kpt_red) std::copy(ks.red.begin(), ks.red.end(), kpt_red);
+        if (weights) std::copy(ks.weight.begin(), ks.weight.end(), weights);
         return LYNX_OK;
     } catch (const Error &e) {
         return e.status;

@@ -74,6 +74,14 @@ struct NonlocalHost {
     std::vector<int> ip_displ;         // n_atom+1
     std::vector<double> gamma;         // per projector
 };
+// Monkhorst-Pack k-points reduced by time reversal (KPoints.cpp:26-90):
+// k_cart = 2 pi k_red / L per lattice direction; a point whose negative is
+// already listed (modulo a reciprocal vector) doubles that point's weight.
+struct KPointSet {
+    std::vector<double> cart, red, weight;   // 3 per point (cart, red), 1 per point
+};
+void kpoints_generate(int k1, int k2, int k3, const double shift[3], const Lattice &lat, KPointSet &out);
+
 void build_nonlocal(const Lattice &lat, const Stencil &st, const std::vector<Psp8> &psps,
                     const std::vector<int> &atom_type, const std::vector<double> &pos_nc,
                     NonlocalHost &out);

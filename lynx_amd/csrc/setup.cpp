@@ -480,3 +480,57 @@ void build_nonlocal(const Lattice &lat, const Stencil &st, const std::vector<Psp
 }
 
 }  // namespace lynx
+
+namespace lynx {
+
+// ---------------------------------------------------------------- k-points --
+void kpoints_generate(int k1, int k2, int k3, const double shift[3], const Lattice &lat, KPointSet &out)
+{
+    if (k1 < 1 || k2 < 1 || k3 < 1) fail(1, "k-point grid must be at least 1 x 1 x 1");
+    const int K[3] = {k1, k2, k3};
+    const double tol = 1e-12;
+    int start[3];
+    double sums[3];
+    for (int a = 0; a < 3; ++a) {
+        start[a] = -(int)std::floor((K[a] - 1) / 2.0);
+        sums[a] = 2.0 * M_PI / lat.L[a];
+    }
+    out.cart.clear();
+    out.red.clear();
+    out.weight.clear();
+    for (int n1 = start[0]; n1 < start[0] + K[0]; ++n1)
+        for (int n2 = start[1]; n2 < start[1] + K[1]; ++n2)
+            for (int n3 = start[2]; n3 < start[2] + K[2]; ++n3) {
+                const int nn[3] = {n1, n2, n3};
+                double kr[3], kc[3];
+                for (int a = 0; a < 3; ++a) {
+                    // fold into (-1/2, 1/2] with the reference's tolerance shift
+                    const double v = (double)nn[a] / K[a];
+                    kr[a] = std::fmod(v + shift[a] / K[a] + 0.5 - tol, 1.0) - 0.5 + tol;
+                    kc[a] = kr[a] * 2.0 * M_PI / lat.L[a];
+                }
+                long hit = -1;
+                const long np = (long)out.weight.size();
+                for (long q = 0; q < np && hit < 0; ++q) {
+                    bool same = true;
+                    for (int a = 0; a < 3; ++a) {
+                        const double s = kc[a] + out.cart[3 * q + a];
+                        same = same && (std::fabs(s) < tol || std::fabs(s - sums[a]) < tol);
+                    }
+                    if (same) hit = q;
+                }
+                if (hit >= 0) {
+                    out.weight[hit] = 2.0;
+                } else {
+                    for (int a = 0; a < 3; ++a) {
+                        out.cart.push_back(kc[a]);
+                        out.red.push_back(kr[a]);
+                    }
+                    out.weight.push_back(1.0);
+                }
+            }
+    const double tot = (double)K[0] * K[1] * K[2];
+    for (double &w : out.weight) w /= tot;
+}
+
+}  // namespace lynx

@@ -82,7 +82,7 @@ class System:
     (optionally) the GPU Hamiltonian built from the same inputs."""
 
     def __init__(self, n, a, frac, psp_name="C.pot", order=12, with_nonlocal=True, kpt=(0.0, 0.0, 0.0),
-                 latvec=None):
+                 latvec=None, psp_names=None, types=None, extra_wells=None):
         if isinstance(n, int):
             n = (n, n, n)
         self.n = tuple(n)
@@ -90,14 +90,19 @@ class System:
         self.frac = np.asarray(frac, float)
         self.order = order
         self.orth = latvec is None
-        self.latvec = np.diag([a, a, a]).astype(float) if latvec is None else np.asarray(latvec, float)
-        self.psp_files = [psp(psp_name)]
-        self.types = np.zeros(len(self.frac), np.int32)
-        self.with_nonlocal = with_nonlocal and len(self.frac) > 0
-        if self.orth:
-            self.veff = gaussian_veff(self.n, a, self.frac)
+        if latvec is None:
+            L = np.broadcast_to(np.asarray(a, float), (3,))
+            self.latvec = np.diag(L).astype(float)
         else:
-            self.veff = gaussian_veff_cell(self.n, self.latvec, self.frac)
+            self.latvec = np.asarray(latvec, float)
+        self.psp_files = [psp(p) for p in (psp_names or [psp_name])]
+        self.types = np.zeros(len(self.frac), np.int32) if types is None else np.asarray(types, np.int32)
+        self.with_nonlocal = with_nonlocal and len(self.frac) > 0
+        wells = self.frac if extra_wells is None else np.concatenate([self.frac, np.asarray(extra_wells, float)])
+        if self.orth and np.ndim(a) == 0:
+            self.veff = gaussian_veff(self.n, a, wells)
+        else:
+            self.veff = gaussian_veff_cell(self.n, self.latvec, wells)
         self.nd = int(np.prod(self.n))
         self.kpt = tuple(kpt)
 
@@ -143,6 +148,37 @@ class System:
 
 def si8(n=30, **kw):
     return System(n, A_SI, DIAMOND, **kw)
+
+
+def si16(seed, sigma=0.05):
+    """Si16: the Si8 cell doubled along z (10.26 x 10.26 x 20.52 Bohr), 30 x 30 x 60."""
+    base = np.concatenate([DIAMOND * [1, 1, 0.5], (DIAMOND + [0, 0, 1]) * [1, 1, 0.5]])
+    rng = np.random.default_rng(seed)
+    L = np.array([A_SI, A_SI, 2 * A_SI])
+    frac = base + rng.normal(0.0, sigma, base.shape) / L
+    return System((30, 30, 60), L, frac)
+
+
+def batio3(seed, n=28, a=7.57, sigma=0.05):
+    """Cubic BaTiO3 (a = 7.57 Bohr), Ti displaced 0.1 Bohr along z.  No Ba or
+    Ti psp8 ships with the reference; La.pot and Zr.pot (its test-suite
+    neighbours of Ba and Ti) stand in, with O.pot for oxygen."""
+    frac = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5 + 0.1 / a],
+                     [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    rng = np.random.default_rng(seed)
+    frac = frac + rng.normal(0.0, sigma, frac.shape) / a
+    return System(n, a, frac, psp_names=["La.pot", "Zr.pot", "O.pot"], types=[0, 1, 2, 2, 2])
+
+
+def h2o(seed, n=50, box=20.0, sigma=0.05):
+    """H2O in a 20 Bohr box.  The reference ships no H psp8: the two H atoms
+    enter only through Veff wells; O carries O.pot projectors."""
+    o = np.array([0.5, 0.5, 0.5])
+    hs = o + np.array([[1.431, 1.108, 0.0], [-1.431, 1.108, 0.0]]) / box
+    rng = np.random.default_rng(seed)
+    o = o + rng.normal(0.0, sigma, 3) / box
+    hs = hs + rng.normal(0.0, sigma, hs.shape) / box
+    return System(n, box, o[None, :], psp_names=["O.pot"], extra_wells=hs)
 
 
 def si2_fcc(n=24, **kw):

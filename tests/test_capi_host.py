@@ -168,3 +168,47 @@ def test_cpp_header_builds_and_runs_host_calls(tmp_path):
     assert out.returncode == 0, out.stdout + out.stderr
     nimg, ngpos, natom, nproj, threw = map(int, out.stdout.split())
     assert (natom, nproj, threw) == (8, 64, 1) and nimg >= 8 and ngpos > 8 * 100
+
+
+@pytest.mark.parametrize("K,shift,latvec,cell", [
+    ((6, 6, 6), (0.0, 0.0, 0.0), 0.5 * A_SI * np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]]), L.CELL_NONORTHOGONAL),
+    ((4, 3, 2), (0.5, 0.5, 0.0), np.diag([10.0, 11.0, 12.0]), L.CELL_ORTHOGONAL),
+    ((3, 3, 3), (0.0, 0.5, 0.0), np.diag([7.0, 7.0, 9.0]), L.CELL_ORTHOGONAL),
+])
+def test_kpoints_match_oracle(K, shift, latvec, cell):
+    lv = np.ascontiguousarray(latvec, float)
+    sh = np.asarray(shift, float)
+    n = ctypes.c_int()
+    L.check(L.lib().lynx_kpoints(*K, L.dptr(sh), L.dptr(lv), cell, None, None, None, 0, ctypes.byref(n)))
+    kc = np.zeros(3 * n.value); kr = np.zeros(3 * n.value); w = np.zeros(n.value)
+    L.check(L.lib().lynx_kpoints(*K, L.dptr(sh), L.dptr(lv), cell, L.dptr(kc), L.dptr(kr), L.dptr(w), n.value,
+                                 ctypes.byref(n)))
+    rc, rr, rw = oracle.kpoints(K, shift, oracle.Lattice(lv, cell == L.CELL_ORTHOGONAL))
+    assert n.value == len(rw)
+    assert np.array_equal(kc.reshape(-1, 3), rc) and np.array_equal(kr.reshape(-1, 3), rr)
+    assert np.array_equal(w, rw) and abs(w.sum() - 1) < 1e-12
+    if K == (6, 6, 6):
+        assert n.value == 112
+
+
+def test_nonlocal_builder_multitype_matches_oracle():
+    # three species, f projectors (La.pot, lmax 3) and a small cell
+    from systems import batio3
+    sysm = batio3(0, n=20)
+    c = host_ctx()
+    lv = np.ascontiguousarray(sysm.latvec, float)
+    L.check(L.lib().lynx_set_grid(c, 20, 20, 20, L.dptr(lv), L.CELL_ORTHOGONAL, 12), c)
+    paths = (ctypes.c_char_p * 3)(*[p.encode() for p in sysm.psp_files])
+    coords = np.ascontiguousarray(sysm.frac, float)
+    L.check(L.lib().lynx_set_atoms(c, 3, paths, len(sysm.types), L.iptr(sysm.types), L.dptr(coords), 1), c)
+    got = get_nonlocal(c)
+    op = sysm.oracle_op()
+    assert np.array_equal(got["img_atom"], [im.atom for im in op.images])
+    assert np.array_equal(got["gpos"], np.concatenate([im.gpos for im in op.images]))
+    chi = np.concatenate([np.asfortranarray(im.chi).ravel(order="F") for im in op.images])
+    assert np.abs(got["chi"] - chi).max() < 1e-12 * np.abs(chi).max()
+    ps = [oracle.load_psp8(f) for f in sysm.psp_files]
+    ip, gam = oracle.atom_gamma(ps, list(sysm.types))
+    assert np.array_equal(got["ip_displ"], ip) and np.array_equal(got["gamma"], gam)
+    assert ip[1] - ip[0] == 32    # La: 2 projectors for each of l = 0..3
+    L.lib().lynx_ctx_destroy(c)
