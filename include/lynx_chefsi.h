@@ -199,6 +199,16 @@ lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int la
 /* Enable (1) or disable (0) HIP-event timing of filter calls. */
 lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
 
+/* Precision of the Chebyshev filter inside lynx_chefsi_solve.  FP32 filters
+ * a single-precision copy of psi (complex64 for complex solves) and returns
+ * to FP64 for Cholesky-QR, the projected eigenproblem and the rotation, which
+ * always run in FP64.  No reference counterpart: the reference filters in
+ * FP64 only (EigenSolver.cpp:558-619). */
+enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1 };
+lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
+/* Grid point counts nx, ny, nz of the context. */
+lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3]);
+
 /* Launch shape of the fused stencil kernel.  By default the library times
  * the candidate shapes on the first call for each (grid, dtype, ncol) and
  * caches the fastest (its autotuner; no reference counterpart).
@@ -218,6 +228,46 @@ lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, 
 lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, int *threads,
                                     int *points_div, int *z_chunks, int *vector_runs, double *tune_ms,
                                     int *n_tune);
+
+/* ---- SCF eigen stage --------------------------------------------------------
+ * The per-iteration contract of SCF::solve_eigenproblem (SCF.cpp:310-467)
+ * with the spectral bounds of SCF::estimate_spectral_bounds
+ * (SCF.cpp:883-935): the caller's SCF loop (density, potential, mixing)
+ * calls lynx_scf_eig_step once per SCF iteration with the current Veff and
+ * residual.  Iteration 0 runs rho_trigger CheFSI passes, later iterations
+ * nchefsi; after each pass eig_min[s] <- eig(s, k0)[0] (iterations > 0),
+ * lambda_cutoff <- max_s eig(s, k0)[N-1] + 0.1, and the occupations and Fermi
+ * level are recomputed (Occupation.cpp:53-176).  With fp64_switch_residual
+ * > 0 the filter runs in FP32 in iteration 0 and while scf_residual exceeds
+ * it, and in FP64 afterwards; the subspace steps are always FP64. */
+typedef struct lynx_scf_eig lynx_scf_eig;
+typedef struct {
+    int nspin;                    /* 1 or 2 */
+    int nkpt;                     /* k-points of this context */
+    int complex_kpt;              /* 1: complex Bloch blocks (k-point runs), 0: real (Gamma) */
+    int nband;
+    int cheb_degree;
+    int rho_trigger;              /* SCFParams::rho_trigger (4 in the reference) */
+    int nchefsi;                  /* SCFParams::nchefsi (1) */
+    double nelectron;
+    double beta;                  /* 1 / (k_B T) in 1/Ha */
+    int smearing;                 /* 0 Fermi-Dirac, 1 Gaussian */
+    double fp64_switch_residual;  /* 0: FP64 filter throughout */
+} lynx_scf_eig_params;
+lynx_status lynx_scf_eig_create(lynx_scf_eig **driver, lynx_ctx *ctx, const lynx_scf_eig_params *params,
+                                const double *kpts, const double *kpt_weights);
+lynx_status lynx_scf_eig_destroy(lynx_scf_eig *driver);
+const char *lynx_scf_eig_last_error(const lynx_scf_eig *driver);
+/* veff: nspin x Nd doubles (host); psi[s * nkpt + k]: blocks of nband
+ * columns (ld, mem as lynx_chefsi_solve), updated in place; eig and occ:
+ * nspin x nkpt x nband; *fermi the Fermi level; *precision_used the filter
+ * precision of this step (LYNX_FILTER_*). */
+lynx_status lynx_scf_eig_step(lynx_scf_eig *driver, int scf_iter, double scf_residual, const double *veff,
+                              void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
+                              int *precision_used);
+/* Current lambda_cutoff and per-spin eig_min / eig_max (nspin each). */
+lynx_status lynx_scf_eig_state(const lynx_scf_eig *driver, double *lambda_cutoff, double *eig_min,
+                               double *eig_max);
 
 /* ---- Batched filter ------------------------------------------------------
  * A batch filters many independent units together: unit u applies the

@@ -52,6 +52,13 @@ SIGNATURES = {
     "lynx_kpoints": (_I, [_I, _I, _I, _DP, _DP, _I, _DP, _DP, _DP, _I, _IP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
+    "lynx_set_filter_precision": (_I, [_P, _I]),
+    "lynx_get_grid_dims": (_I, [_P, _IP]),
+    "lynx_scf_eig_create": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _DP]),
+    "lynx_scf_eig_destroy": (_I, [_P]),
+    "lynx_scf_eig_last_error": (ctypes.c_char_p, [_P]),
+    "lynx_scf_eig_step": (_I, [_P, _I, _D, _DP, ctypes.POINTER(_P), _I, _I, _DP, _DP, _DP, _IP]),
+    "lynx_scf_eig_state": (_I, [_P, _DP, _DP, _DP]),
     "lynx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), _DP]),
     "lynx_batch_destroy": (_I, [_P]),
     "lynx_batch_last_error": (ctypes.c_char_p, [_P]),

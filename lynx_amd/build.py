@@ -21,6 +21,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = [
     ("setup.cpp", "host"),
+    ("scf_eig.cpp", "host"),
     ("capi.cpp", "hip"),
     ("kernels.hip", "hip"),
 ]

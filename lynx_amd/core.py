@@ -417,3 +417,66 @@ class BatchFilter:
             self.close()
         except Exception:
             pass
+
+
+class _ScfEigParams(ctypes.Structure):
+    _fields_ = [("nspin", ctypes.c_int), ("nkpt", ctypes.c_int), ("complex_kpt", ctypes.c_int),
+                ("nband", ctypes.c_int), ("cheb_degree", ctypes.c_int), ("rho_trigger", ctypes.c_int),
+                ("nchefsi", ctypes.c_int), ("nelectron", ctypes.c_double), ("beta", ctypes.c_double),
+                ("smearing", ctypes.c_int), ("fp64_switch_residual", ctypes.c_double)]
+
+
+class ScfEigenStage:
+    """The eigen stage of an SCF iteration (SCF::solve_eigenproblem with its
+    spectral-bound bookkeeping) on one Hamiltonian: ``step(scf_iter,
+    residual, Veff, psis)`` runs the CheFSI passes over every (spin, k-point)
+    block and returns eigenvalues, occupations and the Fermi level.  With
+    ``fp64_switch_residual`` > 0 the filter runs in FP32 until the SCF
+    residual drops below it (lynx_scf_eig_*)."""
+
+    def __init__(self, H, nband, nelectron, beta, kpts=None, kpt_weights=None, nspin=1, cheb_degree=20,
+                 rho_trigger=4, nchefsi=1, smearing=0, fp64_switch_residual=0.0):
+        self.H = H
+        self.kpts = None if kpts is None else np.ascontiguousarray(kpts, float).reshape(-1, 3)
+        nk = 1 if self.kpts is None else len(self.kpts)
+        self.nspin, self.nkpt, self.nband = nspin, nk, nband
+        self.complex_ = self.kpts is not None
+        p = _ScfEigParams(nspin, nk, int(self.complex_), nband, cheb_degree, rho_trigger, nchefsi,
+                          float(nelectron), float(beta), smearing, float(fp64_switch_residual))
+        kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
+        self._d = ctypes.c_void_p()
+        L.check(L.lib().lynx_scf_eig_create(ctypes.byref(self._d), H.ctx, ctypes.byref(p),
+                                            None if self.kpts is None else L.dptr(self.kpts),
+                                            None if kw is None else L.dptr(kw)), H.ctx)
+
+    def step(self, scf_iter, residual, Veff, psis):
+        """psis: list of nspin*nkpt (Nd, nband) Fortran-ordered arrays, updated in place."""
+        dt = np.complex128 if self.complex_ else np.float64
+        for p in psis:
+            if p.dtype != dt or not p.flags.f_contiguous:
+                raise ValueError("psi blocks must be Fortran-ordered %s arrays" % np.dtype(dt).name)
+        n = len(psis)
+        ptrs = (ctypes.c_void_p * n)(*[p.ctypes.data for p in psis])
+        v = np.ascontiguousarray(Veff, np.float64)
+        eig = np.zeros(n * self.nband)
+        occ = np.zeros(n * self.nband)
+        ef = ctypes.c_double()
+        prec = ctypes.c_int()
+        st = L.lib().lynx_scf_eig_step(self._d, int(scf_iter), float(residual), L.dptr(v), ptrs, psis[0].shape[0],
+                                       L.MEM_HOST, L.dptr(eig), L.dptr(occ), ctypes.byref(ef), ctypes.byref(prec))
+        if st != L.OK:
+            raise L.LynxError(st, L.lib().lynx_scf_eig_last_error(self._d).decode())
+        return eig, occ, ef.value, prec.value
+
+    def state(self):
+        lc = ctypes.c_double()
+        mn = np.zeros(self.nspin); mx = np.zeros(self.nspin)
+        L.check(L.lib().lynx_scf_eig_state(self._d, ctypes.byref(lc), L.dptr(mn), L.dptr(mx)))
+        return lc.value, mn, mx
+
+    def __del__(self):
+        try:
+            if self._d:
+                L.lib().lynx_scf_eig_destroy(self._d)
+        except Exception:
+            pass

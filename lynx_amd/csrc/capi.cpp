@@ -105,6 +105,9 @@ struct lynx_ctx {
     std::map<std::vector<long>, gpu::StepCfg> tuned;
     std::map<std::vector<long>, std::vector<double>> tune_log;
 
+    // precision of the filter inside lynx_chefsi_solve (LYNX_FILTER_*)
+    int filter_precision = 0;
+
     bool timing = false;
     double t_last[4] = {0, 0, 0, 0};
     int n_last[4] = {0, 0, 0, 0};
@@ -1023,7 +1026,20 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
         void *C = block(ctx->wC, dtype, ldw, nband);
         timing_begin(ctx);
         // a8: Y = p_m(H) psi -> B
-        filter(ctx, dtype, P, ldP, B, ldw, C, ldw, nband, degree, lambda_c, eig_min, eig_max);
+        if (ctx->filter_precision == LYNX_FILTER_FP32) {
+            // filter in single precision inside the FP64 work blocks: X in
+            // the first half of B, Y and the recurrence work block in C,
+            // then Y back to FP64 in B for the FP64 subspace steps
+            const int es1 = dtype == LYNX_F64 ? LYNX_F32 : LYNX_C64;
+            void *Xs = B;
+            void *Ys = C;
+            void *Ws = static_cast<char *>(C) + (size_t)ldw * nband * gpu::elem_size(es1);
+            gpu::convert_cols(dtype, P, ldP, es1, Xs, ldw, nd, nband, ctx->stream);
+            filter(ctx, es1, Xs, ldw, Ys, ldw, Ws, ldw, nband, degree, lambda_c, eig_min, eig_max);
+            gpu::convert_cols(es1, Ys, ldw, dtype, B, ldw, nd, nband, ctx->stream);
+        } else {
+            filter(ctx, dtype, P, ldP, B, ldw, C, ldw, nband, degree, lambda_c, eig_min, eig_max);
+        }
         timing_end(ctx);
         // a9: C = B R^-1 with dV B^H B = R^H R
         cholesky_inverse(ctx, dtype, B, ldw, nband);
@@ -1222,6 +1238,27 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
             for (int i = 0; i < std::min(n, *n_tune); ++i) tune_ms[i] = it->second[i];
         *n_tune = n;
     }
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3])
+{
+    if (!ctx || !dims) return LYNX_E_ARG;
+    if (!ctx->has_grid) return LYNX_E_STATE;
+    dims[0] = ctx->st.nx;
+    dims[1] = ctx->st.ny;
+    dims[2] = ctx->st.nz;
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision)
+{
+    if (!ctx) return LYNX_E_ARG;
+    if (precision != LYNX_FILTER_FP64 && precision != LYNX_FILTER_FP32) {
+        ctx->err = "filter precision must be LYNX_FILTER_FP64 or LYNX_FILTER_FP32";
+        return LYNX_E_ARG;
+    }
+    ctx->filter_precision = precision;
     return LYNX_OK;
 }
 

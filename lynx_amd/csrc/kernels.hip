@@ -1397,6 +1397,17 @@ __global__ void k_copy_cols(const S *src, long lds, S *dst, long ldd, long nd)
         d[i] = s[i];
 }
 
+// Precision conversion of a column block (double <-> float, element-wise;
+// complex blocks are converted as interleaved pairs).
+template <class D, class T>
+__global__ void k_convert_cols(const T *src, long lds, D *dst, long ldd, long n)
+{
+    const T *s = src + (long)blockIdx.y * lds;
+    D *d = dst + (long)blockIdx.y * ldd;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = static_cast<D>(s[i]);
+}
+
 // ---------------------------------------------------------- dispatch helpers --
 template <class F>
 void dispatch(int e, F &&f)
@@ -2068,6 +2079,25 @@ void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, i
         hipLaunchKernelGGL((k_copy_cols<S>), dim3(nb, ncol), dim3(256), 0, st, (const S *)src, lds, (S *)dst,
                            ldd, nd);
     });
+}
+
+void convert_cols(int e_src, const void *src, long lds, int e_dst, void *dst, long ldd, long nd, int ncol,
+                  hipStream_t st)
+{
+    if (ncol <= 0 || nd <= 0) return;
+    if (elem_complex(e_src) != elem_complex(e_dst)) fail(1, "conversion between real and complex blocks");
+    const int m = elem_complex(e_src) ? 2 : 1;      // scalars per element
+    const long n = nd * m;
+    const int nb = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    const bool sd = !elem_single(e_src), dd = !elem_single(e_dst);
+    if (sd && !dd)
+        hipLaunchKernelGGL((k_convert_cols<float, double>), dim3(nb, ncol), dim3(256), 0, st, (const double *)src,
+                           lds * m, (float *)dst, ldd * m, n);
+    else if (!sd && dd)
+        hipLaunchKernelGGL((k_convert_cols<double, float>), dim3(nb, ncol), dim3(256), 0, st, (const float *)src,
+                           lds * m, (double *)dst, ldd * m, n);
+    else
+        copy_cols(e_src, src, lds, dst, ldd, nd, ncol, st);
 }
 
 }  // namespace gpu

@@ -119,6 +119,10 @@ void scale_copy(int e, const void *src, void *dst, double s, long n, hipStream_t
 // Copies ncol columns between leading dimensions (device to device).
 void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, int ncol,
                hipStream_t st);
+// Same, converting between precisions of the same kind (FP64 <-> FP32,
+// complex128 <-> complex64).
+void convert_cols(int e_src, const void *src, long lds, int e_dst, void *dst, long ldd, long nd, int ncol,
+                  hipStream_t st);
 constexpr int kPartialBlocks = 512;
 
 }  // namespace gpu

@@ -1,0 +1,198 @@
+// Eigen-stage driver: the contract between LYNX's SCF loop and the CheFSI
+// eigensolver, run on the library's C ABI.  Follows
+// SCF::estimate_spectral_bounds (SCF.cpp:883-935) for the first call and
+// SCF::solve_eigenproblem (SCF.cpp:310-467) for every SCF iteration:
+// rho_trigger CheFSI passes in iteration 0 and nchefsi afterwards; each pass
+// solves every (spin, k-point) block, then sets eig_min[s] = eig(s, k0)[0]
+// (iterations > 0 only), lambda_cutoff = max_s eig(s, k0)[N-1] + 0.1, and
+// recomputes the occupations and Fermi level (Occupation.cpp:53-176).
+// eig_max stays at its Lanczos value.  The filter runs in FP32 while the SCF
+// residual is above a threshold and in FP64 below it.
+#include "lynx_chefsi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+struct lynx_scf_eig {
+    lynx_ctx *ctx = nullptr;
+    lynx_scf_eig_params p{};
+    std::vector<double> kpts, kw;
+    long nd = 0;
+    bool have_bounds = false;
+    std::vector<double> eig_min, eig_max;
+    double lambda_c = 0.0;
+    double fermi = 0.0;
+    int last_precision = LYNX_FILTER_FP64;
+    std::string err;
+};
+
+namespace {
+
+struct Fail {
+    lynx_status st;
+    std::string msg;
+};
+
+void ok(lynx_status st, const lynx_scf_eig *d, const char *what)
+{
+    if (st != LYNX_OK) throw Fail{st, std::string(what) + ": " + lynx_last_error(d->ctx)};
+}
+
+double smear(double x, double beta, int type)
+{
+    if (type == 1) return 0.5 * std::erfc(beta * x);
+    const double a = x * beta;
+    if (a > 50.0) return 0.0;
+    if (a < -50.0) return 1.0;
+    return 1.0 / (1.0 + std::exp(a));
+}
+
+// Fermi level: the root of sum_i w_i f(e_i - Ef) = Ne, bracketed as the
+// reference does and found by bisection to the reference's 1e-14 tolerance.
+double fermi_level(const std::vector<double> &e, const std::vector<double> &w, double ne, double beta, int type)
+{
+    auto occ = [&](double ef) {
+        double s = 0.0;
+        for (size_t i = 0; i < e.size(); ++i) s += w[i] * smear(e[i] - ef, beta, type);
+        return s - ne;
+    };
+    const double emin = *std::min_element(e.begin(), e.end());
+    const double emax = *std::max_element(e.begin(), e.end());
+    double a = emin - 10.0 / beta, b = emax + 10.0 / beta;
+    double fa = occ(a), fb = occ(b);
+    if (fa * fb > 0) {
+        a = emin - 100.0 / beta;
+        b = emax + 100.0 / beta;
+        fa = occ(a);
+        fb = occ(b);
+        if (fa * fb > 0) throw Fail{LYNX_E_ARG, "cannot bracket the Fermi level"};
+    }
+    for (int it = 0; it < 400 && b - a > 1e-14; ++it) {
+        const double m = 0.5 * (a + b);
+        const double fm = occ(m);
+        if (fm == 0.0) return m;
+        if ((fm < 0) == (fa < 0)) {
+            a = m;
+            fa = fm;
+        } else {
+            b = m;
+        }
+    }
+    return 0.5 * (a + b);
+}
+
+}  // namespace
+
+extern "C" {
+
+lynx_status lynx_scf_eig_create(lynx_scf_eig **out, lynx_ctx *ctx, const lynx_scf_eig_params *p,
+                                const double *kpts, const double *kpt_weights)
+{
+    if (!out || !ctx || !p) return LYNX_E_ARG;
+    *out = nullptr;
+    if (p->nspin < 1 || p->nspin > 2 || p->nkpt < 1 || p->nband < 1 || p->cheb_degree < 1 || p->rho_trigger < 1 ||
+        p->nchefsi < 1 || p->beta <= 0 || (p->smearing != 0 && p->smearing != 1))
+        return LYNX_E_ARG;
+    int dims[3] = {0, 0, 0};
+    const lynx_status st = lynx_get_grid_dims(ctx, dims);
+    if (st != LYNX_OK) return st;
+    auto *d = new lynx_scf_eig();
+    d->ctx = ctx;
+    d->p = *p;
+    d->nd = (long)dims[0] * dims[1] * dims[2];
+    d->kpts.assign(3 * p->nkpt, 0.0);
+    if (kpts) d->kpts.assign(kpts, kpts + 3 * p->nkpt);
+    d->kw.assign(p->nkpt, 1.0 / p->nkpt);
+    if (kpt_weights) d->kw.assign(kpt_weights, kpt_weights + p->nkpt);
+    d->eig_min.assign(p->nspin, 0.0);
+    d->eig_max.assign(p->nspin, 0.0);
+    *out = d;
+    return LYNX_OK;
+}
+
+lynx_status lynx_scf_eig_destroy(lynx_scf_eig *d)
+{
+    delete d;
+    return LYNX_OK;
+}
+
+const char *lynx_scf_eig_last_error(const lynx_scf_eig *d) { return d ? d->err.c_str() : "null driver"; }
+
+lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual, const double *veff,
+                              void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
+                              int *precision_used)
+{
+    if (!d || !veff || !psi || !eig) return LYNX_E_ARG;
+    try {
+        const lynx_scf_eig_params &p = d->p;
+        const int dtype = p.complex_kpt ? LYNX_C128 : LYNX_F64;
+        const int N = p.nband;
+        // spectral bounds once, per spin, at the first k-point (SCF.cpp:912-929)
+        if (!d->have_bounds) {
+            for (int s = 0; s < p.nspin; ++s) {
+                ok(lynx_set_veff(d->ctx, veff + s * d->nd, LYNX_MEM_HOST), d, "lynx_set_veff");
+                if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[0]), d, "lynx_set_kpoint");
+                ok(lynx_lanczos_bounds(d->ctx, dtype, &d->eig_min[s], &d->eig_max[s], 1e-2, 1000, 1, nullptr), d,
+                   "lynx_lanczos_bounds");
+            }
+            d->lambda_c = 0.5 * (d->eig_min[0] + d->eig_max[0]);
+            d->have_bounds = true;
+        }
+        const int passes = scf_iter == 0 ? p.rho_trigger : p.nchefsi;
+        const bool single = p.fp64_switch_residual > 0 && (scf_iter == 0 || scf_residual > p.fp64_switch_residual);
+        d->last_precision = single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64;
+        ok(lynx_set_filter_precision(d->ctx, d->last_precision), d, "lynx_set_filter_precision");
+        const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
+        for (int pass = 0; pass < passes; ++pass) {
+            for (int s = 0; s < p.nspin; ++s) {
+                ok(lynx_set_veff(d->ctx, veff + s * d->nd, LYNX_MEM_HOST), d, "lynx_set_veff");
+                for (int k = 0; k < p.nkpt; ++k) {
+                    if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[3 * k]), d, "lynx_set_kpoint");
+                    ok(lynx_chefsi_solve(d->ctx, dtype, psi[s * p.nkpt + k], ld, N, eig + (long)(s * p.nkpt + k) * N,
+                                         d->lambda_c, d->eig_min[s], d->eig_max[s], p.cheb_degree, nullptr, mem),
+                       d, "lynx_chefsi_solve");
+                }
+            }
+            // bound update after every pass (SCF.cpp:436-449)
+            double last_max = -1e30;
+            for (int s = 0; s < p.nspin; ++s) {
+                const double *e0 = eig + (long)(s * p.nkpt) * N;
+                if (scf_iter > 0) d->eig_min[s] = e0[0];
+                last_max = std::max(last_max, e0[N - 1]);
+            }
+            d->lambda_c = last_max + 0.1;
+            // occupations over every spin and k-point
+            std::vector<double> e, w;
+            for (int s = 0; s < p.nspin; ++s)
+                for (int k = 0; k < p.nkpt; ++k)
+                    for (int n = 0; n < N; ++n) {
+                        e.push_back(eig[(long)(s * p.nkpt + k) * N + n]);
+                        w.push_back(d->kw[k] * spin_fac);
+                    }
+            d->fermi = fermi_level(e, w, p.nelectron, p.beta, p.smearing);
+            if (occ)
+                for (size_t i = 0; i < e.size(); ++i) occ[i] = smear(e[i] - d->fermi, p.beta, p.smearing);
+        }
+        if (fermi) *fermi = d->fermi;
+        if (precision_used) *precision_used = d->last_precision;
+        return LYNX_OK;
+    } catch (const Fail &f) {
+        d->err = f.msg;
+        return f.st;
+    }
+}
+
+lynx_status lynx_scf_eig_state(const lynx_scf_eig *d, double *lambda_cutoff, double *eig_min, double *eig_max)
+{
+    if (!d) return LYNX_E_ARG;
+    if (lambda_cutoff) *lambda_cutoff = d->lambda_c;
+    for (int s = 0; s < d->p.nspin; ++s) {
+        if (eig_min) eig_min[s] = d->eig_min[s];
+        if (eig_max) eig_max[s] = d->eig_max[s];
+    }
+    return LYNX_OK;
+}
+
+}  // extern "C"

@@ -719,3 +719,69 @@ def setup_system(grid: Grid, psps, types, frac, order=12, kpt=(0.0, 0.0, 0.0)):
     build_chi(grid, psps, images, pos)
     ip, gam = atom_gamma(psps, types)
     return Operator(grid, order, images, len(types), ip, gam, kpt)
+
+
+# ---------------------------------------------------------------------------
+# SCF eigen stage (SCF.cpp:310-467, 883-935; Occupation.cpp:11-176)
+
+def smearing(x, beta, kind=0):
+    """Occupation.cpp:11-35: Fermi-Dirac (0) or 0.5 erfc (1, Gaussian)."""
+    from scipy.special import erfc
+    x = np.asarray(x, float)
+    if kind == 1:
+        return 0.5 * erfc(beta * x)
+    a = x * beta
+    return np.where(a > 50.0, 0.0, np.where(a < -50.0, 1.0, 1.0 / (1.0 + np.exp(np.clip(a, -50, 50)))))
+
+
+def fermi_level(eigs, weights, nelectron, beta, kind=0):
+    """Occupation.cpp:53-87: root of sum w f(e - Ef) = Ne on the bracket
+    [emin - 10/beta, emax + 10/beta] (scipy's brentq, xtol 1e-14)."""
+    from scipy.optimize import brentq
+    e = np.asarray(eigs, float)
+    w = np.asarray(weights, float)
+    g = lambda ef: float((w * smearing(e - ef, beta, kind)).sum() - nelectron)  # noqa: E731
+    a, b = e.min() - 10.0 / beta, e.max() + 10.0 / beta
+    if g(a) * g(b) > 0:
+        a, b = e.min() - 100.0 / beta, e.max() + 100.0 / beta
+    return brentq(g, a, b, xtol=1e-14, rtol=4 * np.finfo(float).eps, maxiter=400)
+
+
+class ScfEigStage:
+    """SCF::estimate_spectral_bounds + SCF::solve_eigenproblem for one spin
+    and the k-points of one operator, FP64 throughout."""
+
+    def __init__(self, op, nband, nelectron, beta, kpts=None, kpt_weights=None, degree=20, rho_trigger=4,
+                 nchefsi=1, kind=0):
+        self.op, self.nband, self.ne, self.beta, self.kind = op, nband, nelectron, beta, kind
+        self.kpts = [None] if kpts is None else list(kpts)
+        nk = len(self.kpts)
+        self.kw = np.full(nk, 1.0 / nk) if kpt_weights is None else np.asarray(kpt_weights, float)
+        self.degree, self.rho_trigger, self.nchefsi = degree, rho_trigger, nchefsi
+        self.bounds = None
+
+    def step(self, scf_iter, veff, psis):
+        op = self.op
+        if self.bounds is None:
+            if self.kpts[0] is not None:
+                op.set_kpoint(self.kpts[0])
+            mn, mx, _ = op.lanczos_bounds(veff)
+            self.eig_min, self.eig_max = mn, mx
+            self.lambda_c = 0.5 * (mn + mx)
+            self.bounds = True
+        passes = self.rho_trigger if scf_iter == 0 else self.nchefsi
+        eigs = [None] * len(self.kpts)
+        for _ in range(passes):
+            for k, kp in enumerate(self.kpts):
+                if kp is not None:
+                    op.set_kpoint(kp)
+                psis[k], eigs[k], _ = op.solve(psis[k], veff, self.lambda_c, self.eig_min, self.eig_max,
+                                               self.degree)
+            if scf_iter > 0:
+                self.eig_min = eigs[0][0]
+            self.lambda_c = eigs[0][-1] + 0.1
+            e = np.concatenate(eigs)
+            w = np.concatenate([np.full(self.nband, 2.0 * self.kw[k]) for k in range(len(self.kpts))])
+            self.fermi = fermi_level(e, w, self.ne, self.beta, self.kind)
+            self.occ = smearing(e - self.fermi, self.beta, self.kind)
+        return psis, np.concatenate(eigs)

@@ -474,3 +474,22 @@ def test_kpoints_fcc_6x6x6_reduces_to_112():
     assert len(cart) == 112
     assert abs(w.sum() - 1.0) < 1e-12
     assert np.sum(w == 1.0 / 216) == 8
+
+
+# ------------------------------------------------------------- occupations
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fermi_level_closed_forms(kind):
+    beta = 315.77
+    # f(x) + f(-x) = 1 for both smearings: levels symmetric about 0 with
+    # half filling put Ef at 0
+    e = np.array([-0.7, -0.2, 0.2, 0.7])
+    assert abs(oracle.fermi_level(e, [2.0] * 4, 4.0, beta, kind)) < 1e-12
+    # one level, half occupied: Ef at the level
+    assert abs(oracle.fermi_level([0.3], [2.0], 1.0, beta, kind) - 0.3) < 1e-12
+    # occupations sum to Ne
+    e = np.linspace(-0.5, 0.4, 17)
+    w = np.full(17, 2.0)
+    ef = oracle.fermi_level(e, w, 19.0, 50.0, kind)
+    assert abs((w * oracle.smearing(e - ef, 50.0, kind)).sum() - 19.0) < 1e-11
