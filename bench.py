@@ -295,6 +295,54 @@ def run_hetero_batch(steps, warmup):
             "max_rel_err_f32_vs_f64": err, "setup_s": round(setup_s, 2)}
 
 
+def run_mixed_si216():
+    """Mixed precision on Si216: the 3x3x3 diamond supercell (30.78 Bohr),
+    sigma = 0.03 Bohr (seed 3), 90^3 grid, 480 orbitals (seed 7), degree 20.
+    The SCF eigen stage (lynx_scf_eig) runs iteration 0 (4 CheFSI passes from
+    the random start, Lanczos bounds) and iteration 1 (one pass) twice from
+    the same start: FP64 throughout, and with the filter in FP32 while the
+    SCF residual exceeds 1e-4 (residual 1e-3 given for iteration 1).
+    Cholesky-QR and Rayleigh-Ritz stay FP64.  The band-energy difference per
+    atom uses the occupations of the FP64 run (Fermi-Dirac, 1000 K)."""
+    from systems import System, displace, supercell
+    from lynx_amd import core
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    a = 30.78
+    s = System(90, a, displace(supercell(3), a, 0.03, seed=3))
+    nb, ne, beta = 480, 4.0 * 216, 1.0 / (3.166811563e-6 * 1000.0)
+    rng = np.random.default_rng(7)
+    X0 = rng.uniform(-0.5, 0.5, (nb, s.nd))
+    out = {}
+    eigs = {}
+    for name, thr in (("fp64", 0.0), ("fp32_until_1e-4", 1e-4)):
+        H = s.hamiltonian(0)
+        drv = core.ScfEigenStage(H, nb, ne, beta, fp64_switch_residual=thr)
+        d = DeviceArray.from_numpy(X0)
+        synchronize()
+        t0 = time.perf_counter()
+        e0, occ0, ef0, p0 = drv.step_device(0, 1.0, s.veff, [d.value], s.nd)
+        synchronize()
+        t1 = time.perf_counter()
+        e1, occ1, ef1, p1 = drv.step_device(1, 1e-3, s.veff, [d.value], s.nd)
+        synchronize()
+        t2 = time.perf_counter()
+        out[name] = {"iter0_s": round(t1 - t0, 3), "iter1_s": round(t2 - t1, 3), "filter_precision": [p0, p1],
+                     "fermi": ef1}
+        eigs[name] = (e1, occ1)
+        d.free()
+        del drv, H
+    e64, f64 = eigs["fp64"]
+    emx, _ = eigs["fp32_until_1e-4"]
+    occ_n = int(ne // 2)
+    return {"workload": "Si216 3x3x3 supercell a=30.78 Bohr, sigma=0.03 (seed 3), 90^3 grid, 480 orbitals, degree 20; "
+                        "SCF iteration 0 (4 passes + Lanczos) and 1 (1 pass) of the eigen stage",
+            "fp64": out["fp64"], "mixed": out["fp32_until_1e-4"],
+            "speedup_iter1": round(out["fp64"]["iter1_s"] / out["fp32_until_1e-4"]["iter1_s"], 3),
+            "speedup_iter0": round(out["fp64"]["iter0_s"] / out["fp32_until_1e-4"]["iter0_s"], 3),
+            "max_abs_eig_err_occupied_Ha": float(np.abs(emx[:occ_n] - e64[:occ_n]).max()),
+            "band_energy_err_Ha_per_atom": float(abs((2.0 * f64 * (emx - e64)).sum()) / 216)}
+
+
 def cpu_baseline(sysm, X64, lam, lo, hi):
     """The oracle (single-threaded C restatement) on a bounded sample:
     the same grid, atoms and degree, 48 of the 160 columns."""
@@ -318,6 +366,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batches", action="store_true", help="skip the k-point and heterogeneous batch legs")
+    ap.add_argument("--no-si216", action="store_true", help="skip the Si216 mixed-precision eigen-stage leg")
     ap.add_argument("--stencil-shape", default=None, metavar="VEC,THREADS,DIV,ZCHUNKS",
                     help="force the FP64 stencil launch shape instead of the autotuned one (e.g. 1,512,2,1)")
     args = ap.parse_args()
@@ -347,6 +396,10 @@ def main():
                            "roofline_frac": round(r32["stencil_gbs"] / HBM_PEAK_GBS, 4),
                            "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
                            "launch_shape": r32["stencil_config"]}
+
+    if not args.no_si216:
+        variants["si216_mixed_precision"] = run_mixed_si216()
+        print(f"[bench] Si216 mixed: {variants['si216_mixed_precision']}", file=sys.stderr)
 
     if not args.no_batches:
         variants["c64_kpoint_batch"] = run_kpoint_batch(args.steps, args.warmup)

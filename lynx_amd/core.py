@@ -468,6 +468,21 @@ class ScfEigenStage:
             raise L.LynxError(st, L.lib().lynx_scf_eig_last_error(self._d).decode())
         return eig, occ, ef.value, prec.value
 
+    def step_device(self, scf_iter, residual, Veff, ptrs, ld):
+        """Device-resident blocks: ptrs are device addresses (lynx_amd.hipmem)."""
+        n = len(ptrs)
+        pp = (ctypes.c_void_p * n)(*ptrs)
+        v = np.ascontiguousarray(Veff, np.float64)
+        eig = np.zeros(n * self.nband)
+        occ = np.zeros(n * self.nband)
+        ef = ctypes.c_double()
+        prec = ctypes.c_int()
+        st = L.lib().lynx_scf_eig_step(self._d, int(scf_iter), float(residual), L.dptr(v), pp, int(ld),
+                                       L.MEM_DEVICE, L.dptr(eig), L.dptr(occ), ctypes.byref(ef), ctypes.byref(prec))
+        if st != L.OK:
+            raise L.LynxError(st, L.lib().lynx_scf_eig_last_error(self._d).decode())
+        return eig, occ, ef.value, prec.value
+
     def state(self):
         lc = ctypes.c_double()
         mn = np.zeros(self.nspin); mx = np.zeros(self.nspin)
