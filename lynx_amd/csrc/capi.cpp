@@ -507,6 +507,64 @@ void filter(lynx_ctx *c, int e, const void *x, long ldx, void *y, long ldy, void
 
 long padded_ld(long nd) { return (nd + 63) / 64 * 64; }
 
+// ------------------------------------------------------ subspace GEMMs --
+// Plain GEMMs on rocBLAS (MFMA).  Gram: C = alpha A^H B (n x n) with the
+// long K = Nd dimension split into chunks, one strided-batched GEMM over the
+// chunks plus the tail, then a fixed-order sum of the partial products, so
+// that a few hundred workgroups cover the small n x n output.
+void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, int n, double alpha, void *C)
+{
+    const long nd = c->nd;
+    const int ks = (int)std::max<long>(1, std::min<long>(16, nd / 8192));
+    const long kch = nd / ks;
+    const long tail = nd - (long)ks * kch;
+    const int nsplit = ks + (tail > 0 ? 1 : 0);
+    const size_t es = gpu::elem_size(e);
+    void *W = c->gwork.get((size_t)nsplit * n * n * es);
+    const rocblas_stride sc = (rocblas_stride)n * n;
+    if (e == gpu::F64) {
+        const double one = 1.0, zero = 0.0;
+        BLAS_OK(rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
+                                              (int)kch, &one, (const double *)A, (int)lda, kch, (const double *)B,
+                                              (int)ldb, kch, &zero, (double *)W, n, sc, ks));
+        if (tail > 0)
+            BLAS_OK(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, n, (int)tail,
+                                  &one, (const double *)A + ks * kch, (int)lda, (const double *)B + ks * kch,
+                                  (int)ldb, &zero, (double *)W + (size_t)ks * n * n, n));
+    } else if (e == gpu::C128) {
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+        using Z = rocblas_double_complex;
+        BLAS_OK(rocblas_zgemm_strided_batched(c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none,
+                                              n, n, (int)kch, &one, (const Z *)A, (int)lda, kch, (const Z *)B,
+                                              (int)ldb, kch, &zero, (Z *)W, n, sc, ks));
+        if (tail > 0)
+            BLAS_OK(rocblas_zgemm(c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n, n,
+                                  (int)tail, &one, (const Z *)A + ks * kch, (int)lda, (const Z *)B + ks * kch,
+                                  (int)ldb, &zero, (Z *)W + (size_t)ks * n * n, n));
+    } else {
+        fail(LYNX_E_UNSUPPORTED, "subspace products run in FP64");
+    }
+    gpu::gram_reduce(e, W, nsplit, n, alpha, C, c->stream);
+}
+
+// Rotation: C (Nd x n) = A (Nd x n) Q (n x n).
+void rotate(lynx_ctx *c, int e, const void *A, long lda, const void *Q, int n, void *C, long ldc)
+{
+    const int nd = c->nd;
+    if (e == gpu::F64) {
+        const double one = 1.0, zero = 0.0;
+        BLAS_OK(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nd, n, n, &one,
+                              (const double *)A, (int)lda, (const double *)Q, n, &zero, (double *)C, (int)ldc));
+    } else if (e == gpu::C128) {
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+        using Z = rocblas_double_complex;
+        BLAS_OK(rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nd, n, n, &one, (const Z *)A,
+                              (int)lda, (const Z *)Q, n, &zero, (Z *)C, (int)ldc));
+    } else {
+        fail(LYNX_E_UNSUPPORTED, "subspace products run in FP64");
+    }
+}
+
 void *block(DevBuf &b, int e, long ld, int ncol) { return b.get((size_t)ld * ncol * gpu::elem_size(e)); }
 
 // ------------------------------------------------------- dense FP64 steps --
@@ -523,8 +581,7 @@ void cholesky_inverse(lynx_ctx *c, int e, const void *X, long ldx, int n)
 {
     const size_t es = gpu::elem_size(e);
     void *S = c->s1.get((size_t)n * n * es);
-    c->gwork.get(gpu::gram_work_bytes(e, c->nd, n));
-    gpu::gram(e, X, ldx, X, ldx, c->nd, n, c->st.dV, S, c->gwork.p, c->stream);
+    gram(c, e, X, ldx, X, ldx, n, c->st.dV, S);
     c->info.get(sizeof(int));
     if (e == gpu::F64) {
         BLAS_OK(rocsolver_dpotrf(c->blas, rocblas_fill_upper, n, (double *)S, n, c->info.as<rocblas_int>()));
@@ -958,7 +1015,7 @@ lynx_status lynx_orthogonalize(lynx_ctx *ctx, int dtype, void *x, int ld, int nc
         }
         cholesky_inverse(ctx, dtype, dx, ldd, ncol);
         void *tmp = block(ctx->wB, dtype, padded_ld(nd), ncol);
-        gpu::rotate(dtype, dx, ldd, ctx->s1.p, ncol, tmp, padded_ld(nd), (int)nd, ctx->stream);
+        rotate(ctx, dtype, dx, ldd, ctx->s1.p, ncol, tmp, padded_ld(nd));
         gpu::copy_cols(dtype, tmp, padded_ld(nd), dx, ldd, nd, ncol, ctx->stream);
         if (mem != LYNX_MEM_DEVICE)
             HIP_OK(hipMemcpy2DAsync(x, ldx * es, dx, nd * es, nd * es, ncol, hipMemcpyDeviceToHost, ctx->stream));
@@ -990,8 +1047,7 @@ lynx_status lynx_project_hamiltonian(lynx_ctx *ctx, int dtype, const void *x, in
         void *hx = block(ctx->wB, dtype, ldw, ncol);
         apply_op(ctx, dtype, dx, ldd, nullptr, 0, hx, ldw, ncol, 1.0, 0.0, 0.0);
         void *H = ctx->s2.get((size_t)ncol * ncol * es);
-        ctx->gwork.get(gpu::gram_work_bytes(dtype, (int)nd, ncol));
-        gpu::gram(dtype, dx, ldd, hx, ldw, (int)nd, ncol, ctx->st.dV, H, ctx->gwork.p, ctx->stream);
+        gram(ctx, dtype, dx, ldd, hx, ldw, ncol, ctx->st.dV, H);
         gpu::hermitian_part(dtype, H, ncol, ctx->stream);
         HIP_OK(hipMemcpyAsync(hs, H, (size_t)ncol * ncol * es, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -1043,17 +1099,16 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
         timing_end(ctx);
         // a9: C = B R^-1 with dV B^H B = R^H R
         cholesky_inverse(ctx, dtype, B, ldw, nband);
-        gpu::rotate(dtype, B, ldw, ctx->s1.p, nband, C, ldw, (int)nd, ctx->stream);
+        rotate(ctx, dtype, B, ldw, ctx->s1.p, nband, C, ldw);
         // a10: Hs = dV C^H H C (B <- H C)
         apply_op(ctx, dtype, C, ldw, nullptr, 0, B, ldw, nband, 1.0, 0.0, 0.0);
         void *H = ctx->s2.get((size_t)nband * nband * es);
-        ctx->gwork.get(gpu::gram_work_bytes(dtype, (int)nd, nband));
-        gpu::gram(dtype, C, ldw, B, ldw, (int)nd, nband, ctx->st.dV, H, ctx->gwork.p, ctx->stream);
+        gram(ctx, dtype, C, ldw, B, ldw, nband, ctx->st.dV, H);
         gpu::hermitian_part(dtype, H, nband, ctx->stream);
         // a11: Hs = Q diag(eig) Q^H
         subspace_eig(ctx, dtype, nband, eigvals);
         // a12: psi = C Q
-        gpu::rotate(dtype, C, ldw, H, nband, P, ldP, (int)nd, ctx->stream);
+        rotate(ctx, dtype, C, ldw, H, nband, P, ldP);
         if (mem != LYNX_MEM_DEVICE)
             HIP_OK(hipMemcpy2DAsync(psi, ldp * es, P, nd * es, nd * es, nband, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));

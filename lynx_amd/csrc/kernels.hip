@@ -1183,63 +1183,8 @@ __global__ void __launch_bounds__(256) k_nl_scatter_batch(const BatchRec<typenam
                             r.ld[iout], r.ncol, coef[3 * u], lb % ncb, lb / ncb);
 }
 
-// ---------------------------------------------------------------- GEMMs --
-// Gram: C = A^H B over a split K range; tiles of 64x64 outputs per
-// workgroup, 16 K-rows staged in LDS per step; partial tiles go to work and a
-// second kernel sums them in a fixed order (deterministic).
-constexpr int GT = 64, GK = 16;
-
-template <class S>
-__global__ void __launch_bounds__(256) k_gram_partial(const S *__restrict__ A, long lda,
-                                                      const S *__restrict__ B, long ldb, int nd,
-                                                      int n, int kchunk, S *__restrict__ work)
-{
-    __shared__ S As[GK][GT + 1];
-    __shared__ S Bs[GK][GT + 1];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int i0 = blockIdx.x * GT, j0 = blockIdx.y * GT;
-    const long kb = (long)blockIdx.z * kchunk;
-    const long ke = min((long)nd, kb + kchunk);
-    S acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = zero<S>();
-    for (long k0 = kb; k0 < ke; k0 += GK) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int t = threadIdx.x + 256 * r;
-            const int kk = t & (GK - 1), ii = t >> 4;
-            const long k = k0 + kk;
-            const int ia = i0 + ii, jb = j0 + ii;
-            As[kk][ii] = (k < ke && ia < n) ? A[(long)ia * lda + k] : zero<S>();
-            Bs[kk][ii] = (k < ke && jb < n) ? B[(long)jb * ldb + k] : zero<S>();
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GK; ++kk) {
-            S av[4], bv[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = cfma(av[a], bv[b], acc[a][b]);
-        }
-        __syncthreads();
-    }
-    S *wz = work + (long)blockIdx.z * n * n;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
-            if (i < n && j < n) wz[(long)j * n + i] = acc[a][b];
-        }
-}
-
+// ------------------------------------------------------ Gram reduction --
+// Sum of split-K partial Gram products in a fixed order, times alpha.
 template <class S>
 __global__ void k_gram_reduce(const S *__restrict__ work, int nsplit, int n, double alpha, S *C)
 {
@@ -1251,59 +1196,6 @@ __global__ void k_gram_reduce(const S *__restrict__ work, int nsplit, int n, dou
     C[idx] = scl((typename Tr<S>::A)alpha, s);
 }
 
-// Rotation: C = A Q for tall A; 64 rows x 64 columns per workgroup.
-template <class S>
-__global__ void __launch_bounds__(256) k_rotate(const S *__restrict__ A, long lda,
-                                                const S *__restrict__ Q, int n, S *__restrict__ C,
-                                                long ldc, int nd)
-{
-    __shared__ S As[GK][GT + 1];
-    __shared__ S Qs[GK][GT + 1];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const long k0 = (long)blockIdx.x * GT;
-    const int j0 = blockIdx.y * GT;
-    S acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = zero<S>();
-    for (int i0 = 0; i0 < n; i0 += GK) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int t = threadIdx.x + 256 * r;
-            // A tile: 64 rows (k) x 16 (i), consecutive threads -> consecutive rows
-            const int kk = t & 63, ii = t >> 6;
-            const long k = k0 + kk;
-            const int i = i0 + ii;
-            As[ii][kk] = (k < nd && i < n) ? A[(long)i * lda + k] : zero<S>();
-            // Q tile: 16 (i) x 64 (j)
-            const int qi = t & 15, qj = t >> 4;
-            Qs[qi][qj] = (i0 + qi < n && j0 + qj < n) ? Q[(long)(j0 + qj) * n + i0 + qi] : zero<S>();
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ii = 0; ii < GK; ++ii) {
-            S av[4], qv[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = As[ii][tx + 16 * a];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) qv[b] = Qs[ii][ty + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = mfma_(av[a], qv[b], acc[a][b]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const long k = k0 + tx + 16 * a;
-            const int j = j0 + ty + 16 * b;
-            if (k < nd && j < n) C[(long)j * ldc + k] = acc[a][b];
-        }
-}
 
 // ------------------------------------------------------------ small helpers --
 template <class S>
@@ -1982,46 +1874,16 @@ void batch_step(BatchPlan *P, int step, int ix, int iold, int iout, hipStream_t 
 
 int batch_launches_per_step(const BatchPlan *P) { return (int)(P->sg.size() + 2 * P->ng.size()); }
 
-static int gram_splits(int nd, int n)
+void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void *C, hipStream_t st)
 {
-    const int tiles = ((n + GT - 1) / GT) * ((n + GT - 1) / GT);
-    int ks = std::max(1, 1024 / tiles);
-    const int maxks = std::max(1, nd / 512);
-    return std::min(ks, maxks);
-}
-
-size_t gram_work_bytes(int e, int nd, int n)
-{
-    return (size_t)gram_splits(nd, n) * n * n * elem_size(e);
-}
-
-void gram(int e, const void *A, long lda, const void *B, long ldb, int nd, int n, double alpha, void *C,
-          void *work, hipStream_t st)
-{
-    const int ks = gram_splits(nd, n);
-    int kchunk = (nd + ks - 1) / ks;
-    kchunk = (kchunk + GK - 1) / GK * GK;
-    const int nsplit = (nd + kchunk - 1) / kchunk;
     dispatch(e, [&](auto t) {
         using S = decltype(t);
-        dim3 grid((n + GT - 1) / GT, (n + GT - 1) / GT, nsplit);
-        hipLaunchKernelGGL((k_gram_partial<S>), grid, dim3(256), 0, st, (const S *)A, lda, (const S *)B, ldb,
-                           nd, n, kchunk, (S *)work);
         const long nn = (long)n * n;
-        hipLaunchKernelGGL((k_gram_reduce<S>), dim3((nn + 255) / 256), dim3(256), 0, st, (const S *)work,
-                           nsplit, n, alpha, (S *)C);
+        hipLaunchKernelGGL((k_gram_reduce<S>), dim3((nn + 255) / 256), dim3(256), 0, st, (const S *)work, nsplit, n,
+                           alpha, (S *)C);
     });
 }
 
-void rotate(int e, const void *A, long lda, const void *Q, int n, void *C, long ldc, int nd, hipStream_t st)
-{
-    dispatch(e, [&](auto t) {
-        using S = decltype(t);
-        dim3 grid((nd + GT - 1) / GT, (n + GT - 1) / GT);
-        hipLaunchKernelGGL((k_rotate<S>), grid, dim3(256), 0, st, (const S *)A, lda, (const S *)Q, n, (S *)C,
-                           ldc, nd);
-    });
-}
 
 void hermitian_part(int e, void *H, int n, hipStream_t st)
 {

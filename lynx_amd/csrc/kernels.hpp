@@ -96,14 +96,9 @@ void batch_set_coef(BatchPlan *p, const double *coef, int steps, hipStream_t st)
 void batch_step(BatchPlan *p, int step, int ix, int iold, int iout, hipStream_t st);
 int batch_launches_per_step(const BatchPlan *p);
 
-// C (n x n, ld n) = alpha * A^H B for A, B (nd x n); split-K partials in work.
-size_t gram_work_bytes(int e, int nd, int n);
-void gram(int e, const void *A, long lda, const void *B, long ldb, int nd, int n, double alpha,
-          void *C, void *work, hipStream_t st);
-// C (nd x n) = A (nd x n) * Q (n x n, ld n)
-void rotate(int e, const void *A, long lda, const void *Q, int n, void *C, long ldc, int nd,
-            hipStream_t st);
-
+// C (n x n) = alpha * sum of nsplit consecutive n x n partial products in work
+// (fixed order).
+void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void *C, hipStream_t st);
 // Small dense helpers on n x n column-major matrices.
 void hermitian_part(int e, void *H, int n, hipStream_t st);
 void zero_lower(int e, void *H, int n, hipStream_t st);

@@ -249,8 +249,10 @@ def test_errors_are_reported():
     # leading dimension too small
     st = L.lib().lynx_h_apply(H.ctx, L.F64, L.vptr(X), s.nd - 1, L.vptr(Y), s.nd, 2, 0.0, L.MEM_HOST)
     assert st == L.E_ARG
-    # non-SPD overlap: duplicated column -> Cholesky failure status
-    Z = np.asfortranarray(np.concatenate([X, X[:, :1]], axis=1))
+    # singular overlap: a zero column gives a zero pivot in any summation
+    # order -> Cholesky failure status (a duplicated column is singular only
+    # up to the GEMM's rounding)
+    Z = np.asfortranarray(np.concatenate([X, np.zeros((s.nd, 1))], axis=1))
     st = L.lib().lynx_orthogonalize(H.ctx, L.F64, L.vptr(Z), s.nd, 3, L.MEM_HOST)
     assert st == L.E_CHOLESKY
     # non-finite potential
