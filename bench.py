@@ -413,7 +413,8 @@ def main():
         cpu = cpu_baseline(sysm, X64, 0.5 * (lo + hi), lo, hi)
 
     achieved = r64["stencil_gbs"]
-    kname = {0: "k_stencil_step<double", 1: "k_stencil_vec<double", 2: "k_stencil_vacc<double"}[
+    kname = {0: "k_stencil_step<double", 1: "k_stencil_vec<double", 2: "k_stencil_vacc<double",
+             3: "k_stencil_vacc<double"}[
         r64["stencil_config"]["vector_runs"]]
     traffic, traffic_src = profiled_traffic(r64["stencil_config"], kname)
     line = {
@@ -443,8 +444,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": {0: "k_stencil_step<double>", 1: "k_stencil_vec<double>",
-                       2: "k_stencil_vacc<double>"}[r64["stencil_config"]["vector_runs"]]
+            "kernel": {0: "k_stencil_step<double>", 1: "k_stencil_vec<double>", 2: "k_stencil_vacc<double>",
+                       3: "k_stencil_vacc<double> (2 planes in flight)"}[r64["stencil_config"]["vector_runs"]]
             + " (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
             "launch_shape": r64["stencil_config"],

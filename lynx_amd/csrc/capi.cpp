@@ -386,7 +386,7 @@ void upload_phases(lynx_ctx *c)
 std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c, int e)
 {
     std::vector<gpu::StepCfg> v;
-    for (int vec : {0, 1, 2})
+    for (int vec : {0, 1, 2, 3})
         for (int nt : {256, 512})
             for (int div : {1, 2})
                 for (int nz : {1, 2, 3, 4}) {
@@ -1260,7 +1260,7 @@ lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, 
         return LYNX_OK;
     }
     if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1 ||
-        vector_runs < 0 || vector_runs > 2)
+        vector_runs < 0 || vector_runs > 3)
         return LYNX_E_ARG;
     ctx->forced.nt = threads;
     ctx->forced.rp_div = points_div;

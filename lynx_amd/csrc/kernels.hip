@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 // read those rows as their own newest plane, so they come from L2 instead of
 // being re-fetched R steps later.  Planes outside the chunk only feed the
 // z-terms and skip LDS.
-template <class S, int R, int RV, int NT, int HP, int W>
+template <class S, int R, int RV, int NT, int HP, int W, int PD>
 __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const int w)
 {
     constexpr int VW = int(16 / sizeof(S));
@@ -925,8 +925,9 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
 
     const S diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
     int buf = 0;
+    // PD: planes of loads in flight ahead of the one consumed (1 or 2)
     auto step = [&](int f, VecLoads<S, RV, HP> &cur, VecLoads<S, RV, HP> &nxt) {
-        issue(f + 1, nxt);
+        issue(f + PD, nxt);
         if (f >= z0 && f < z1) {
             S *L = lds + buf * plane;
 #pragma unroll
@@ -993,22 +994,37 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
         }
     };
 
-    VecLoads<S, RV, HP> la, lb;
     int f = z0 - R;
-    issue(f, la);
-    for (; f + 1 <= f_end; f += 2) {
-        step(f, la, lb);
-        step(f + 1, lb, la);
+    if constexpr (PD == 1) {
+        VecLoads<S, RV, HP> la, lb;
+        issue(f, la);
+        for (; f + 1 <= f_end; f += 2) {
+            step(f, la, lb);
+            step(f + 1, lb, la);
+        }
+        if (f <= f_end) step(f, la, lb);
+    } else {
+        // three load sets in rotation: the set refilled at step f is the
+        // one consumed at step f - 1
+        VecLoads<S, RV, HP> la, lb, lc;
+        issue(f, la);
+        issue(f + 1, lb);
+        for (; f + 2 <= f_end; f += 3) {
+            step(f, la, lc);
+            step(f + 1, lb, la);
+            step(f + 2, lc, lb);
+        }
+        if (f <= f_end) step(f, la, lc);
+        if (f + 1 <= f_end) step(f + 1, lb, la);
     }
-    if (f <= f_end) step(f, la, lb);
 }
 
-template <class S, int R, int RV, int NT, int HP, int W>
+template <class S, int R, int RV, int NT, int HP, int W, int PD>
 __global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vacc_body<S, R, RV, NT, HP, W>(p, w);
+    stencil_vacc_body<S, R, RV, NT, HP, W, PD>(p, w);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -1410,7 +1426,7 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     }
 }
 
-template <class S, int RV, int NT, bool ACC>
+template <class S, int RV, int NT, int MODE>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                      int nzch, hipStream_t st)
@@ -1450,13 +1466,13 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT * VW);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS * VW;
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), shm, st, p); };
-    if constexpr (ACC) {
+    if constexpr (MODE > 0) {
         if (pitch == 64) {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64, MODE>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64, MODE>);
         } else {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128, MODE>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128, MODE>);
         }
     } else {
         if (pitch == 64) {
@@ -1538,15 +1554,14 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
                 using I2 = std::integral_constant<int, 2>;
                 using N256 = std::integral_constant<int, 256>;
                 using N512 = std::integral_constant<int, 512>;
-                if (cfg.vec == 2) {
-                    using T = std::true_type;
-                    if (cfg.nt >= 512) ok = cfg.rp_div >= 2 ? run(I1{}, N512{}, T{}) : run(I2{}, N512{}, T{});
-                    else ok = cfg.rp_div >= 2 ? run(I1{}, N256{}, T{}) : run(I2{}, N256{}, T{});
-                } else {
-                    using F = std::false_type;
-                    if (cfg.nt >= 512) ok = cfg.rp_div >= 2 ? run(I1{}, N512{}, F{}) : run(I2{}, N512{}, F{});
-                    else ok = cfg.rp_div >= 2 ? run(I1{}, N256{}, F{}) : run(I2{}, N256{}, F{});
-                }
+                auto by_mode = [&](auto mode) {
+                    using M = decltype(mode);
+                    if (cfg.nt >= 512) return cfg.rp_div >= 2 ? run(I1{}, N512{}, M{}) : run(I2{}, N512{}, M{});
+                    return cfg.rp_div >= 2 ? run(I1{}, N256{}, M{}) : run(I2{}, N256{}, M{});
+                };
+                if (cfg.vec == 3) ok = by_mode(std::integral_constant<int, 2>{});
+                else if (cfg.vec == 2) ok = by_mode(std::integral_constant<int, 1>{});
+                else ok = by_mode(std::integral_constant<int, 0>{});
                 if (ok) return;
             }
         }
