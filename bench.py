@@ -102,8 +102,8 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
     synchronize()
     elapsed = time.perf_counter() - t0
     L.check(lib.lynx_set_timing(ctx, 0), ctx)
-    thr = ctypes.c_int(); pd = ctypes.c_int(); zc = ctypes.c_int(); vr = ctypes.c_int(); nt = ctypes.c_int(64)
-    tms = (ctypes.c_double * 64)()
+    thr = ctypes.c_int(); pd = ctypes.c_int(); zc = ctypes.c_int(); vr = ctypes.c_int(); nt = ctypes.c_int(128)
+    tms = (ctypes.c_double * 128)()
     L.check(lib.lynx_get_stencil_config(ctx, dtype_code, NCOL, ctypes.byref(thr), ctypes.byref(pd),
                                         ctypes.byref(zc), ctypes.byref(vr), tms, ctypes.byref(nt)), ctx)
     Y = np.zeros((NCOL, nd), dtype=Xh.dtype)
@@ -414,7 +414,7 @@ def main():
 
     achieved = r64["stencil_gbs"]
     kname = {0: "k_stencil_step<double", 1: "k_stencil_vec<double", 2: "k_stencil_vacc<double",
-             3: "k_stencil_vacc<double"}[
+             3: "k_stencil_vacc<double"}[3 &
         r64["stencil_config"]["vector_runs"]]
     traffic, traffic_src = profiled_traffic(r64["stencil_config"], kname)
     line = {
@@ -445,7 +445,7 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "kernel": {0: "k_stencil_step<double>", 1: "k_stencil_vec<double>", 2: "k_stencil_vacc<double>",
-                       3: "k_stencil_vacc<double> (2 planes in flight)"}[r64["stencil_config"]["vector_runs"]]
+                       3: "k_stencil_vacc<double> (2 planes in flight)"}[r64["stencil_config"]["vector_runs"] & 3]
             + " (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
             "launch_shape": r64["stencil_config"],

@@ -216,12 +216,15 @@ lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3]);
  * points_div 1 or 2 (points per thread = base / points_div), z_chunks >= 1,
  * vector_runs 0..3 (real types: each thread owns 16-byte runs of x points;
  * with 2 the z-direction terms are accumulated forward, with 3 as 2 with
- * loads issued two planes ahead instead of one; ignored
+ * loads issued two planes ahead instead of one; adding 4 starts every band
+ * row at a lane multiple of 16, so 16-byte LDS accesses of one cycle never
+ * span two rows; ignored
  * for complex types, non-orthogonal cells, and grids or buffers where nx or
  * the leading dimensions are not multiples of the run or the pointers are
  * not 16-byte aligned); (0, 0, 0, 0) returns to tuning.  lynx_get_stencil_config
  * reports the shape in use and, in tune_ms (capacity *n_tune on input, count
- * on output), the timed candidates in order vector_runs 0..3 x threads
+ * on output), the timed candidates in order vector_runs 0..3 x row alignment
+ * 0..1 (none for 0) x threads
  * 256..512 x points_div 1..2 x z_chunks 1..4 (shapes that do not fit a grid
  * are skipped). */
 lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, int z_chunks,

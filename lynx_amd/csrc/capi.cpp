@@ -387,14 +387,17 @@ std::vector<gpu::StepCfg> step_candidates(const lynx_ctx *c, int e)
 {
     std::vector<gpu::StepCfg> v;
     for (int vec : {0, 1, 2, 3})
+      for (int align : {0, 1})
         for (int nt : {256, 512})
             for (int div : {1, 2})
                 for (int nz : {1, 2, 3, 4}) {
+                    if (align && !vec) continue;
                     gpu::StepCfg k;
                     k.rp_div = div;
                     k.nzch = nz;
                     k.nt = nt;
                     k.vec = vec;
+                    k.align = align;
                     if (nz > 1 && c->st.nz / nz < 2 * gpu::kMaxR) continue;
                     if (vec && !c->lat.orth) continue;   // the non-orthogonal kernel has one form
                     if (!gpu::stencil_shape_fits(e, c->st.nx, k)) continue;
@@ -437,8 +440,8 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
         // fewest z chunks (each chunk re-reads 2R planes): timing noise
         // should not pick the shape with more HBM traffic
         for (size_t i = 0; i < cands.size(); ++i)
-            if (times[i] <= 1.015 * tbest && cands[i].vec == best.vec && cands[i].nt == best.nt &&
-                cands[i].rp_div == best.rp_div && cands[i].nzch < best.nzch)
+            if (times[i] <= 1.015 * tbest && cands[i].vec == best.vec && cands[i].align == best.align &&
+                cands[i].nt == best.nt && cands[i].rp_div == best.rp_div && cands[i].nzch < best.nzch)
                 best = cands[i];
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1260,12 +1263,13 @@ lynx_status lynx_set_stencil_config(lynx_ctx *ctx, int threads, int points_div, 
         return LYNX_OK;
     }
     if ((threads != 256 && threads != 512) || (points_div != 1 && points_div != 2) || z_chunks < 1 ||
-        vector_runs < 0 || vector_runs > 3)
+        vector_runs < 0 || vector_runs > 7 || vector_runs == 4)
         return LYNX_E_ARG;
     ctx->forced.nt = threads;
     ctx->forced.rp_div = points_div;
     ctx->forced.nzch = z_chunks;
-    ctx->forced.vec = vector_runs;
+    ctx->forced.vec = vector_runs & 3;
+    ctx->forced.align = (vector_runs >> 2) & 1;
     ctx->has_forced = true;
     return LYNX_OK;
 }
@@ -1285,7 +1289,7 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
     if (threads) *threads = k->nt;
     if (points_div) *points_div = k->rp_div;
     if (z_chunks) *z_chunks = k->nzch;
-    if (vector_runs) *vector_runs = k->vec;
+    if (vector_runs) *vector_runs = k->vec | (k->align << 2);
     if (n_tune) {
         auto it = ctx->tune_log.find(key);
         const int n = it == ctx->tune_log.end() ? 0 : (int)it->second.size();

@@ -113,6 +113,7 @@ struct StepArgs {
     long ldx, ldo, ldout;
     int nx, ny, nz, ncol;
     int tyr, nbands, zch, nzch, nwork;
+    int lpr;            // 16-byte-run kernels: lanes per band row (>= runs per row)
     A c[3][kMaxR + 1];
     A a, b, s;
     A ph[3][2][2];
@@ -660,34 +661,39 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const int
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
-    const int nvec = rows * nx / VW;
     // owned runs: plane offset (element index), LDS index and x of the first
-    // element (kNoX for lanes without a run, whose stores go to the dummy)
+    // element (kNoX for lanes without a run, whose stores go to the dummy).
+    // Lane q takes run q % lpr of band row q / lpr; with lpr a multiple of 16
+    // no group of 16 lanes (one LDS cycle of 16-byte accesses) straddles two
+    // rows, so row starts never collide in the same banks.
     constexpr int kNoX = -(1 << 20);
+    const int runs = nx / VW;
+    const int lpr = p.lpr;
     unsigned qoff[RV];
     int qli[RV], qx0[RV];
     bool qv[RV];
 #pragma unroll
     for (int r = 0; r < RV; ++r) {
         const int q = tid + r * NT;
-        qv[r] = q < nvec;
-        const int e0 = (qv[r] ? q : 0) * VW;
-        const int qy = e0 / nx;
-        const int qx = e0 - qy * nx;
-        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
-        qli[r] = qv[r] ? (qy + R) * W + XO + qx : dummy;
+        const int qy = q / lpr;
+        const int qr = q - qy * lpr;
+        qv[r] = qy < rows && qr < runs;
+        const int qx = qv[r] ? qr * VW : 0;
+        const int qyy = qv[r] ? qy : 0;
+        qoff[r] = qv[r] ? (y0 + qyy) * nx + qx : 0;
+        qli[r] = qv[r] ? (qyy + R) * W + XO + qx : dummy;
         qx0[r] = qv[r] ? qx : kNoX;
     }
-    const int nhv = 2 * R * nx / VW;
     unsigned hsrc[HP];
     int hli[HP], hx0[HP];
 #pragma unroll
     for (int h = 0; h < HP; ++h) {
         const int t = tid + h * NT;
-        const bool v = t < nhv;
-        const int e0 = (v ? t : 0) * VW;
-        const int i = e0 / nx;
-        const int xx = e0 - i * nx;
+        const int ti = t / lpr;
+        const int tr = t - ti * lpr;
+        const bool v = ti < 2 * R && tr < runs;
+        const int i = v ? ti : 0;
+        const int xx = v ? tr * VW : 0;
         const int yb = (i < R) ? (i - R) : (rows + i - R);
         int yg = y0 + yb;
         if (yg < 0) yg += ny;
@@ -848,32 +854,35 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
-    const int nvec = rows * nx / VW;
+    // lane-to-run mapping as in stencil_vec_body
     constexpr int kNoX = -(1 << 20);
+    const int runs = nx / VW;
+    const int lpr = p.lpr;
     unsigned qoff[RV];
     int qli[RV], qx0[RV];
     bool qv[RV];
 #pragma unroll
     for (int r = 0; r < RV; ++r) {
         const int q = tid + r * NT;
-        qv[r] = q < nvec;
-        const int e0 = (qv[r] ? q : 0) * VW;
-        const int qy = e0 / nx;
-        const int qx = e0 - qy * nx;
-        qoff[r] = qv[r] ? (y0 + qy) * nx + qx : 0;
-        qli[r] = qv[r] ? (qy + R) * W + XO + qx : dummy;
+        const int qy = q / lpr;
+        const int qr = q - qy * lpr;
+        qv[r] = qy < rows && qr < runs;
+        const int qx = qv[r] ? qr * VW : 0;
+        const int qyy = qv[r] ? qy : 0;
+        qoff[r] = qv[r] ? (y0 + qyy) * nx + qx : 0;
+        qli[r] = qv[r] ? (qyy + R) * W + XO + qx : dummy;
         qx0[r] = qv[r] ? qx : kNoX;
     }
-    const int nhv = 2 * R * nx / VW;
     unsigned hsrc[HP];
     int hli[HP], hx0[HP];
 #pragma unroll
     for (int h = 0; h < HP; ++h) {
         const int t = tid + h * NT;
-        const bool v = t < nhv;
-        const int e0 = (v ? t : 0) * VW;
-        const int i = e0 / nx;
-        const int xx = e0 - i * nx;
+        const int ti = t / lpr;
+        const int tr = t - ti * lpr;
+        const bool v = ti < 2 * R && tr < runs;
+        const int i = v ? ti : 0;
+        const int xx = v ? tr * VW : 0;
         const int yb = (i < R) ? (i - R) : (rows + i - R);
         int yg = y0 + yb;
         if (yg < 0) yg += ny;
@@ -1429,7 +1438,7 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
 template <class S, int RV, int NT, int MODE>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
-                     int nzch, hipStream_t st)
+                     int nzch, bool align, hipStream_t st)
 {
     constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (kMaxR + VW - 1) / VW * VW;
@@ -1439,17 +1448,17 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     if (g.nx % VW || (ldx % VW) || (old && ldo % VW) || (ldout % VW) || !aligned(x) || (old && !aligned(old)) ||
         !aligned(out) || !aligned(veff))
         return false;
-    const long pts = (long)NT * RV * VW;
-    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB * VW || g.nx + 2 * XO > 128) return false;
-    static_assert(2 * kMaxR * (128 - 2 * XO) <= NT * HB * VW, "halo slots cover every row that fits the pitch");
+    const int runs = g.nx / VW;
+    const int lpr = align ? (runs + 15) / 16 * 16 : runs;   // lanes per band row
+    const int rpb = NT * RV / lpr;                          // band rows per workgroup
+    if (rpb < 1 || 2 * kMaxR * lpr > NT * HB || g.nx + 2 * XO > 128) return false;
     StepArgs<S> p;
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const S *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
-    int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
-    nb = std::max(1, std::min(nb, g.ny));
+    p.lpr = lpr;
+    const int nb = (g.ny + rpb - 1) / rpb;
     p.tyr = (g.ny + nb - 1) / nb;
-    while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
     p.nbands = (g.ny + p.tyr - 1) / p.tyr;
     nzch = std::max(1, std::min(nzch, g.nz));
     p.zch = (g.nz + nzch - 1) / nzch;
@@ -1464,7 +1473,7 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     const int blocks = ((p.nwork + 7) / 8) * 8;
     const int pitch = g.nx + 2 * XO <= 64 ? 64 : 128;
     const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT * VW);
-    const bool small_halo = 2 * kMaxR * g.nx <= NT * HS * VW;
+    const bool small_halo = 2 * kMaxR * lpr <= NT * HS;
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), shm, st, p); };
     if constexpr (MODE > 0) {
         if (pitch == 64) {
@@ -1509,8 +1518,10 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
                 constexpr int VW = int(16 / sizeof(S));
                 constexpr int XO = (kMaxR + VW - 1) / VW * VW;
                 const int hb = nt >= 512 ? 2 : 3;
-                const long pts = (long)nt * (cfg.rp_div >= 2 ? 1 : 2) * VW;
-                ok = nx % VW == 0 && nx <= pts && 2 * kMaxR * nx <= nt * hb * VW && nx + 2 * XO <= 128;
+                const int runs = nx / VW;
+                const int lpr = cfg.align ? (runs + 15) / 16 * 16 : runs;
+                const int lanes = nt * (cfg.rp_div >= 2 ? 1 : 2);
+                ok = nx % VW == 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128;
             }
         } else {
             const long pts = (long)nt * (base_rp<S>() / std::max(1, cfg.rp_div));
@@ -1548,7 +1559,7 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
                 bool ok;
                 auto run = [&](auto rv, auto nt, auto acc) {
                     return launch_step_vec<S, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value>(
-                        g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                        g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
                 };
                 using I1 = std::integral_constant<int, 1>;
                 using I2 = std::integral_constant<int, 2>;

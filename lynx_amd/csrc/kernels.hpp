@@ -34,7 +34,8 @@ struct StepCfg {
     int nzch = 1;      // number of z chunks per column
     int nt = 256;      // threads per workgroup (256 or 512)
     int vec = 1;       // real types: threads own 16-byte x runs (falls back when unaligned);
-                       // 2: the same with z-terms accumulated forward
+                       // 2: the same with z-terms accumulated forward, 3: 2 with loads two planes ahead
+    int align = 0;     // 16-byte-run kernels: band rows start at lane multiples of 16
 };
 
 // out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)
