@@ -1130,6 +1130,69 @@ __device__ __forceinline__ void nl_scatter_body(const NlArgs<typename Tr<S>::A> 
     Y[gi] = fm(a, acc, Y[gi]);
 }
 
+// Many columns per wave: the point's entry metadata and projector row are
+// loaded once and reused for CPW columns, so per column only the atom's
+// beta row (shared by the wave's neighbouring points) and y itself move.
+template <class S, int NPM, int CPW>
+__global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A> p, const S *__restrict__ beta,
+                                                       S *__restrict__ Y, long ldy, int ncol,
+                                                       typename Tr<S>::A a)
+{
+    using A = typename Tr<S>::A;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int u = blockIdx.x * 64 + lane;
+    const int c0 = (blockIdx.y * 4 + wave) * CPW;
+    if (u >= p.n_cov || c0 >= ncol) return;
+    const int c1 = min(c0 + CPW, ncol);
+    const int e0 = p.cov_off[u], e1 = p.cov_off[u + 1];
+    const long g = p.cov_gpos[u];
+    const long nb = (long)p.n_atom * NPM;
+    if (e1 - e0 == 1) {
+        // one entry (spheres do not overlap here): keep its row in registers
+        const int pt = p.ent_pt[e0];
+        A ch[NPM];
+#pragma unroll
+        for (int j = 0; j < NPM; ++j) ch[j] = p.chi[(long)pt * NPM + j];
+        const long boff = (long)p.pt_atom[pt] * NPM;
+        A pr = A(1), pi = A(0);
+        if constexpr (Tr<S>::C) {
+            const int im = p.pt_img[pt];
+            pr = p.phase[2 * im];
+            pi = -p.phase[2 * im + 1];
+        }
+        for (int c = c0; c < c1; ++c) {
+            const S *bt = beta + c * nb + boff;
+            S s = zero<S>();
+#pragma unroll
+            for (int j = 0; j < NPM; ++j) s = fm(ch[j], bt[j], s);
+            if constexpr (Tr<S>::C) s = cmul(s, pr, pi);
+            const long gi = (long)c * ldy + g;
+            Y[gi] = fm(a, s, Y[gi]);
+        }
+        return;
+    }
+    for (int c = c0; c < c1; ++c) {
+        const S *bn = beta + c * nb;
+        S acc = zero<S>();
+        for (int e = e0; e < e1; ++e) {
+            const int pt = p.ent_pt[e];
+            const A *chp = p.chi + (long)pt * NPM;
+            const S *bt = bn + (long)p.pt_atom[pt] * NPM;
+            S s = zero<S>();
+#pragma unroll
+            for (int j = 0; j < NPM; ++j) s = fm(chp[j], bt[j], s);
+            if constexpr (Tr<S>::C) {
+                const int im = p.pt_img[pt];
+                s = cmul(s, p.phase[2 * im], -p.phase[2 * im + 1]);
+            }
+            acc = add(acc, s);
+        }
+        const long gi = (long)c * ldy + g;
+        Y[gi] = fm(a, acc, Y[gi]);
+    }
+}
+
 template <class S, int NPM>
 __global__ void __launch_bounds__(256) k_nl_scatter(NlArgs<typename Tr<S>::A> p,
                                                     const S *__restrict__ beta, S *__restrict__ Y,
@@ -1607,8 +1670,9 @@ static void scatter_npm(const NlDev &nl, const void *beta, void *y, long ldy, in
     using A = typename Tr<S>::A;
     NlArgs<A> p{nl.atom_pt_off, nl.pt_gpos, nl.pt_img, nl.pt_atom, nl.cov_off, nl.cov_gpos, nl.ent_pt,
                 (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
-    dim3 grid((nl.n_cov + 63) / 64, (ncol + 3) / 4);
-    hipLaunchKernelGGL((k_nl_scatter<S, NPM>), grid, dim3(256), 0, st, p, (const S *)beta, (S *)y, ldy,
+    constexpr int CPW = 4;
+    dim3 grid((nl.n_cov + 63) / 64, (ncol + 4 * CPW - 1) / (4 * CPW));
+    hipLaunchKernelGGL((k_nl_scatter_mc<S, NPM, CPW>), grid, dim3(256), 0, st, p, (const S *)beta, (S *)y, ldy,
                        ncol, (A)a);
 }
 
