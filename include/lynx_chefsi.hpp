@@ -144,6 +144,14 @@ public:
     double lambda_cutoff() const { return lambda_cutoff_; }
     void set_lambda_cutoff(double v) { lambda_cutoff_ = v; }
 
+    // Filter precision of solve/solve_kpt (no reference counterpart): FP32
+    // filter, FP64 Cholesky-QR, projection, eigensolve and rotation.
+    void set_filter_fp32(bool single)
+    {
+        if (!H_) throw std::runtime_error("EigenSolver::setup has not been called");
+        check(lynx_set_filter_precision(H_->ctx(), single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64), H_->ctx());
+    }
+
 private:
     void need(int Nd_d) const
     {
@@ -152,6 +160,89 @@ private:
     }
     const Hamiltonian *H_ = nullptr;
     double lambda_cutoff_ = 0.0;
+};
+
+// The eigen stage of one SCF iteration, in place of SCF::solve_eigenproblem
+// (SCF.cpp:310-467) with estimate_spectral_bounds (SCF.cpp:883-935): the
+// caller's loop passes the iteration index, its density residual and Veff.
+class ScfEigenStage {
+public:
+    ScfEigenStage(const Hamiltonian &H, const lynx_scf_eig_params &p, const std::vector<Vec3> &kpts = {},
+                  const std::vector<double> &kpt_weights = {})
+        : ctx_(H.ctx())
+    {
+        std::vector<double> k;
+        for (const Vec3 &v : kpts) k.insert(k.end(), {v.x, v.y, v.z});
+        check(lynx_scf_eig_create(&d_, ctx_, &p, k.empty() ? nullptr : k.data(),
+                                  kpt_weights.empty() ? nullptr : kpt_weights.data()), ctx_);
+    }
+    ~ScfEigenStage() { lynx_scf_eig_destroy(d_); }
+    ScfEigenStage(const ScfEigenStage &) = delete;
+    ScfEigenStage &operator=(const ScfEigenStage &) = delete;
+
+    // psi[s * nkpt + k]: blocks of nband columns (leading dimension ld);
+    // returns the Fermi level; eig and occ hold nspin x nkpt x nband values.
+    double step(int scf_iter, double scf_residual, const double *Veff, const std::vector<void *> &psi, int ld,
+                double *eig, double *occ, bool device_buffers = false)
+    {
+        double ef = 0.0;
+        const lynx_status st = lynx_scf_eig_step(d_, scf_iter, scf_residual, Veff, psi.data(), ld,
+                                                 device_buffers ? LYNX_MEM_DEVICE : LYNX_MEM_HOST, eig, occ, &ef,
+                                                 &last_precision_);
+        if (st != LYNX_OK)
+            throw std::runtime_error(std::string(lynx_status_string(st)) + ": " + lynx_scf_eig_last_error(d_));
+        return ef;
+    }
+    // LYNX_FILTER_FP32 while the residual was above the switch, else FP64
+    int last_filter_precision() const { return last_precision_; }
+    double lambda_cutoff() const
+    {
+        double lc = 0.0;
+        lynx_scf_eig_state(d_, &lc, nullptr, nullptr);
+        return lc;
+    }
+
+private:
+    lynx_ctx *ctx_ = nullptr;
+    lynx_scf_eig *d_ = nullptr;
+    int last_precision_ = LYNX_FILTER_FP64;
+};
+
+// Chebyshev filter of many (Hamiltonian, k-point) units in one pass per
+// step (batched form of the per-k-point loop, SCF.cpp:391-411).
+class BatchFilter {
+public:
+    BatchFilter(const std::vector<const Hamiltonian *> &H, const std::vector<Vec3> &kpts = {})
+    {
+        std::vector<lynx_ctx *> c;
+        std::vector<double> k;
+        for (size_t u = 0; u < H.size(); ++u) {
+            c.push_back(H[u]->ctx());
+            const Vec3 v = u < kpts.size() ? kpts[u] : Vec3{};
+            k.insert(k.end(), {v.x, v.y, v.z});
+        }
+        const lynx_status st = lynx_batch_create(&b_, (int)c.size(), c.data(), k.data());
+        if (st != LYNX_OK) throw std::runtime_error(std::string("lynx_batch_create: ") + lynx_status_string(st));
+    }
+    ~BatchFilter() { lynx_batch_destroy(b_); }
+    BatchFilter(const BatchFilter &) = delete;
+    BatchFilter &operator=(const BatchFilter &) = delete;
+
+    // Device buffers: x[u] -> y[u] (leading dimensions ld[u], ncol[u]
+    // columns), dtype LYNX_F64 / LYNX_C128 / LYNX_F32 / LYNX_C64.
+    void filter(int dtype, const std::vector<void *> &x, const std::vector<void *> &y, const std::vector<long> &ld,
+                const std::vector<int> &ncol, int degree, const std::vector<double> &lambda_cutoff,
+                const std::vector<double> &eig_min, const std::vector<double> &eig_max)
+    {
+        const lynx_status st = lynx_batch_filter(b_, dtype, x.data(), ld.data(), y.data(), ld.data(), ncol.data(),
+                                                 degree, lambda_cutoff.data(), eig_min.data(), eig_max.data(),
+                                                 LYNX_MEM_DEVICE);
+        if (st != LYNX_OK)
+            throw std::runtime_error(std::string(lynx_status_string(st)) + ": " + lynx_batch_last_error(b_));
+    }
+
+private:
+    lynx_batch *b_ = nullptr;
 };
 
 }  // namespace lynx_mi355x
