@@ -631,7 +631,7 @@ struct VecLoads {
 };
 
 template <class S, int R, int RV, int NT, int HP, int W>
-__device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const int w)
+__device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
     constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (R + VW - 1) / VW * VW;   // LDS column of x = 0, vector aligned
@@ -654,10 +654,10 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const int
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
-    const bool has_old = p.b != S(0);
+    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const bool has_old = io.b != S(0);
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
@@ -783,9 +783,9 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const int
             V16<S> res;
 #pragma unroll
             for (int e = 0; e < VW; ++e) {
-                const S hval = fm(cur.v[r].v[e] + p.s, ring[r][R].v[e], S(-0.5) * lap[e]);
-                res.v[e] = p.a * hval;
-                if (has_old) res.v[e] = fm(p.b, cur.old[r].v[e], res.v[e]);
+                const S hval = fm(cur.v[r].v[e] + io.s, ring[r][R].v[e], S(-0.5) * lap[e]);
+                res.v[e] = io.a * hval;
+                if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
             }
             if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
         }
@@ -811,7 +811,7 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vec_body<S, R, RV, NT, HP, W>(p, w);
+    stencil_vec_body<S, R, RV, NT, HP, W>(p, step_io(p), w);
 }
 
 // The same 16-byte-run step with the z-direction accumulated forward
@@ -824,7 +824,7 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 // being re-fetched R steps later.  Planes outside the chunk only feed the
 // z-terms and skip LDS.
 template <class S, int R, int RV, int NT, int HP, int W, int PD>
-__device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const int w)
+__device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
     constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (R + VW - 1) / VW * VW;
@@ -847,10 +847,10 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(p.x) + (long)col * p.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(p.old) + (long)col * p.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(p.out) + (long)col * p.ldout;
-    const bool has_old = p.b != S(0);
+    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
+    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
+    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const bool has_old = io.b != S(0);
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
@@ -956,7 +956,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
                 S lap[VW];
 #pragma unroll
                 for (int e = 0; e < VW; ++e)
-                    lap[e] = fm(diag - S(2) * (cur.v[r].v[e] + p.s), cur.front[r].v[e], acc[r][R].v[e]);
+                    lap[e] = fm(diag - S(2) * (cur.v[r].v[e] + io.s), cur.front[r].v[e], acc[r][R].v[e]);
 #pragma unroll
                 for (int k = 1; k <= R; ++k) {
                     const V16<S> up = *reinterpret_cast<const V16<S> *>(L + qli[r] + k * W);
@@ -989,8 +989,8 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const in
                 V16<S> res;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
-                    res.v[e] = (S(-0.5) * p.a) * acc[r][0].v[e];
-                    if (has_old) res.v[e] = fm(p.b, cur.old[r].v[e], res.v[e]);
+                    res.v[e] = (S(-0.5) * io.a) * acc[r][0].v[e];
+                    if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
                 }
                 if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
             }
@@ -1033,7 +1033,7 @@ __global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vacc_body<S, R, RV, NT, HP, W, PD>(p, w);
+    stencil_vacc_body<S, R, RV, NT, HP, W, PD>(p, step_io(p), w);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -1220,7 +1220,9 @@ struct BatchRec {
     NlArgs<A> nl;
 };
 
-template <class S, int R, int RP, int NT, int HP, int W, bool NONORTH>
+// KIND: 0 orthogonal (scalar body), 1 non-orthogonal, 2 orthogonal with
+// 16-byte runs and row-aligned lanes (real types, RP runs per thread)
+template <class S, int R, int RP, int NT, int HP, int W, int KIND>
 __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
                                                       const int *__restrict__ item_unit, int nitems,
                                                       const typename Tr<S>::A *__restrict__ coef, int ix,
@@ -1233,8 +1235,13 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
     const BatchRec<A> &r = recs[u];
     const StepIO<A> io{r.buf[ix], r.buf[iold], const_cast<void *>(r.buf[iout]), r.ld[ix], r.ld[iold], r.ld[iout],
                        coef[3 * u], coef[3 * u + 1], coef[3 * u + 2]};
-    if constexpr (NONORTH) stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
-    else stencil_step_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
+    if constexpr (KIND == 1) {
+        stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
+    } else if constexpr (KIND == 2) {
+        if constexpr (!Tr<S>::C) stencil_vec_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
+    } else {
+        stencil_step_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
+    }
 }
 
 template <class S, int NPM>
@@ -1704,12 +1711,12 @@ struct BatchPlan {
     int e = 0, n = 0;
     void *recs = nullptr;
     struct SGroup {
-        bool orth = true;
+        int kind = 0;             // 0 orthogonal, 1 non-orthogonal, 2 orthogonal 16-byte runs
         int W = 64;
         int *items = nullptr;
         int nitems = 0;
         int max_tyr = 0;
-        bool small_halo = true;   // 2R nx <= 3 NT for every unit: 3 halo slots per thread, else 6
+        bool small_halo = true;   // fewer halo slots per thread suffice for every unit of the group
     };
     struct NGroup {
         int npm = 8;
@@ -1750,16 +1757,33 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
     std::vector<BatchRec<A>> recs(n);
     // stencil groups by (orth, pitch), nonlocal groups by npm
     std::vector<int> sgi(n), ngi(n, -1);
+    std::vector<int> kind(n);
     for (int u = 0; u < n; ++u) {
         const StencilGeom &g = units[u].g;
-        const int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
         if (g.nx + 2 * kMaxR > 128) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
-        const bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
+        kind[u] = g.orth ? 0 : 1;
+        int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+        bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
+        if constexpr (!Tr<S>::C) {
+            constexpr int VW = int(16 / sizeof(S));
+            constexpr int XO = (kMaxR + VW - 1) / VW * VW;
+            auto al = [](const void *ptr, long ld) {
+                return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0 && ld % VW == 0;
+            };
+            const BatchUnit &U = units[u];
+            const int lpr = (g.nx / VW + 15) / 16 * 16;
+            if (g.orth && g.nx % VW == 0 && g.nx + 2 * XO <= 128 && lpr <= kBatchNT &&
+                al(U.buf[0], U.ld[0]) && al(U.buf[1], U.ld[1]) && al(U.buf[2], U.ld[2]) && al(U.veff, 0)) {
+                kind[u] = 2;
+                W = g.nx + 2 * XO <= 64 ? 64 : 128;
+                sh = 2 * kMaxR * lpr <= kBatchNT * 2;
+            }
+        }
         int k = -1;
         for (size_t i = 0; i < P->sg.size(); ++i)
-            if (P->sg[i].orth == g.orth && P->sg[i].W == W && P->sg[i].small_halo == sh) k = (int)i;
+            if (P->sg[i].kind == kind[u] && P->sg[i].W == W && P->sg[i].small_halo == sh) k = (int)i;
         if (k < 0) {
-            P->sg.push_back({g.orth, W, nullptr, 0, 0, sh});
+            P->sg.push_back({kind[u], W, nullptr, 0, 0, sh});
             k = (int)P->sg.size() - 1;
         }
         sgi[u] = k;
@@ -1782,14 +1806,24 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         BatchRec<A> &r = recs[u];
         NonorthArgs<A> &q = r.q;
         StepArgs<A> &p = q.b;
-        const long pts = (long)kBatchNT * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
-        if (g.nx > pts || 2 * kMaxR * g.nx > kBatchNT * 6)
-            fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
         p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = U.ncol;
-        int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
-        nb = std::max(1, std::min(nb, g.ny));
-        p.tyr = (g.ny + nb - 1) / nb;
-        while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+        if (kind[u] == 2) {
+            // one 16-byte run per thread, band rows at lane multiples of 16
+            constexpr int VW = int(16 / sizeof(S));
+            p.lpr = (g.nx / VW + 15) / 16 * 16;
+            const int rpb = kBatchNT / p.lpr;
+            const int nb = (g.ny + rpb - 1) / rpb;
+            p.tyr = (g.ny + nb - 1) / nb;
+        } else {
+            const long pts = (long)kBatchNT * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
+            if (g.nx > pts || 2 * kMaxR * g.nx > kBatchNT * 6)
+                fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
+            p.lpr = 0;
+            int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
+            nb = std::max(1, std::min(nb, g.ny));
+            p.tyr = (g.ny + nb - 1) / nb;
+            while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+        }
         p.nbands = (g.ny + p.tyr - 1) / p.tyr;
         p.zch = g.nz;
         p.nzch = 1;
@@ -1859,12 +1893,23 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
     const int blocks = ((G.nitems + 7) / 8) * 8;
     const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
     const A *cf = static_cast<const A *>(coef);
+    if (G.kind == 2) {
+        if constexpr (!Tr<S>::C) {
+            // vector kind: halo slots counted in runs (2 or 3 per thread)
+            constexpr int VW = int(16 / sizeof(S));
+            constexpr int HV = HP == 3 ? 2 : 3;
+            const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT * VW);
+            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>), dim3(blocks), dim3(kBatchNT), shm,
+                               st, recs, G.items, G.nitems, cf, ix, iold, iout);
+        }
+        return;
+    }
     const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
-    if (G.orth)
-        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, false>), dim3(blocks),
+    if (G.kind == 0)
+        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, 0>), dim3(blocks),
                            dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
     else
-        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, true>), dim3(blocks),
+        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, 1>), dim3(blocks),
                            dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
 }
 
