@@ -344,6 +344,9 @@ void upload_nonlocal(lynx_ctx *c)
         v->n_pts = npts;
         v->n_cov = (int)cov_gpos.size();
         v->n_img = h.n_img;
+        v->max_atom_pts = 0;
+        for (int a = 0; a < h.n_atom; ++a)
+            v->max_atom_pts = std::max(v->max_atom_pts, atom_pt_off[a + 1] - atom_pt_off[a]);
         v->atom_pt_off = c->d_atom_pt_off.as<int>();
         v->pt_gpos = c->d_pt_gpos.as<int>();
         v->pt_img = c->d_pt_img.as<int>();

@@ -1037,9 +1037,10 @@ __global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
 }
 
 // --------------------------------------------------------- nonlocal gather --
-// One workgroup per physical atom and 4 columns (one wave per column).  Lanes
-// stride over the atom's sphere points (all images); each lane accumulates
-// npm projections, then a wave reduction and one store per projector.
+// One workgroup per (physical atom, column).  The four waves take
+// interleaved runs of the atom's sphere points (all images); each lane
+// accumulates npm projections, then a wave reduction, and the four wave sums
+// are added in a fixed order through LDS (deterministic, no atomics).
 template <class A>
 struct NlArgs {
     const int *atom_pt_off, *pt_gpos, *pt_img, *pt_atom, *cov_off, *cov_gpos, *ent_pt;
@@ -1047,21 +1048,26 @@ struct NlArgs {
     int n_atom, n_cov, npm;
 };
 
-template <class S, int NPM>
+// SPLIT = false: one wave per column (four columns per workgroup, index c
+// is the column group); SPLIT = true: the four waves share one column (c)
+// and split the points, for atoms with many sphere points.
+template <class S, int NPM, bool SPLIT>
 __device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ X,
-                                               long ldx, S *__restrict__ beta, int ncol, int at, int cgroup)
+                                               long ldx, S *__restrict__ beta, int ncol, int at, int c)
 {
     using A = typename Tr<S>::A;
+    __shared__ S part[4][NPM];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int n = cgroup * 4 + wave;
-    if (n >= ncol) return;
+    const int n = SPLIT ? c : c * 4 + wave;
+    if (n >= ncol) return;       // uniform over the workgroup when SPLIT
     const int pb = p.atom_pt_off[at], pe = p.atom_pt_off[at + 1];
     const S *Xn = X + (long)n * ldx;
     S acc[NPM];
 #pragma unroll
     for (int j = 0; j < NPM; ++j) acc[j] = zero<S>();
-    for (int e = pb + lane; e < pe; e += 64) {
+    constexpr int kStride = SPLIT ? 256 : 64;
+    for (int e = pb + (SPLIT ? wave * 64 : 0) + lane; e < pe; e += kStride) {
         S v = Xn[p.pt_gpos[e]];
         if constexpr (Tr<S>::C) {
             const int im = p.pt_img[e];
@@ -1083,18 +1089,36 @@ __device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &
             }
         }
     }
-    S *bn = beta + (long)n * p.n_atom * NPM + (long)at * NPM;
+    if constexpr (!SPLIT) {
+        S *bn = beta + (long)n * p.n_atom * NPM + (long)at * NPM;
+#pragma unroll
+        for (int j = 0; j < NPM; ++j)
+            if (lane == j) bn[j] = scl(p.gamma[at * NPM + j], acc[j]);
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < NPM; ++j)
-        if (lane == j) bn[j] = scl(p.gamma[at * NPM + j], acc[j]);
+        if (lane == j) part[wave][j] = acc[j];
+    __syncthreads();
+    if (wave == 0 && lane < NPM) {
+        S t = zero<S>();
+        // fixed order over the waves
+#pragma unroll
+        for (int w = 0; w < 4; ++w) t = add(t, part[w][lane]);
+        S *bn = beta + (long)n * p.n_atom * NPM + (long)at * NPM;
+        bn[lane] = scl(p.gamma[at * NPM + lane], t);
+    }
 }
 
-template <class S, int NPM>
+template <class S, int NPM, bool SPLIT>
 __global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, const S *__restrict__ X,
                                                    long ldx, S *__restrict__ beta, int ncol)
 {
-    nl_gather_body<S, NPM>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
+    nl_gather_body<S, NPM, SPLIT>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
 }
+
+// atoms with more sphere points than this are gathered by four waves
+constexpr int kGatherSplitPts = 1024;
 
 // -------------------------------------------------------- nonlocal scatter --
 // Owner-computes: one lane per covered grid point, one wave per column; sums
@@ -1244,7 +1268,7 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
     }
 }
 
-template <class S, int NPM>
+template <class S, int NPM, bool SPLIT>
 __global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
                                                          const int *__restrict__ item_unit, int nitems,
                                                          S *__restrict__ beta, int ix)
@@ -1256,8 +1280,8 @@ __global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename
     const BatchRec<A> &r = recs[u];
     const NlArgs<A> nl = r.nl;
     const int lb = w - r.g_item0;
-    nl_gather_body<S, NPM>(nl, reinterpret_cast<const S *>(r.buf[ix]), r.ld[ix], beta + r.beta_off, r.ncol,
-                           lb % nl.n_atom, lb / nl.n_atom);
+    nl_gather_body<S, NPM, SPLIT>(nl, reinterpret_cast<const S *>(r.buf[ix]), r.ld[ix], beta + r.beta_off,
+                                  r.ncol, lb % nl.n_atom, lb / nl.n_atom);
 }
 
 template <class S, int NPM>
@@ -1666,8 +1690,15 @@ static void gather_npm(const NlDev &nl, const void *x, long ldx, void *beta, int
     using A = typename Tr<S>::A;
     NlArgs<A> p{nl.atom_pt_off, nl.pt_gpos, nl.pt_img, nl.pt_atom, nl.cov_off, nl.cov_gpos, nl.ent_pt,
                 (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
-    dim3 grid(nl.n_atom, (ncol + 3) / 4);
-    hipLaunchKernelGGL((k_nl_gather<S, NPM>), grid, dim3(256), 0, st, p, (const S *)x, ldx, (S *)beta, ncol);
+    if (nl.max_atom_pts > kGatherSplitPts) {
+        dim3 grid(nl.n_atom, ncol);
+        hipLaunchKernelGGL((k_nl_gather<S, NPM, true>), grid, dim3(256), 0, st, p, (const S *)x, ldx, (S *)beta,
+                           ncol);
+    } else {
+        dim3 grid(nl.n_atom, (ncol + 3) / 4);
+        hipLaunchKernelGGL((k_nl_gather<S, NPM, false>), grid, dim3(256), 0, st, p, (const S *)x, ldx, (S *)beta,
+                           ncol);
+    }
 }
 
 template <class S, int NPM>
@@ -1720,6 +1751,7 @@ struct BatchPlan {
     };
     struct NGroup {
         int npm = 8;
+        bool split = false;     // gather with four waves per (atom, column)
         int *gitems = nullptr, *sitems = nullptr;
         int ngitems = 0, nsitems = 0;
     };
@@ -1788,11 +1820,15 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         }
         sgi[u] = k;
         if (units[u].nl.n_atom > 0 && units[u].nl.n_pts > 0) {
+            const bool split = units[u].nl.max_atom_pts > kGatherSplitPts;
             int j = -1;
             for (size_t i = 0; i < P->ng.size(); ++i)
-                if (P->ng[i].npm == units[u].nl.npm) j = (int)i;
+                if (P->ng[i].npm == units[u].nl.npm && P->ng[i].split == split) j = (int)i;
             if (j < 0) {
-                P->ng.push_back({units[u].nl.npm, nullptr, nullptr, 0, 0});
+                BatchPlan::NGroup G;
+                G.npm = units[u].nl.npm;
+                G.split = split;
+                P->ng.push_back(G);
                 j = (int)P->ng.size() - 1;
             }
             ngi[u] = j;
@@ -1863,7 +1899,8 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
             const int cg = (U.ncol + 3) / 4;
             auto &gi = gitems[ngi[u]];
             r.g_item0 = (int)gi.size();
-            for (int w = 0; w < nl.n_atom * cg; ++w) gi.push_back(u);
+            const int gcols = P->ng[ngi[u]].split ? U.ncol : cg;   // (atom, column or column group)
+            for (int w = 0; w < nl.n_atom * gcols; ++w) gi.push_back(u);
             auto &ci = citems[ngi[u]];
             r.s_item0 = (int)ci.size();
             for (int w = 0; w < ((nl.n_cov + 63) / 64) * cg; ++w) ci.push_back(u);
@@ -1920,9 +1957,14 @@ void launch_nl_batch(const BatchPlan *P, const BatchPlan::NGroup &G, const void 
     using A = typename Tr<S>::A;
     const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
     if (gather) {
-        if (G.ngitems > 0)
-            hipLaunchKernelGGL((k_nl_gather_batch<S, NPM>), dim3(G.ngitems), dim3(256), 0, st, recs, G.gitems,
-                               G.ngitems, static_cast<S *>(P->beta), ix);
+        if (G.ngitems > 0) {
+            if (G.split)
+                hipLaunchKernelGGL((k_nl_gather_batch<S, NPM, true>), dim3(G.ngitems), dim3(256), 0, st, recs,
+                                   G.gitems, G.ngitems, static_cast<S *>(P->beta), ix);
+            else
+                hipLaunchKernelGGL((k_nl_gather_batch<S, NPM, false>), dim3(G.ngitems), dim3(256), 0, st, recs,
+                                   G.gitems, G.ngitems, static_cast<S *>(P->beta), ix);
+        }
     } else if (G.nsitems > 0) {
         hipLaunchKernelGGL((k_nl_scatter_batch<S, NPM>), dim3(G.nsitems), dim3(256), 0, st, recs, G.sitems,
                            G.nsitems, static_cast<const S *>(P->beta), static_cast<const A *>(coef), iout);

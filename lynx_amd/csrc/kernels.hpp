@@ -54,6 +54,7 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg);
 //   entries (indices into points) for the owner-computes scatter.
 struct NlDev {
     int n_atom = 0, npm = 0, n_pts = 0, n_cov = 0, n_img = 0;
+    int max_atom_pts = 0;               // most sphere points of one atom (all images)
     const int *atom_pt_off = nullptr;   // n_atom+1
     const int *pt_gpos = nullptr;       // n_pts
     const int *pt_img = nullptr;        // n_pts

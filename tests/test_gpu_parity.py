@@ -289,3 +289,24 @@ def test_every_stencil_launch_shape(n):
         g32 = H._apply(Xr.T.copy(), s.veff, 0.1, np.float32).reshape(2, -1).T
         assert rel(g32, ref_r) < 2e-6, shape
     L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0, 0), H.ctx)
+
+
+def test_multispecies_large_spheres_apply():
+    # BaTiO3 stand-in (La, Zr, O psp8): 32 projectors per atom and spheres of
+    # several thousand points, so the gather splits each atom over four waves
+    from systems import batio3
+    s = batio3(0, n=28)
+    op, H = s.oracle_op(), s.hamiltonian()
+    assert max(len(im.gpos) for im in op.images) > 1024
+    X = s.random_block(5, 41)
+    ref = op.apply(X, s.veff, c=0.05)
+    got = H.apply(H.halo, X.T.copy(), s.veff, 0.05).reshape(5, -1).T
+    assert rel(got, ref) < 1e-12
+    k = (0.5 * 2 * np.pi / s.a, 0.0, 0.25 * 2 * np.pi / s.a)
+    op.set_kpoint(k)
+    Xc = s.random_block(3, 42, complex_=True)
+    refc = op.apply(Xc, s.veff)
+    gotc = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k).reshape(3, -1).T
+    assert rel(gotc, refc) < 1e-12
+    g32 = H._apply(Xc.T.copy(), s.veff, 0.0, np.complex64).reshape(3, -1).T
+    assert rel(g32, refc) < 2e-6
