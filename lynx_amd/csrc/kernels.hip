@@ -722,16 +722,15 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
 #pragma unroll
         for (int h = 0; h < HP; ++h) L.halo[h] = vld(X + zo, hsrc[h]);
     };
-    // a run and the wrapped copies of its elements that lie within R of an
-    // x face (x + nx to the right of the row, x - nx to the left)
+    // a run, and one wrapped copy of it when it lies within XO of an x face
+    // (x + nx to the right of the row, x - nx to the left; nx >= 2 XO, so a
+    // run is near at most one face): one 16-byte write per run, to the
+    // lane's dummy slot when no copy is needed
     auto put = [&](S *L, int li, int x0, const V16<S> &v) {
         *reinterpret_cast<V16<S> *>(L + li) = v;
-#pragma unroll
-        for (int e = 0; e < VW; ++e) {
-            const int xe = x0 + e;
-            L[unsigned(xe) < unsigned(R) ? li + e + nx : dummy + e] = v.v[e];
-            L[unsigned(xe - (nx - R)) < unsigned(R) ? li + e - nx : dummy + e] = v.v[e];
-        }
+        const bool lo = unsigned(x0) < unsigned(XO);
+        const bool hi = unsigned(x0 - (nx - XO)) < unsigned(XO);
+        *reinterpret_cast<V16<S> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
     };
 
     V16<S> ring[RV][NR];
@@ -915,14 +914,11 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
             for (int r = 0; r < RV; ++r) L.old[r] = vld(O + zo, qoff[r]);
         }
     };
-    auto put = [&](S *L, int li, int x0, const V16<S> &v) {
+    auto put = [&](S *L, int li, int x0, const V16<S> &v) {   // as in stencil_vec_body
         *reinterpret_cast<V16<S> *>(L + li) = v;
-#pragma unroll
-        for (int e = 0; e < VW; ++e) {
-            const int xe = x0 + e;
-            L[unsigned(xe) < unsigned(R) ? li + e + nx : dummy + e] = v.v[e];
-            L[unsigned(xe - (nx - R)) < unsigned(R) ? li + e - nx : dummy + e] = v.v[e];
-        }
+        const bool lo = unsigned(x0) < unsigned(XO);
+        const bool hi = unsigned(x0 - (nx - XO)) < unsigned(XO);
+        *reinterpret_cast<V16<S> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
     };
 
     // acc[j]: -2 (H - s) x at output plane f - R + j, so H x = -acc / 2
@@ -1545,7 +1541,7 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     const int runs = g.nx / VW;
     const int lpr = align ? (runs + 15) / 16 * 16 : runs;   // lanes per band row
     const int rpb = NT * RV / lpr;                          // band rows per workgroup
-    if (rpb < 1 || 2 * kMaxR * lpr > NT * HB || g.nx + 2 * XO > 128) return false;
+    if (rpb < 1 || 2 * kMaxR * lpr > NT * HB || g.nx + 2 * XO > 128 || g.nx < 2 * XO) return false;
     StepArgs<S> p;
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const S *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
@@ -1615,7 +1611,8 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
                 const int runs = nx / VW;
                 const int lpr = cfg.align ? (runs + 15) / 16 * 16 : runs;
                 const int lanes = nt * (cfg.rp_div >= 2 ? 1 : 2);
-                ok = nx % VW == 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128;
+                ok = nx % VW == 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128 &&
+                     nx >= 2 * XO;
             }
         } else {
             const long pts = (long)nt * (base_rp<S>() / std::max(1, cfg.rp_div));
@@ -1804,7 +1801,7 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
             };
             const BatchUnit &U = units[u];
             const int lpr = (g.nx / VW + 15) / 16 * 16;
-            if (g.orth && g.nx % VW == 0 && g.nx + 2 * XO <= 128 && lpr <= kBatchNT &&
+            if (g.orth && g.nx % VW == 0 && g.nx + 2 * XO <= 128 && g.nx >= 2 * XO && lpr <= kBatchNT &&
                 al(U.buf[0], U.ld[0]) && al(U.buf[1], U.ld[1]) && al(U.buf[2], U.ld[2]) && al(U.veff, 0)) {
                 kind[u] = 2;
                 W = g.nx + 2 * XO <= 64 ? 64 : 128;
