@@ -370,8 +370,9 @@ __global__ void __launch_bounds__(NT) k_stencil_step(StepArgs<typename Tr<S>::A>
 // contributions to the 2R+1 pending outputs f-R..f+R held in registers, and
 // output f-R is complete after plane f (the potential term joins the
 // accumulator when the output's own plane passes, so no plane is kept).  A chunk of planes [z0, z1) thus
-// streams planes z0-R .. z1-1+R.  Two barriers per plane, single-buffered
-// LDS (plane + g plane).
+// streams planes z0-R .. z1-1+R.  One barrier per plane: the (plane,
+// g-plane) pair is double-buffered and the xy term of plane f is taken one
+// step later, once its g plane is complete.
 // Coefficients are passed factored, D2_a[k] = beta_a w2[k], mixed
 // c_ab[k] = alpha_ab w1[k] and D1_a[k] = w1[k] / h_a, so that the kernel
 // holds 24 scalars instead of 56 (they live in SGPRs).
@@ -397,8 +398,9 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int dummy = W * (p.tyr + 2 * R) + (int)threadIdx.x;
     const int plane = W * (p.tyr + 2 * R) + NT;
-    S *LX = reinterpret_cast<S *>(smem_raw);
-    S *LG = LX + plane;
+    // LDS: two (plane, g-plane) pairs used alternately, so one barrier per
+    // plane suffices: the g of plane f is read one step later
+    S *const lds0 = reinterpret_cast<S *>(smem_raw);
 
     const int band = w % p.nbands;
     const int rest = w / p.nbands;
@@ -491,8 +493,12 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
         for (int j = 0; j < NA; ++j) acc[r][j] = zero<S>();
     const A diag = (q.beta[0] + q.beta[1] + q.beta[2]) * q.w2[0];
 
+    int buf = 0;
     auto step = [&](int f, StepLoads<S, RP, HP> &cur, StepLoads<S, RP, HP> &nxt) {
         issue(f + 1, nxt);
+        S *LX = lds0 + buf * 2 * plane;
+        S *LG = LX + plane;
+        const S *LGp = lds0 + (buf ^ 1) * 2 * plane + plane;   // g of plane f - 1
         // z-wrap phase of this plane (uniform)
         A zr = A(1), zi = A(0);
         if constexpr (C) {
@@ -516,7 +522,19 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
             put(LX, hli[h], hgh[h], hgl[h], v);
         }
         __syncthreads();
-        S inpl[RP], hz[RP], xzs[RP];
+        // xy term of plane f - 1 (its g plane is complete), into output f - 1
+        if (f - 1 >= z0 && f - 1 < z1) {
+#pragma unroll
+            for (int r = 0; r < RP; ++r) {
+                const S *g = LGp + qli[r];
+                S mxy = zero<S>();
+#pragma unroll
+                for (int k = 1; k <= R; ++k) mxy = fm(q.w1[k], add(g[k], scl(A(-1), g[-k])), mxy);
+                acc[r][R - 1] = fm(q.axy_hy, mxy, acc[r][R - 1]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        S hz[RP], xzs[RP];
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
             const S *c = LX + qli[r];
@@ -538,28 +556,20 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
             }
             // in-plane second differences, the potential term (-2 (V + s) x)
             S a2 = fm(diag - A(2) * (cur.v[r] + io.s), xf[r], scl(q.beta[0], sx2));
-            inpl[r] = fm(q.beta[1], sy2, a2);
+            acc[r][R] = add(acc[r][R], fm(q.beta[1], sy2, a2));
             hz[r] = scl(q.ihz, fm(q.axz, sx1, scl(q.ayz, sy1)));
             xzs[r] = scl(q.beta[2], xf[r]);
             put(LG, qli[r], qgh[r], qgl[r], sy1);
         }
-        __syncthreads();
         const bool out = f - R >= z0;
         S *Yz = Y + (f - R) * nxny;
 #pragma unroll
         for (int r = 0; r < RP; ++r) {
-            const S *g = LG + qli[r];
-            S mxy = zero<S>();
-#pragma unroll
-            for (int k = 1; k <= R; ++k) mxy = fm(q.w1[k], add(g[k], scl(A(-1), g[-k])), mxy);
-            const S a2 = fm(q.axy_hy, mxy, inpl[r]);
-            // plane f contributes to outputs f-R+j, j = 0..2R (distance d = R-j)
+            // z-terms of plane f for outputs f-R+j, j != R (distance d = R-j)
 #pragma unroll
             for (int j = 0; j < NA; ++j) {
                 const int d = R - j;
-                if (d == 0) {
-                    acc[r][j] = add(acc[r][j], a2);
-                } else {
+                if (d != 0) {
                     const int ad = d > 0 ? d : -d;
                     acc[r][j] = fm(q.w2[ad], xzs[r], acc[r][j]);
                     acc[r][j] = fm(d > 0 ? q.w1[ad] : -q.w1[ad], hz[r], acc[r][j]);
@@ -574,6 +584,7 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
             for (int j = 0; j + 1 < NA; ++j) acc[r][j] = acc[r][j + 1];
             acc[r][NA - 1] = zero<S>();
         }
+        buf ^= 1;
     };
 
     StepLoads<S, RP, HP> la, lb;
@@ -1429,6 +1440,17 @@ void dispatch(int e, F &&f)
 
 template <class S> constexpr int base_rp() { return int(32 / sizeof(S)); }
 
+// LDS of one workgroup (gfx950: 160 KiB per CU)
+constexpr long kLdsBytes = 160 * 1024;
+
+// rows per band of the non-orthogonal body within the LDS: four planes of
+// pitch * (rows + 2R) + NT elements (two (plane, g-plane) pairs)
+template <class S>
+int nonorth_max_rows(int pitch, int nt)
+{
+    return (int)((kLdsBytes / (4 * (long)sizeof(S)) - nt) / pitch) - 2 * kMaxR;
+}
+
 template <class S, int RP, int NT>
 void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                     void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
@@ -1488,10 +1510,12 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const A *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
+    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+    const int max_rows = nonorth_max_rows<S>(pitch, NT);
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
     p.tyr = (g.ny + nb - 1) / nb;
-    while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+    while ((long)p.tyr * g.nx > pts || p.tyr > max_rows) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
     p.nbands = (g.ny + p.tyr - 1) / p.tyr;
     nzch = std::max(1, std::min(nzch, g.nz));
     p.zch = (g.nz + nzch - 1) / nzch;
@@ -1513,8 +1537,7 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
         for (int sd = 0; sd < 2; ++sd)
             for (int k = 0; k < 2; ++k) p.ph[ax][sd][k] = (A)g.phase[ax][sd][k];
     const int blocks = ((p.nwork + 7) / 8) * 8;
-    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
-    const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
+    const size_t shm = 4 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
     if (pitch == 64) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, q);
@@ -1852,10 +1875,11 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
             if (g.nx > pts || 2 * kMaxR * g.nx > kBatchNT * 6)
                 fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
             p.lpr = 0;
+            const int max_rows = g.orth ? g.ny : nonorth_max_rows<S>(P->sg[sgi[u]].W, kBatchNT);
             int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
             nb = std::max(1, std::min(nb, g.ny));
             p.tyr = (g.ny + nb - 1) / nb;
-            while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+            while ((long)p.tyr * g.nx > pts || p.tyr > max_rows) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
         }
         p.nbands = (g.ny + p.tyr - 1) / p.tyr;
         p.zch = g.nz;
@@ -1938,7 +1962,8 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
         }
         return;
     }
-    const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
+    // the non-orthogonal body uses two (plane, g-plane) pairs
+    const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
     if (G.kind == 0)
         hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, 0>), dim3(blocks),
                            dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
