@@ -22,6 +22,16 @@ template <> struct Tr<float> { using A = float; static constexpr bool C = false;
 template <> struct Tr<double2> { using A = double; static constexpr bool C = true; };
 template <> struct Tr<float2> { using A = float; static constexpr bool C = true; };
 
+// the FP64 type of each element type (partial sums of single-precision
+// products are added in double precision)
+template <class S> struct Wide { using T = S; };
+template <> struct Wide<float> { using T = double; };
+template <> struct Wide<float2> { using T = double2; };
+__device__ __forceinline__ double widen(double x) { return x; }
+__device__ __forceinline__ double widen(float x) { return (double)x; }
+__device__ __forceinline__ double2 widen(double2 x) { return x; }
+__device__ __forceinline__ double2 widen(float2 x) { return make_double2(x.x, x.y); }
+
 template <class S> __device__ __forceinline__ S zero();
 template <> __device__ __forceinline__ double zero<double>() { return 0.0; }
 template <> __device__ __forceinline__ float zero<float>() { return 0.0f; }
@@ -607,44 +617,44 @@ __global__ void __launch_bounds__(NT) k_stencil_nonorth(NonorthArgs<typename Tr<
 
 // ------------------------------------------------ fused stencil, 16-B runs --
 // Same step as k_stencil_step for real element types, with each thread owning
-// runs of VW = 16/sizeof(S) consecutive x points (two doubles, four floats).
-// A run is loaded, stored and kept in LDS as one 16-byte vector; its x
-// neighbours come from 1 + 2*XO/VW aligned 16-byte LDS reads shared by the
-// run, its y neighbours from one 16-byte read per offset.  This cuts the LDS
-// instructions per point by about 2.5x and makes every global access a
-// dwordx4.  Requires nx % VW == 0 and 16-byte aligned columns.
-template <class S>
-struct alignas(16) V16 {
-    S v[16 / sizeof(S)];
+// runs of VW consecutive x points: 16-byte runs (two doubles, four floats),
+// or 8-byte runs of two floats when nx is not a multiple of four.  A run is
+// loaded, stored and kept in LDS as one vector; its x neighbours come from
+// 1 + 2*XO/VW aligned LDS reads shared by the run, its y neighbours from one
+// read per offset.  This cuts the LDS instructions per point by about 2.5x
+// and makes every global access a dwordx4 (dwordx2).  Requires nx % VW == 0
+// and columns aligned to the run size.
+template <class S, int VW>
+struct alignas(sizeof(S) * VW) VRun {
+    S v[VW];
 };
 
-template <class S>
-__device__ __forceinline__ V16<S> vzero()
+template <class S, int VW>
+__device__ __forceinline__ VRun<S, VW> vzero()
 {
-    V16<S> r;
+    VRun<S, VW> r;
 #pragma unroll
-    for (int e = 0; e < int(16 / sizeof(S)); ++e) r.v[e] = S(0);
+    for (int e = 0; e < VW; ++e) r.v[e] = S(0);
     return r;
 }
 
-template <class S>
-__device__ __forceinline__ V16<S> vld(const S *base, unsigned i)
+template <int VW, class S>
+__device__ __forceinline__ VRun<S, VW> vld(const S *base, unsigned i)
 {
-    return *reinterpret_cast<const V16<S> *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(S));
+    return *reinterpret_cast<const VRun<S, VW> *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(S));
 }
 
-template <class S, int RV, int HP>
+template <class S, int VW, int RV, int HP>
 struct VecLoads {
-    V16<S> front[RV];
-    V16<S> old[RV];
-    V16<S> v[RV];
-    V16<S> halo[HP];
+    VRun<S, VW> front[RV];
+    VRun<S, VW> old[RV];
+    VRun<S, VW> v[RV];
+    VRun<S, VW> halo[HP];
 };
 
-template <class S, int R, int RV, int NT, int HP, int W>
+template <class S, int R, int RV, int NT, int HP, int W, int VW>
 __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
-    constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (R + VW - 1) / VW * VW;   // LDS column of x = 0, vector aligned
     constexpr int NXV = 1 + 2 * XO / VW;         // 16-byte reads covering x-R .. x+VW-1+R
     constexpr int NR = 2 * R + 1;
@@ -714,40 +724,40 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
         hx0[h] = v ? xx : kNoX;
     }
 
-    auto load_plane = [&](int zz, V16<S> *dst) {
+    auto load_plane = [&](int zz, VRun<S, VW> *dst) {
         if (zz < 0) zz += nz;
         else if (zz >= nz) zz -= nz;
         const S *src = X + zz * nxny;
 #pragma unroll
-        for (int r = 0; r < RV; ++r) dst[r] = vld(src, qoff[r]);
+        for (int r = 0; r < RV; ++r) dst[r] = vld<VW>(src, qoff[r]);
     };
-    auto issue = [&](int zs, VecLoads<S, RV, HP> &L) {
+    auto issue = [&](int zs, VecLoads<S, VW, RV, HP> &L) {
         zs = min(zs, z1 - 1);
         load_plane(zs + R, L.front);
         const int zo = zs * nxny;
 #pragma unroll
         for (int r = 0; r < RV; ++r) {
-            L.old[r] = has_old ? vld(O + zo, qoff[r]) : vzero<S>();
-            L.v[r] = vld(p.veff + zo, qoff[r]);
+            L.old[r] = has_old ? vld<VW>(O + zo, qoff[r]) : vzero<S, VW>();
+            L.v[r] = vld<VW>(p.veff + zo, qoff[r]);
         }
 #pragma unroll
-        for (int h = 0; h < HP; ++h) L.halo[h] = vld(X + zo, hsrc[h]);
+        for (int h = 0; h < HP; ++h) L.halo[h] = vld<VW>(X + zo, hsrc[h]);
     };
     // a run, and one wrapped copy of it when it lies within XO of an x face
     // (x + nx to the right of the row, x - nx to the left; nx >= 2 XO, so a
     // run is near at most one face): one 16-byte write per run, to the
     // lane's dummy slot when no copy is needed
-    auto put = [&](S *L, int li, int x0, const V16<S> &v) {
-        *reinterpret_cast<V16<S> *>(L + li) = v;
+    auto put = [&](S *L, int li, int x0, const VRun<S, VW> &v) {
+        *reinterpret_cast<VRun<S, VW> *>(L + li) = v;
         const bool lo = unsigned(x0) < unsigned(XO);
         const bool hi = unsigned(x0 - (nx - XO)) < unsigned(XO);
-        *reinterpret_cast<V16<S> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
+        *reinterpret_cast<VRun<S, VW> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
     };
 
-    V16<S> ring[RV][NR];
+    VRun<S, VW> ring[RV][NR];
 #pragma unroll
     for (int i = 0; i < 2 * R; ++i) {
-        V16<S> tmp[RV];
+        VRun<S, VW> tmp[RV];
         load_plane(z0 - R + i, tmp);
 #pragma unroll
         for (int r = 0; r < RV; ++r) ring[r][i] = tmp[r];
@@ -755,7 +765,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
 
     const S diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
     int buf = 0;
-    auto step = [&](int z, VecLoads<S, RV, HP> &cur, VecLoads<S, RV, HP> &nxt) {
+    auto step = [&](int z, VecLoads<S, VW, RV, HP> &cur, VecLoads<S, VW, RV, HP> &nxt) {
         issue(z + 1, nxt);
         S *L = lds + buf * plane;
 #pragma unroll
@@ -772,7 +782,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
             S xr[NXV * VW];
 #pragma unroll
             for (int j = 0; j < NXV; ++j) {
-                const V16<S> t = *reinterpret_cast<const V16<S> *>(L + qli[r] - XO + j * VW);
+                const VRun<S, VW> t = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] - XO + j * VW);
 #pragma unroll
                 for (int e = 0; e < VW; ++e) xr[j * VW + e] = t.v[e];
             }
@@ -781,8 +791,8 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
             for (int e = 0; e < VW; ++e) lap[e] = diag * ring[r][R].v[e];
 #pragma unroll
             for (int k = 1; k <= R; ++k) {
-                const V16<S> up = *reinterpret_cast<const V16<S> *>(L + qli[r] + k * W);
-                const V16<S> dn = *reinterpret_cast<const V16<S> *>(L + qli[r] - k * W);
+                const VRun<S, VW> up = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] + k * W);
+                const VRun<S, VW> dn = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] - k * W);
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
                     lap[e] = fm(p.c[0][k], add(xr[XO + e + k], xr[XO + e - k]), lap[e]);
@@ -790,14 +800,14 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
                     lap[e] = fm(p.c[2][k], add(ring[r][R + k].v[e], ring[r][R - k].v[e]), lap[e]);
                 }
             }
-            V16<S> res;
+            VRun<S, VW> res;
 #pragma unroll
             for (int e = 0; e < VW; ++e) {
                 const S hval = fm(cur.v[r].v[e] + io.s, ring[r][R].v[e], S(-0.5) * lap[e]);
                 res.v[e] = io.a * hval;
                 if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
             }
-            if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+            if (qv[r]) *reinterpret_cast<VRun<S, VW> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
         }
 #pragma unroll
         for (int r = 0; r < RV; ++r)
@@ -806,7 +816,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
         buf ^= 1;
     };
 
-    VecLoads<S, RV, HP> la, lb;
+    VecLoads<S, VW, RV, HP> la, lb;
     issue(z0, la);
     int z = z0;
     for (; z + 1 < z1; z += 2) {
@@ -816,12 +826,12 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
     if (z < z1) step(z, la, lb);
 }
 
-template <class S, int R, int RV, int NT, int HP, int W>
+template <class S, int R, int RV, int NT, int HP, int W, int VW>
 __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vec_body<S, R, RV, NT, HP, W>(p, step_io(p), w);
+    stencil_vec_body<S, R, RV, NT, HP, W, VW>(p, step_io(p), w);
 }
 
 // The same 16-byte-run step with the z-direction accumulated forward
@@ -833,10 +843,9 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 // read those rows as their own newest plane, so they come from L2 instead of
 // being re-fetched R steps later.  Planes outside the chunk only feed the
 // z-terms and skip LDS.
-template <class S, int R, int RV, int NT, int HP, int W, int PD>
+template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW>
 __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
-    constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (R + VW - 1) / VW * VW;
     constexpr int NXV = 1 + 2 * XO / VW;
     constexpr int NA = 2 * R + 1;
@@ -905,44 +914,44 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     const int f_end = z1 - 1 + R;
     // loads for plane f: its owned runs (z-wrapped), and, for planes of the
     // chunk, its halo runs and Veff; Y_{k-2} of output plane f - R
-    auto issue = [&](int f, VecLoads<S, RV, HP> &L) {
+    auto issue = [&](int f, VecLoads<S, VW, RV, HP> &L) {
         f = min(f, f_end);
         int zz = f;
         if (zz < 0) zz += nz;
         else if (zz >= nz) zz -= nz;
         const S *Xz = X + zz * nxny;
 #pragma unroll
-        for (int r = 0; r < RV; ++r) L.front[r] = vld(Xz, qoff[r]);
+        for (int r = 0; r < RV; ++r) L.front[r] = vld<VW>(Xz, qoff[r]);
         if (f >= z0 && f < z1) {
 #pragma unroll
-            for (int h = 0; h < HP; ++h) L.halo[h] = vld(Xz, hsrc[h]);
+            for (int h = 0; h < HP; ++h) L.halo[h] = vld<VW>(Xz, hsrc[h]);
 #pragma unroll
-            for (int r = 0; r < RV; ++r) L.v[r] = vld(p.veff + zz * nxny, qoff[r]);
+            for (int r = 0; r < RV; ++r) L.v[r] = vld<VW>(p.veff + zz * nxny, qoff[r]);
         }
         if (has_old && f - R >= z0) {
             const int zo = (f - R) * nxny;
 #pragma unroll
-            for (int r = 0; r < RV; ++r) L.old[r] = vld(O + zo, qoff[r]);
+            for (int r = 0; r < RV; ++r) L.old[r] = vld<VW>(O + zo, qoff[r]);
         }
     };
-    auto put = [&](S *L, int li, int x0, const V16<S> &v) {   // as in stencil_vec_body
-        *reinterpret_cast<V16<S> *>(L + li) = v;
+    auto put = [&](S *L, int li, int x0, const VRun<S, VW> &v) {   // as in stencil_vec_body
+        *reinterpret_cast<VRun<S, VW> *>(L + li) = v;
         const bool lo = unsigned(x0) < unsigned(XO);
         const bool hi = unsigned(x0 - (nx - XO)) < unsigned(XO);
-        *reinterpret_cast<V16<S> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
+        *reinterpret_cast<VRun<S, VW> *>(L + (lo ? li + nx : (hi ? li - nx : dummy))) = v;
     };
 
     // acc[j]: -2 (H - s) x at output plane f - R + j, so H x = -acc / 2
-    V16<S> acc[RV][NA];
+    VRun<S, VW> acc[RV][NA];
 #pragma unroll
     for (int r = 0; r < RV; ++r)
 #pragma unroll
-        for (int j = 0; j < NA; ++j) acc[r][j] = vzero<S>();
+        for (int j = 0; j < NA; ++j) acc[r][j] = vzero<S, VW>();
 
     const S diag = p.c[0][0] + p.c[1][0] + p.c[2][0];
     int buf = 0;
     // PD: planes of loads in flight ahead of the one consumed (1 or 2)
-    auto step = [&](int f, VecLoads<S, RV, HP> &cur, VecLoads<S, RV, HP> &nxt) {
+    auto step = [&](int f, VecLoads<S, VW, RV, HP> &cur, VecLoads<S, VW, RV, HP> &nxt) {
         issue(f + PD, nxt);
         if (f >= z0 && f < z1) {
             S *L = lds + buf * plane;
@@ -956,7 +965,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
                 S xr[NXV * VW];
 #pragma unroll
                 for (int j = 0; j < NXV; ++j) {
-                    const V16<S> t = *reinterpret_cast<const V16<S> *>(L + qli[r] - XO + j * VW);
+                    const VRun<S, VW> t = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] - XO + j * VW);
 #pragma unroll
                     for (int e = 0; e < VW; ++e) xr[j * VW + e] = t.v[e];
                 }
@@ -966,8 +975,8 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
                     lap[e] = fm(diag - S(2) * (cur.v[r].v[e] + io.s), cur.front[r].v[e], acc[r][R].v[e]);
 #pragma unroll
                 for (int k = 1; k <= R; ++k) {
-                    const V16<S> up = *reinterpret_cast<const V16<S> *>(L + qli[r] + k * W);
-                    const V16<S> dn = *reinterpret_cast<const V16<S> *>(L + qli[r] - k * W);
+                    const VRun<S, VW> up = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] + k * W);
+                    const VRun<S, VW> dn = *reinterpret_cast<const VRun<S, VW> *>(L + qli[r] - k * W);
 #pragma unroll
                     for (int e = 0; e < VW; ++e) {
                         lap[e] = fm(p.c[0][k], add(xr[XO + e + k], xr[XO + e - k]), lap[e]);
@@ -993,26 +1002,26 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
             S *Yz = Y + (f - R) * nxny;
 #pragma unroll
             for (int r = 0; r < RV; ++r) {
-                V16<S> res;
+                VRun<S, VW> res;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
                     res.v[e] = (S(-0.5) * io.a) * acc[r][0].v[e];
                     if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
                 }
-                if (qv[r]) *reinterpret_cast<V16<S> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+                if (qv[r]) *reinterpret_cast<VRun<S, VW> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
             }
         }
 #pragma unroll
         for (int r = 0; r < RV; ++r) {
 #pragma unroll
             for (int j = 0; j + 1 < NA; ++j) acc[r][j] = acc[r][j + 1];
-            acc[r][NA - 1] = vzero<S>();
+            acc[r][NA - 1] = vzero<S, VW>();
         }
     };
 
     int f = z0 - R;
     if constexpr (PD == 1) {
-        VecLoads<S, RV, HP> la, lb;
+        VecLoads<S, VW, RV, HP> la, lb;
         issue(f, la);
         for (; f + 1 <= f_end; f += 2) {
             step(f, la, lb);
@@ -1022,7 +1031,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     } else {
         // three load sets in rotation: the set refilled at step f is the
         // one consumed at step f - 1
-        VecLoads<S, RV, HP> la, lb, lc;
+        VecLoads<S, VW, RV, HP> la, lb, lc;
         issue(f, la);
         issue(f + 1, lb);
         for (; f + 2 <= f_end; f += 3) {
@@ -1035,12 +1044,12 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     }
 }
 
-template <class S, int R, int RV, int NT, int HP, int W, int PD>
+template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW>
 __global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vacc_body<S, R, RV, NT, HP, W, PD>(p, step_io(p), w);
+    stencil_vacc_body<S, R, RV, NT, HP, W, PD, VW>(p, step_io(p), w);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -1269,7 +1278,7 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
     if constexpr (KIND == 1) {
         stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
     } else if constexpr (KIND == 2) {
-        if constexpr (!Tr<S>::C) stencil_vec_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
+        if constexpr (!Tr<S>::C) stencil_vec_body<S, R, RP, NT, HP, W, int(16 / sizeof(S))>(r.q.b, io, w - r.item0);
     } else {
         stencil_step_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
     }
@@ -1312,14 +1321,16 @@ __global__ void __launch_bounds__(256) k_nl_scatter_batch(const BatchRec<typenam
 // ------------------------------------------------------ Gram reduction --
 // Sum of split-K partial Gram products in a fixed order, times alpha.
 template <class S>
-__global__ void k_gram_reduce(const S *__restrict__ work, int nsplit, int n, double alpha, S *C)
+__global__ void k_gram_reduce(const S *__restrict__ work, int nsplit, int n, double alpha,
+                              typename Wide<S>::T *C)
 {
+    using W = typename Wide<S>::T;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nn = (long)n * n;
     if (idx >= nn) return;
-    S s = zero<S>();
-    for (int z = 0; z < nsplit; ++z) s = add(s, work[(long)z * nn + idx]);
-    C[idx] = scl((typename Tr<S>::A)alpha, s);
+    W s = zero<W>();
+    for (int z = 0; z < nsplit; ++z) s = add(s, widen(work[(long)z * nn + idx]));
+    C[idx] = scl(alpha, s);
 }
 
 
@@ -1548,16 +1559,27 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     }
 }
 
-template <class S, int RV, int NT, int MODE>
+// run width of the vector stencil for rows of nx points: 16-byte runs, else
+// (floats) 8-byte runs, else 0 (no vector form)
+template <class S>
+int vec_width(int nx)
+{
+    constexpr int V16 = int(16 / sizeof(S));
+    if (nx % V16 == 0) return V16;
+    if (V16 == 4 && nx % 2 == 0) return 2;
+    return 0;
+}
+
+template <class S, int VW, int RV, int NT, int MODE>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                      int nzch, bool align, hipStream_t st)
 {
-    constexpr int VW = int(16 / sizeof(S));
     constexpr int XO = (kMaxR + VW - 1) / VW * VW;
     constexpr int HS = NT >= 512 ? 1 : 2;      // halo runs per thread
     constexpr int HB = NT >= 512 ? 2 : 3;
-    auto aligned = [](const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0; };
+    constexpr uintptr_t AM = sizeof(S) * VW - 1;
+    auto aligned = [](const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & AM) == 0; };
     if (g.nx % VW || (ldx % VW) || (old && ldo % VW) || (ldout % VW) || !aligned(x) || (old && !aligned(old)) ||
         !aligned(out) || !aligned(veff))
         return false;
@@ -1590,19 +1612,19 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), shm, st, p); };
     if constexpr (MODE > 0) {
         if (pitch == 64) {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64, MODE>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64, MODE>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64, MODE, VW>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64, MODE, VW>);
         } else {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128, MODE>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128, MODE>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128, MODE, VW>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128, MODE, VW>);
         }
     } else {
         if (pitch == 64) {
-            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 64>);
-            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 64>);
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 64, VW>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 64, VW>);
         } else {
-            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 128>);
-            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 128>);
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 128, VW>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 128, VW>);
         }
     }
     return true;
@@ -1628,14 +1650,13 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
             if constexpr (Tr<S>::C) {
                 ok = false;
             } else {
-                constexpr int VW = int(16 / sizeof(S));
-                constexpr int XO = (kMaxR + VW - 1) / VW * VW;
+                const int VW = vec_width<S>(nx);
+                const int XO = VW ? (kMaxR + VW - 1) / VW * VW : 0;
                 const int hb = nt >= 512 ? 2 : 3;
-                const int runs = nx / VW;
+                const int runs = VW ? nx / VW : 0;
                 const int lpr = cfg.align ? (runs + 15) / 16 * 16 : runs;
                 const int lanes = nt * (cfg.rp_div >= 2 ? 1 : 2);
-                ok = nx % VW == 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128 &&
-                     nx >= 2 * XO;
+                ok = VW > 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128 && nx >= 2 * XO;
             }
         } else {
             const long pts = (long)nt * (base_rp<S>() / std::max(1, cfg.rp_div));
@@ -1671,8 +1692,16 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
             // real types: 16-byte runs; base RV = 2 runs, RV = 1 for rp_div 2
             if (cfg.vec) {
                 bool ok;
+                const int vw = vec_width<S>(g.nx);
                 auto run = [&](auto rv, auto nt, auto acc) {
-                    return launch_step_vec<S, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value>(
+                    constexpr int V16 = int(16 / sizeof(S));
+                    if constexpr (V16 == 4) {
+                        if (vw == 2)
+                            return launch_step_vec<S, 2, decltype(rv)::value, decltype(nt)::value,
+                                                   decltype(acc)::value>(g, x, ldx, old, ldo, out, ldout, veff, ncol,
+                                                                         a, b, s, cfg.nzch, cfg.align != 0, st);
+                    }
+                    return launch_step_vec<S, V16, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value>(
                         g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
                 };
                 using I1 = std::integral_constant<int, 1>;
@@ -2079,7 +2108,7 @@ void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void 
         using S = decltype(t);
         const long nn = (long)n * n;
         hipLaunchKernelGGL((k_gram_reduce<S>), dim3((nn + 255) / 256), dim3(256), 0, st, (const S *)work, nsplit, n,
-                           alpha, (S *)C);
+                           alpha, (typename Wide<S>::T *)C);
     });
 }
 

@@ -99,7 +99,8 @@ void batch_step(BatchPlan *p, int step, int ix, int iold, int iout, hipStream_t 
 int batch_launches_per_step(const BatchPlan *p);
 
 // C (n x n) = alpha * sum of nsplit consecutive n x n partial products in work
-// (fixed order).
+// (fixed order).  C is FP64 (F64 or C128) whatever the precision e of the
+// partials: single-precision partials are summed in double precision.
 void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void *C, hipStream_t st);
 // Small dense helpers on n x n column-major matrices.
 void hermitian_part(int e, void *H, int n, hipStream_t st);

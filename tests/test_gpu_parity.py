@@ -262,10 +262,11 @@ def test_errors_are_reported():
     assert st == L.E_NONFINITE
 
 
-@pytest.mark.parametrize("n", [(24, 24, 24), (60, 40, 26), (90, 30, 24)])
+@pytest.mark.parametrize("n", [(24, 24, 24), (60, 40, 26), (90, 30, 24), (26, 36, 24)])
 def test_every_stencil_launch_shape(n):
     # each launch shape the tuner may pick (points per thread, z chunks,
-    # halo slots, LDS pitch) gives the oracle's result, real and complex
+    # halo slots, LDS pitch) gives the oracle's result, real and complex;
+    # nx = 90 and 26 take 8-byte float runs (nx not a multiple of four)
     from lynx_amd import _lib as L
     k = (0.25 * 2 * np.pi / A_SI, 0.0, -0.125 * 2 * np.pi / A_SI)
     s = System(n, A_SI, DIAMOND, kpt=k)

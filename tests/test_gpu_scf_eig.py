@@ -55,3 +55,25 @@ def test_eigen_stage_fp32_until_residual(si8_16):
         # Rayleigh-Ritz keeps the eigenvalues within 1e-5 Ha of the FP64 run
         assert np.abs(emx - e64).max() < 1e-5, it
         assert abs(efmx - ef64) < 1e-5
+
+
+def test_eigen_stage_fp32_complex_kpoints(si8_16):
+    # the mixed-precision pass in complex arithmetic (complex64 filter and
+    # Nd-sized products, complex128 Cholesky and eigensolve) on two k-points
+    from systems import A_SI
+    from lynx_amd import core
+    s = si8_16
+    kp = 2 * np.pi / A_SI * np.array([[0.0, 0.0, 0.0], [0.25, -0.125, 0.375]])
+    w = np.array([0.5, 0.5])
+    H64, H32 = s.hamiltonian(), s.hamiltonian()
+    X0 = s.random_block(NB, 83, complex_=True)
+    d64 = core.ScfEigenStage(H64, NB, NE, BETA, kpts=kp, kpt_weights=w)
+    dmx = core.ScfEigenStage(H32, NB, NE, BETA, kpts=kp, kpt_weights=w, fp64_switch_residual=1e-4)
+    p64 = [X0.copy(order="F"), X0.copy(order="F")]
+    pmx = [X0.copy(order="F"), X0.copy(order="F")]
+    for it, res in enumerate((1.0, 1e-3)):
+        e64, _, ef64, _ = d64.step(it, res, s.veff, p64)
+        emx, _, efmx, prmx = dmx.step(it, res, s.veff, pmx)
+        assert prmx == 1
+        assert np.abs(emx - e64).max() < 1e-5, it
+        assert abs(efmx - ef64) < 1e-5
