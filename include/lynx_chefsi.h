@@ -199,11 +199,14 @@ lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int la
 /* Enable (1) or disable (0) HIP-event timing of filter calls. */
 lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
 
-/* Precision of the Chebyshev filter inside lynx_chefsi_solve.  FP32 filters
- * a single-precision copy of psi (complex64 for complex solves) and returns
- * to FP64 for Cholesky-QR, the projected eigenproblem and the rotation, which
- * always run in FP64.  No reference counterpart: the reference filters in
- * FP64 only (EigenSolver.cpp:558-619). */
+/* Precision of the pass inside lynx_chefsi_solve.  FP32 filters a
+ * single-precision copy of psi (complex64 for complex solves) and keeps the
+ * Nd-sized steps in single precision: the overlap and projected-Hamiltonian
+ * products (partial sums added in FP64), H applied to the orthonormalised
+ * block, and the two rotations (by R^-1 and by the Ritz vectors, rounded to
+ * FP32).  The N x N steps run in FP64: Cholesky, triangular inverse and the
+ * dense eigensolve.  psi is returned in FP64.  No reference counterpart: the
+ * reference runs the pass in FP64 only (EigenSolver.cpp:558-681). */
 enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1 };
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
 /* Grid point counts nx, ny, nz of the context. */
