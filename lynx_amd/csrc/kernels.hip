@@ -912,11 +912,19 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     }
 
     const int f_end = z1 - 1 + R;
+    // Odd z-chunks run downwards: plane f of the loop is physical plane
+    // z0 + z1 - 1 - f (the z-stencil is symmetric, so the accumulation is
+    // the same with the roles of +d and -d swapped).  Neighbouring chunks of
+    // a column then reach their shared boundary planes (the R planes each
+    // reads beyond its own range) at the same time, at both ends, and the
+    // second read of those planes hits L2 instead of HBM.
+    const bool down = (zc & 1) != 0;
+    auto phys = [&](int f) { return down ? z0 + z1 - 1 - f : f; };
     // loads for plane f: its owned runs (z-wrapped), and, for planes of the
     // chunk, its halo runs and Veff; Y_{k-2} of output plane f - R
     auto issue = [&](int f, VecLoads<S, VW, RV, HP> &L) {
         f = min(f, f_end);
-        int zz = f;
+        int zz = phys(f);
         if (zz < 0) zz += nz;
         else if (zz >= nz) zz -= nz;
         const S *Xz = X + zz * nxny;
@@ -929,7 +937,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
             for (int r = 0; r < RV; ++r) L.v[r] = vld<VW>(p.veff + zz * nxny, qoff[r]);
         }
         if (has_old && f - R >= z0) {
-            const int zo = (f - R) * nxny;
+            const int zo = phys(f - R) * nxny;
 #pragma unroll
             for (int r = 0; r < RV; ++r) L.old[r] = vld<VW>(O + zo, qoff[r]);
         }
@@ -999,7 +1007,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
                 for (int e = 0; e < VW; ++e) acc[r][j].v[e] = fm(p.c[2][d], cur.front[r].v[e], acc[r][j].v[e]);
             }
         if (f - R >= z0) {
-            S *Yz = Y + (f - R) * nxny;
+            S *Yz = Y + phys(f - R) * nxny;
 #pragma unroll
             for (int r = 0; r < RV; ++r) {
                 VRun<S, VW> res;

@@ -278,7 +278,7 @@ def test_every_stencil_launch_shape(n):
     ref_r = op.apply(Xr, s.veff, c=0.1)
     op.set_kpoint(k)
     ref_c = op.apply(Xc, s.veff, c=0.1)
-    shapes = [(v, t, d, z) for v in (0, 1, 2, 3, 5, 6, 7) for t in (256, 512) for d in (1, 2) for z in (1, 2, 4)]
+    shapes = [(v, t, d, z) for v in (0, 1, 2, 3, 5, 6, 7) for t in (256, 512) for d in (1, 2) for z in (1, 2, 3, 4)]
     for shape in shapes:
         L.check(L.lib().lynx_set_stencil_config(H.ctx, *shape[1:], shape[0]), H.ctx)
         H.set_kpoint((0, 0, 0))
