@@ -343,6 +343,18 @@ def run_mixed_si216():
             "band_energy_err_Ha_per_atom": float(abs((2.0 * f64 * (emx - e64)).sum()) / 216)}
 
 
+def stream_ceiling(sysm):
+    """Practical HBM ceiling of the stencil step's traffic on this device
+    (lynx_stream_ceiling: x, Y_(k-2) and y streams of the same 60^3 x 160
+    FP64 shape plus an L2-resident Veff, no stencil), in the algorithmic
+    bytes of the step per second."""
+    from lynx_amd import _lib as L
+    H = sysm.hamiltonian(0)
+    g = ctypes.c_double()
+    L.check(L.lib().lynx_stream_ceiling(H.ctx, sysm.nd, NCOL, ctypes.byref(g)), H.ctx)
+    return round(g.value, 1)
+
+
 def cpu_baseline(sysm, X64, lam, lo, hi):
     """The oracle (single-threaded C restatement) on the same workload: the
     same grid, atoms, degree and all 160 columns (about 10 s of one core)."""
@@ -374,7 +386,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} (WORLD_SIZE {world}) is not run: the filter "
-              "shards over k-points/structures/bands with no communication inside the filter (see DESIGN.md)",
+              "shards over k-points/structures/bands with no communication inside the filter",
               file=sys.stderr)
         sys.exit(2)
 
@@ -413,6 +425,7 @@ def main():
         cpu = cpu_baseline(sysm, X64, 0.5 * (lo + hi), lo, hi)
 
     achieved = r64["stencil_gbs"]
+    ceiling = stream_ceiling(sysm)
     kname = {0: "k_stencil_step<double", 1: "k_stencil_vec<double", 2: "k_stencil_vacc<double",
              3: "k_stencil_vacc<double"}[3 &
         r64["stencil_config"]["vector_runs"]]
@@ -449,6 +462,8 @@ def main():
             + " (fused stencil + Veff + Chebyshev recurrence)",
             "avg_launch_ms": round(r64["stencil_avg_launch_ms"], 5),
             "launch_shape": r64["stencil_config"],
+            "stream_ceiling": ceiling,
+            "frac_of_stream_ceiling": round(achieved / ceiling, 4),
         },
         "cpu_baseline": cpu,
         "variants": variants,

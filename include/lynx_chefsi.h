@@ -198,6 +198,13 @@ lynx_status lynx_psp8_info(const char *path, double *zion, int *lmax, int *lloc,
 lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int launches[4]);
 /* Enable (1) or disable (0) HIP-event timing of filter calls. */
 lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
+/* The practical HBM ceiling for the FP64 stencil step's access pattern on
+ * this device: a streaming kernel that reads x and Y_(k-2) and writes y
+ * (nd x ncol doubles each, 16-byte accesses) and re-reads one nd-sized Veff
+ * column from L2, best of three grid sizes.  *gbps counts the step's
+ * algorithmic bytes (3 x 8 x nd x ncol + 8 x nd) per second.  Allocates and
+ * frees its own buffers.  No reference counterpart (a measurement aid). */
+lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps);
 
 /* Precision of the pass inside lynx_chefsi_solve.  FP32 filters a
  * single-precision copy of psi (complex64 for complex solves) and keeps the

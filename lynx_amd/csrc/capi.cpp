@@ -1398,6 +1398,45 @@ lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
     return LYNX_OK;
 }
 
+lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps)
+{
+    if (!ctx || !gbps) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        if (nd <= 0 || ncol <= 0 || nd % 2) fail(LYNX_E_ARG, "nd must be positive and even, ncol positive");
+        const long n16 = nd * ncol / 2, nv16 = nd / 2;
+        struct Bufs {
+            DevBuf b[4];
+            ~Bufs() { for (DevBuf &d : b) d.release(); }
+        } bufs;
+        DevBuf &x = bufs.b[0], &o = bufs.b[1], &v = bufs.b[2], &y = bufs.b[3];
+        x.get((size_t)n16 * 16);
+        o.get((size_t)n16 * 16);
+        y.get((size_t)n16 * 16);
+        v.get((size_t)nv16 * 16);
+        HIP_OK(hipMemsetAsync(x.p, 0, (size_t)n16 * 16, ctx->stream));
+        HIP_OK(hipMemsetAsync(o.p, 0, (size_t)n16 * 16, ctx->stream));
+        HIP_OK(hipMemsetAsync(v.p, 0, (size_t)nv16 * 16, ctx->stream));
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int blocks : {1024, 4096, 16384})
+            for (int rep = 0; rep < 6; ++rep) {
+                HIP_OK(hipEventRecord(e0, ctx->stream));
+                gpu::stream_probe(x.p, o.p, v.p, y.p, n16, nv16, blocks, ctx->stream);
+                HIP_OK(hipEventRecord(e1, ctx->stream));
+                HIP_OK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) best = std::min(best, ms);
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        *gbps = (3.0 * 8.0 * nd * ncol + 8.0 * nd) / (best * 1e-3) / 1e9;
+    });
+}
+
 }  // extern "C"
 
 // ----------------------------------------------------------------- batches --

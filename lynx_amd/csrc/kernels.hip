@@ -1445,6 +1445,19 @@ __global__ void k_convert_cols(const T *src, long lds, D *dst, long ldd, long n)
         d[i] = static_cast<D>(s[i]);
 }
 
+// Streaming probe with the stencil step's global traffic and no stencil:
+// y = a x + b o + v, x/o/y column blocks of n 16-byte vectors, v (one
+// column, nv vectors) re-read per column from L2.  Its rate is the
+// practical HBM ceiling for the step's access pattern.
+__global__ void k_stream3(const double2 *__restrict__ x, const double2 *__restrict__ o,
+                          const double2 *__restrict__ v, double2 *__restrict__ y, long n, long nv)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double2 a = x[i], b = o[i], c = v[i % nv];
+        y[i] = make_double2(fma(0.5, a.x, fma(0.25, b.x, c.x)), fma(0.5, a.y, fma(0.25, b.y, c.y)));
+    }
+}
+
 // ---------------------------------------------------------- dispatch helpers --
 template <class F>
 void dispatch(int e, F &&f)
@@ -2177,6 +2190,13 @@ void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, i
         hipLaunchKernelGGL((k_copy_cols<S>), dim3(nb, ncol), dim3(256), 0, st, (const S *)src, lds, (S *)dst,
                            ldd, nd);
     });
+}
+
+void stream_probe(const void *x, const void *o, const void *v, void *y, long n16, long nv16, int blocks,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stream3, dim3(blocks), dim3(256), 0, st, (const double2 *)x, (const double2 *)o,
+                       (const double2 *)v, (double2 *)y, n16, nv16);
 }
 
 void convert_cols(int e_src, const void *src, long lds, int e_dst, void *dst, long ldd, long nd, int ncol,

@@ -119,6 +119,10 @@ void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, i
                hipStream_t st);
 // Same, converting between precisions of the same kind (FP64 <-> FP32,
 // complex128 <-> complex64).
+// y = a x + b o + v over n16 16-byte vectors (v: nv16 vectors, re-read
+// cyclically): the stencil step's global traffic without the stencil.
+void stream_probe(const void *x, const void *o, const void *v, void *y, long n16, long nv16, int blocks,
+                  hipStream_t st);
 void convert_cols(int e_src, const void *src, long lds, int e_dst, void *dst, long ldd, long nd, int ncol,
                   hipStream_t st);
 constexpr int kPartialBlocks = 512;
