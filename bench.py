@@ -305,7 +305,7 @@ def run_mixed_si216():
     Cholesky-QR and Rayleigh-Ritz stay FP64.  The band-energy difference per
     atom uses the occupations of the FP64 run (Fermi-Dirac, 1000 K)."""
     from systems import System, displace, supercell
-    from lynx_amd import core
+    from lynx_amd import _lib as L, core
     from lynx_amd.hipmem import DeviceArray, synchronize
     a = 30.78
     s = System(90, a, displace(supercell(3), a, 0.03, seed=3))
@@ -326,8 +326,10 @@ def run_mixed_si216():
         e1, occ1, ef1, p1 = drv.step_device(1, 1e-3, s.veff, [d.value], s.nd)
         synchronize()
         t2 = time.perf_counter()
+        nfb, nq2 = ctypes.c_long(), ctypes.c_long()
+        L.check(L.lib().lynx_mixed_fallbacks(H.ctx, ctypes.byref(nfb), ctypes.byref(nq2)), H.ctx)
         out[name] = {"iter0_s": round(t1 - t0, 3), "iter1_s": round(t2 - t1, 3), "filter_precision": [p0, p1],
-                     "fermi": ef1}
+                     "fermi": ef1, "fp32_passes_redone_in_fp64": nfb.value, "cholesky_qr2_passes": nq2.value}
         eigs[name] = (e1, occ1)
         d.free()
         del drv, H

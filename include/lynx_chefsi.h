@@ -216,6 +216,15 @@ lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps);
  * reference runs the pass in FP64 only (EigenSolver.cpp:558-681). */
 enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1 };
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
+/* Counters of the FP32 passes of this context.  *count: passes redone in
+ * FP64 from the unchanged psi, because the filtered block was too
+ * ill-conditioned for single precision (Cholesky factor diagonal spread above
+ * 100x, overlap not positive definite, or Ritz values outside the spectral
+ * bounds), as in the first passes from a random start.  *cholqr2 (may be
+ * null): passes whose factor spread its diagonal by 10-100x and that took a
+ * second Cholesky-QR of the orthonormalised block.  No reference
+ * counterpart. */
+lynx_status lynx_mixed_fallbacks(const lynx_ctx *ctx, long *count, long *cholqr2);
 /* Grid point counts nx, ny, nz of the context. */
 lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3]);
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
     "lynx_stream_ceiling": (_I, [_P, _L, _I, _DP]),
+    "lynx_mixed_fallbacks": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)]),
     "lynx_set_filter_precision": (_I, [_P, _I]),
     "lynx_get_grid_dims": (_I, [_P, _IP]),
     "lynx_scf_eig_create": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _DP]),

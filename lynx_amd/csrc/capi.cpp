@@ -107,6 +107,8 @@ struct lynx_ctx {
 
     // precision of the filter inside lynx_chefsi_solve (LYNX_FILTER_*)
     int filter_precision = 0;
+    long fp64_overlap_fallbacks = 0;   // FP32 passes redone in FP64
+    long cholqr2_passes = 0;           // FP32 passes with a second Cholesky-QR
 
     bool timing = false;
     double t_last[4] = {0, 0, 0, 0};
@@ -531,6 +533,15 @@ long padded_ld(long nd) { return (nd + 63) / 64 * 64; }
 // that a few hundred workgroups cover the small n x n output.
 int wide_of(int e) { return (e == gpu::F32) ? gpu::F64 : (e == gpu::C64 ? gpu::C128 : e); }
 
+// Mixed pass, by the spread of the Cholesky factor's diagonal (a lower bound
+// of the block's condition number kappa): one Cholesky-QR leaves the block
+// orthonormal to about kappa^2 times the FP32 rounding of the overlap
+// (~1e-6 with partial sums of a few thousand terms); a second pass restores
+// it while kappa^2 * 1e-6 << 1.  Beyond kCholQrFp64Ratio the pass is redone
+// in FP64.
+constexpr double kCholQr2Ratio = 10.0;
+constexpr double kCholQrFp64Ratio = 100.0;
+
 // Single-precision operands (the mixed-precision pass) use up to 64 chunks,
 // so that each partial product sums at most a few thousand terms in FP32
 // before the FP64 reduction.  C is FP64 (wide_of(e)).
@@ -646,6 +657,24 @@ void cholesky_inverse(lynx_ctx *c, int e, const void *X, long ldx, int n)
     }
     check_info(c, LYNX_E_CHOLESKY, "triangular inverse");
     gpu::zero_lower(e, S, n, c->stream);
+}
+
+// Spread max|r_ii| / min|r_ii| of the Cholesky factor whose inverse
+// cholesky_inverse left in c->s1 (FP64 or complex128, n x n): a lower bound
+// of the condition number of the factorised block.
+double triangular_diag_ratio(lynx_ctx *c, int ew, int n)
+{
+    const size_t es = gpu::elem_size(ew);
+    std::vector<double> d(ew == gpu::C128 ? 2 * n : n);
+    HIP_OK(hipMemcpy2DAsync(d.data(), es, c->s1.p, (n + 1) * es, es, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    double lo = 1e300, hi = 0;
+    for (int i = 0; i < n; ++i) {
+        const double v = ew == gpu::C128 ? std::hypot(d[2 * i], d[2 * i + 1]) : std::fabs(d[i]);
+        lo = std::min(lo, v);
+        hi = std::max(hi, v);
+    }
+    return lo > 0 ? hi / lo : 1e300;
 }
 
 // Hermitian eigensolve of c->s2 (n x n) in place; eigenvalues to host.
@@ -1143,28 +1172,64 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
             timing_begin(ctx);
             filter(ctx, es1, Xs, ldw, Ys, ldw, Ws, ldw, nband, degree, lambda_c, eig_min, eig_max);
             timing_end(ctx);
-            // a9: Cs = Ys R^-1 with dV Ys^H Ys = R^H R (R in FP64)
-            cholesky_inverse(ctx, es1, Ys, ldw, nband);
-            void *Rs = ctx->s3.get((size_t)nband * nband * gpu::elem_size(es1));
-            gpu::convert_cols(dtype, ctx->s1.p, nband, es1, Rs, nband, nband, nband, ctx->stream);
-            void *Cs = Xs;
-            rotate(ctx, es1, Ys, ldw, Rs, nband, Cs, ldw);
-            // a10: Hs = dV Cs^H H Cs
-            apply_op(ctx, es1, Cs, ldw, nullptr, 0, Ws, ldw, nband, 1.0, 0.0, 0.0);
-            void *H = ctx->s2.get((size_t)nband * nband * es);
-            gram(ctx, es1, Cs, ldw, Ws, ldw, nband, ctx->st.dV, H);
-            gpu::hermitian_part(dtype, H, nband, ctx->stream);
-            // a11 in FP64; a12: psi = Cs Q
-            subspace_eig(ctx, dtype, nband, eigvals);
-            gpu::convert_cols(dtype, H, nband, es1, Rs, nband, nband, nband, ctx->stream);
-            rotate(ctx, es1, Cs, ldw, Rs, nband, Ys, ldw);
-            gpu::convert_cols(es1, Ys, ldw, dtype, P, ldP, nd, nband, ctx->stream);
-            if (mem != LYNX_MEM_DEVICE)
-                HIP_OK(hipMemcpy2DAsync(psi, ldp * es, P, nd * es, nd * es, nband, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-            HIP_OK(hipStreamSynchronize(ctx->stream));
-            if (lambda_c_out) *lambda_c_out = eigvals[nband - 1] + 0.1;
-            return;
+            // a9: Cs = Ys R^-1 with dV Ys^H Ys = R^H R (R in FP64).  The
+            // orthonormality of Cs degrades with the square of the block's
+            // condition number times the FP32 rounding of the overlap, so a
+            // block whose R spreads its diagonal by more than kCholQr2Ratio
+            // gets a second Cholesky-QR pass on Cs (CholeskyQR2).  A block
+            // too ill-conditioned for that (spread above kCholQrFp64Ratio, an
+            // overlap that is not positive definite, or Ritz values outside
+            // the spectral bounds; the first passes from a random start,
+            // where the filter aligns the columns with the lowest
+            // eigenvectors) is redone in FP64 from psi, still unchanged.
+            auto fp32_pass = [&]() -> bool {
+                auto chol = [&](const void *Y) {
+                    try {
+                        cholesky_inverse(ctx, es1, Y, ldw, nband);
+                        return true;
+                    } catch (const Error &err) {
+                        if (err.status != LYNX_E_CHOLESKY) throw;
+                        return false;
+                    }
+                };
+                void *Rs = ctx->s3.get((size_t)nband * nband * gpu::elem_size(es1));
+                void *Cs = Xs, *Free = Ys;
+                if (!chol(Ys)) return false;
+                gpu::convert_cols(dtype, ctx->s1.p, nband, es1, Rs, nband, nband, nband, ctx->stream);
+                rotate(ctx, es1, Ys, ldw, Rs, nband, Cs, ldw);
+                const double spread = triangular_diag_ratio(ctx, dtype, nband);
+                if (spread > kCholQrFp64Ratio) return false;
+                if (spread > kCholQr2Ratio) {
+                    if (!chol(Cs)) return false;
+                    gpu::convert_cols(dtype, ctx->s1.p, nband, es1, Rs, nband, nband, nband, ctx->stream);
+                    rotate(ctx, es1, Cs, ldw, Rs, nband, Free, ldw);
+                    std::swap(Cs, Free);
+                    ++ctx->cholqr2_passes;
+                }
+                // a10: Hs = dV Cs^H H Cs
+                apply_op(ctx, es1, Cs, ldw, nullptr, 0, Ws, ldw, nband, 1.0, 0.0, 0.0);
+                void *H = ctx->s2.get((size_t)nband * nband * es);
+                gram(ctx, es1, Cs, ldw, Ws, ldw, nband, ctx->st.dV, H);
+                gpu::hermitian_part(dtype, H, nband, ctx->stream);
+                // a11 in FP64
+                subspace_eig(ctx, dtype, nband, eigvals);
+                if (!(eigvals[0] >= eig_min - 0.05 * (eig_max - eig_min)) || !(eigvals[nband - 1] <= eig_max))
+                    return false;
+                // a12: psi = Cs Q
+                gpu::convert_cols(dtype, H, nband, es1, Rs, nband, nband, nband, ctx->stream);
+                rotate(ctx, es1, Cs, ldw, Rs, nband, Free, ldw);
+                gpu::convert_cols(es1, Free, ldw, dtype, P, ldP, nd, nband, ctx->stream);
+                return true;
+            };
+            if (fp32_pass()) {
+                if (mem != LYNX_MEM_DEVICE)
+                    HIP_OK(hipMemcpy2DAsync(psi, ldp * es, P, nd * es, nd * es, nband, hipMemcpyDeviceToHost,
+                                            ctx->stream));
+                HIP_OK(hipStreamSynchronize(ctx->stream));
+                if (lambda_c_out) *lambda_c_out = eigvals[nband - 1] + 0.1;
+                return;
+            }
+            ++ctx->fp64_overlap_fallbacks;
         }
         timing_begin(ctx);
         // a8: Y = p_m(H) psi -> B
@@ -1395,6 +1460,14 @@ lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
 {
     if (!ctx) return LYNX_E_ARG;
     ctx->timing = enable != 0;
+    return LYNX_OK;
+}
+
+lynx_status lynx_mixed_fallbacks(const lynx_ctx *ctx, long *count, long *cholqr2)
+{
+    if (!ctx || !count) return LYNX_E_ARG;
+    *count = ctx->fp64_overlap_fallbacks;
+    if (cholqr2) *cholqr2 = ctx->cholqr2_passes;
     return LYNX_OK;
 }
 

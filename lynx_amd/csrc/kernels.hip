@@ -1007,7 +1007,8 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
                 for (int e = 0; e < VW; ++e) acc[r][j].v[e] = fm(p.c[2][d], cur.front[r].v[e], acc[r][j].v[e]);
             }
         if (f - R >= z0) {
-            S *Yz = Y + phys(f - R) * nxny;
+            const int zp = phys(f - R);
+            S *Yz = Y + zp * nxny;
 #pragma unroll
             for (int r = 0; r < RV; ++r) {
                 VRun<S, VW> res;

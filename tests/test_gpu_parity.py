@@ -286,8 +286,37 @@ def test_every_stencil_launch_shape(n):
         assert rel(got, ref_r) < 1e-12, shape
         got = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k, 0.1).reshape(2, -1).T
         assert rel(got, ref_c) < 1e-12, shape
+        g64 = H._apply(Xc.T.copy(), s.veff, 0.1, np.complex64).reshape(2, -1).T
+        assert rel(g64, ref_c) < 2e-6, shape
         H.set_kpoint((0, 0, 0))
         g32 = H._apply(Xr.T.copy(), s.veff, 0.1, np.float32).reshape(2, -1).T
+        assert rel(g32, ref_r) < 2e-6, shape
+    L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0, 0), H.ctx)
+
+
+def test_every_launch_shape_twenty_columns():
+    # the single-precision kernels on a block as wide as the eigen-stage
+    # tests use (20 columns: several column groups per launch), every shape
+    from lynx_amd import _lib as L
+    k = (0.25 * 2 * np.pi / A_SI, -0.125 * 2 * np.pi / A_SI, 0.375 * 2 * np.pi / A_SI)
+    s = System(16, A_SI, DIAMOND, kpt=k)
+    op, H = s.oracle_op(), s.hamiltonian()
+    Xr = s.random_block(20, 23)
+    Xc = s.random_block(20, 24, complex_=True)
+    op.set_kpoint((0, 0, 0))
+    ref_r = op.apply(Xr, s.veff, c=0.1)
+    op.set_kpoint(k)
+    ref_c = op.apply(Xc, s.veff, c=0.1)
+    shapes = [(v, t, d, z) for v in (0, 3, 7) for t in (256, 512) for d in (1, 2) for z in (1, 2, 3, 4)]
+    for shape in shapes:
+        L.check(L.lib().lynx_set_stencil_config(H.ctx, *shape[1:], shape[0]), H.ctx)
+        H.set_kpoint(k)
+        g64 = H._apply(Xc.T.copy(), s.veff, 0.1, np.complex64).reshape(20, -1).T
+        assert rel(g64, ref_c) < 2e-6, shape
+        got = H._apply(Xc.T.copy(), s.veff, 0.1, np.complex128).reshape(20, -1).T
+        assert rel(got, ref_c) < 1e-12, shape
+        H.set_kpoint((0, 0, 0))
+        g32 = H._apply(Xr.T.copy(), s.veff, 0.1, np.float32).reshape(20, -1).T
         assert rel(g32, ref_r) < 2e-6, shape
     L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0, 0), H.ctx)
 
