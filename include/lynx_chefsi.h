@@ -42,7 +42,10 @@ enum lynx_dtype {
     LYNX_F64 = 0,            /* real double (Gamma point)                    */
     LYNX_C128 = 1,           /* complex double (k-point, Bloch)              */
     LYNX_F32 = 2,            /* real float                                   */
-    LYNX_C64 = 3             /* complex float                                */
+    LYNX_C64 = 3,            /* complex float                                */
+    LYNX_BF16 = 4            /* real bfloat16 storage, float arithmetic:
+                                lynx_h_apply and lynx_cheb_filter only, on
+                                orthogonal cells with an even row length    */
 };
 
 enum lynx_mem { LYNX_MEM_HOST = 0, LYNX_MEM_DEVICE = 1 };
@@ -212,9 +215,12 @@ lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps);
  * products (partial sums added in FP64), H applied to the orthonormalised
  * block, and the two rotations (by R^-1 and by the Ritz vectors, rounded to
  * FP32).  The N x N steps run in FP64: Cholesky, triangular inverse and the
- * dense eigensolve.  psi is returned in FP64.  No reference counterpart: the
- * reference runs the pass in FP64 only (EigenSolver.cpp:558-681). */
-enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1 };
+ * dense eigensolve.  psi is returned in FP64.  BF16 is FP32 with the filter's
+ * blocks stored in bfloat16 (float arithmetic, FP32 Veff and projectors),
+ * for real solves on orthogonal cells with an even row length; otherwise it
+ * runs as FP32.  No reference counterpart: the reference runs the pass in
+ * FP64 only (EigenSolver.cpp:558-681). */
+enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1, LYNX_FILTER_BF16 = 2 };
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
 /* Counters of the FP32 passes of this context.  *count: passes redone in
  * FP64 from the unchanged psi, because the filtered block was too
@@ -261,8 +267,9 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
  * nchefsi; after each pass eig_min[s] <- eig(s, k0)[0] (iterations > 0),
  * lambda_cutoff <- max_s eig(s, k0)[N-1] + 0.1, and the occupations and Fermi
  * level are recomputed (Occupation.cpp:53-176).  With fp64_switch_residual
- * > 0 the filter runs in FP32 in iteration 0 and while scf_residual exceeds
- * it, and in FP64 afterwards; the subspace steps are always FP64. */
+ * > 0 the pass runs in reduced precision (low_precision: LYNX_FILTER_FP32,
+ * the default when 0, or LYNX_FILTER_BF16; see lynx_set_filter_precision) in
+ * iteration 0 and while scf_residual exceeds it, and in FP64 afterwards. */
 typedef struct lynx_scf_eig lynx_scf_eig;
 typedef struct {
     int nspin;                    /* 1 or 2 */
@@ -276,6 +283,7 @@ typedef struct {
     double beta;                  /* 1 / (k_B T) in 1/Ha */
     int smearing;                 /* 0 Fermi-Dirac, 1 Gaussian */
     double fp64_switch_residual;  /* 0: FP64 filter throughout */
+    int low_precision;            /* before the switch: LYNX_FILTER_FP32 (or 0) or LYNX_FILTER_BF16 */
 } lynx_scf_eig_params;
 lynx_status lynx_scf_eig_create(lynx_scf_eig **driver, lynx_ctx *ctx, const lynx_scf_eig_params *params,
                                 const double *kpts, const double *kpt_weights);

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblynx_chefsi.so")
 
 OK, E_ARG, E_HIP, E_CHOLESKY, E_EIG, E_NONFINITE, E_IO, E_STATE, E_UNSUPPORTED = range(9)
-F64, C128, F32, C64 = 0, 1, 2, 3
+F64, C128, F32, C64, BF16 = 0, 1, 2, 3, 4
+FILTER_FP64, FILTER_FP32, FILTER_BF16 = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 CELL_ORTHOGONAL, CELL_NONORTHOGONAL = 0, 1
 

@@ -174,7 +174,7 @@ void need_veff(const lynx_ctx *c)
 
 void check_dtype(int dtype)
 {
-    if (dtype < 0 || dtype > 3) fail(LYNX_E_ARG, "unknown dtype");
+    if (dtype < 0 || dtype > 4) fail(LYNX_E_ARG, "unknown dtype");
 }
 
 gpu::StencilGeom geom_at(const lynx_ctx *c, const double *kpt);
@@ -466,6 +466,17 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
     return c->tuned[key] = best;
 }
 
+// The bfloat16 filter runs on the run-vector stencil kernels only: an
+// orthogonal cell whose rows hold an even number of points that fits one of
+// their shapes.
+bool bf16_filter_fits(const lynx_ctx *c)
+{
+    if (!c->lat.orth) return false;
+    for (const auto &k : step_candidates(c, gpu::B16))
+        if (k.vec) return true;
+    return false;
+}
+
 // ------------------------------------------------------------- operator --
 // y = a * (H - shift) x + b * old, H = -1/2 lap + Veff + V_nl.
 void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long ldo, void *y, long ldy,
@@ -475,7 +486,7 @@ void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long
     const bool nl = c->has_nl;
     const gpu::NlDev &nd = nl_for(c, e);
     if (nl) {
-        c->beta.get((size_t)ncol * nd.n_atom * nd.npm * gpu::elem_size(e));
+        c->beta.get((size_t)ncol * nd.n_atom * nd.npm * gpu::elem_size(gpu::elem_compute(e)));
         Marker m(c, 2);
         gpu::nl_gather(e, nd, x, ldx, c->beta.p, ncol, c->stream);
     }
@@ -1158,20 +1169,38 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
         }
         void *B = block(ctx->wB, dtype, ldw, nband);
         void *C = block(ctx->wC, dtype, ldw, nband);
-        if (ctx->filter_precision == LYNX_FILTER_FP32) {
+        if (ctx->filter_precision != LYNX_FILTER_FP64) {
             // Mixed precision: the filter, the Nd-sized products and the
             // rotations in single precision (inside the FP64 work blocks:
             // two single-precision blocks per FP64 block); the overlap and
             // projected matrices summed and factorised in FP64 (Cholesky,
-            // triangular inverse, eigensolve).
+            // triangular inverse, eigensolve).  With LYNX_FILTER_BF16 (real
+            // blocks whose rows allow the run-vector stencil) the filter
+            // stores its blocks in bfloat16 (arithmetic in float) and the
+            // filtered block is widened to FP32 for the rest of the pass.
             const int es1 = dtype == LYNX_F64 ? LYNX_F32 : LYNX_C64;
             const size_t half = (size_t)ldw * nband * gpu::elem_size(es1);
+            const bool bf = ctx->filter_precision == LYNX_FILTER_BF16 && dtype == LYNX_F64 && bf16_filter_fits(ctx);
             void *Xs = B, *Ys = C;
             void *Ws = static_cast<char *>(C) + half;
-            gpu::convert_cols(dtype, P, ldP, es1, Xs, ldw, nd, nband, ctx->stream);
-            timing_begin(ctx);
-            filter(ctx, es1, Xs, ldw, Ys, ldw, Ws, ldw, nband, degree, lambda_c, eig_min, eig_max);
-            timing_end(ctx);
+            if (bf) {
+                // bf16 X, Y and work block in the first quarters of B and C
+                const size_t quarter = (size_t)ldw * nband * 2;
+                void *Xb = B, *Yb = C, *Wb = static_cast<char *>(C) + quarter;
+                gpu::convert_cols(dtype, P, ldP, LYNX_BF16, Xb, ldw, nd, nband, ctx->stream);
+                timing_begin(ctx);
+                filter(ctx, LYNX_BF16, Xb, ldw, Yb, ldw, Wb, ldw, nband, degree, lambda_c, eig_min, eig_max);
+                timing_end(ctx);
+                Ys = B;
+                Xs = static_cast<char *>(B) + half;
+                Ws = C;
+                gpu::convert_cols(LYNX_BF16, Yb, ldw, es1, Ys, ldw, nd, nband, ctx->stream);
+            } else {
+                gpu::convert_cols(dtype, P, ldP, es1, Xs, ldw, nd, nband, ctx->stream);
+                timing_begin(ctx);
+                filter(ctx, es1, Xs, ldw, Ys, ldw, Ws, ldw, nband, degree, lambda_c, eig_min, eig_max);
+                timing_end(ctx);
+            }
             // a9: Cs = Ys R^-1 with dV Ys^H Ys = R^H R (R in FP64).  The
             // orthonormality of Cs degrades with the square of the block's
             // condition number times the FP32 rounding of the overlap, so a
@@ -1448,8 +1477,8 @@ lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3])
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision)
 {
     if (!ctx) return LYNX_E_ARG;
-    if (precision != LYNX_FILTER_FP64 && precision != LYNX_FILTER_FP32) {
-        ctx->err = "filter precision must be LYNX_FILTER_FP64 or LYNX_FILTER_FP32";
+    if (precision != LYNX_FILTER_FP64 && precision != LYNX_FILTER_FP32 && precision != LYNX_FILTER_BF16) {
+        ctx->err = "filter precision must be LYNX_FILTER_FP64, LYNX_FILTER_FP32 or LYNX_FILTER_BF16";
         return LYNX_E_ARG;
     }
     ctx->filter_precision = precision;
@@ -1645,6 +1674,7 @@ lynx_status lynx_batch_filter(lynx_batch *b, int dtype, void *const *x, const lo
     if (!b) return LYNX_E_ARG;
     return bguarded(b, [&] {
         check_dtype(dtype);
+        if (dtype == LYNX_BF16) fail(LYNX_E_UNSUPPORTED, "batches filter in FP64 or FP32");
         const int n = (int)b->units.size();
         if (!x || !y || !ncol || !lambda_c || !eig_min || !eig_max) fail(LYNX_E_ARG, "null batch argument");
         if (degree < 1) fail(LYNX_E_ARG, "degree must be >= 1");

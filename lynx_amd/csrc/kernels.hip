@@ -32,6 +32,31 @@ __device__ __forceinline__ double widen(float x) { return (double)x; }
 __device__ __forceinline__ double2 widen(double2 x) { return x; }
 __device__ __forceinline__ double2 widen(float2 x) { return make_double2(x.x, x.y); }
 
+// bfloat16 storage (the filter's BF16 mode): 8-bit mantissa in memory,
+// arithmetic in float.  Conversion to float is exact; from float it rounds
+// to nearest even (finite values).
+struct bf16 {
+    unsigned short b;
+};
+__device__ __forceinline__ float to_compute(bf16 v) { return __uint_as_float((unsigned)v.b << 16); }
+__device__ __forceinline__ float to_compute(float v) { return v; }
+__device__ __forceinline__ double to_compute(double v) { return v; }
+__device__ __forceinline__ double2 to_compute(double2 v) { return v; }
+__device__ __forceinline__ float2 to_compute(float2 v) { return v; }
+template <class T> struct StoreAs;
+template <> struct StoreAs<bf16> {
+    static __device__ __forceinline__ bf16 from(float f)
+    {
+        unsigned u = __float_as_uint(f);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return bf16{(unsigned short)(u >> 16)};
+    }
+};
+template <> struct StoreAs<float> { static __device__ __forceinline__ float from(float f) { return f; } };
+template <> struct StoreAs<double> { static __device__ __forceinline__ double from(double f) { return f; } };
+template <> struct StoreAs<float2> { static __device__ __forceinline__ float2 from(float2 f) { return f; } };
+template <> struct StoreAs<double2> { static __device__ __forceinline__ double2 from(double2 f) { return f; } };
+
 template <class S> __device__ __forceinline__ S zero();
 template <> __device__ __forceinline__ double zero<double>() { return 0.0; }
 template <> __device__ __forceinline__ float zero<float>() { return 0.0f; }
@@ -644,6 +669,29 @@ __device__ __forceinline__ VRun<S, VW> vld(const S *base, unsigned i)
     return *reinterpret_cast<const VRun<S, VW> *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(S));
 }
 
+// a run of storage type T loaded into compute type S, and stored back
+template <int VW, class S, class T>
+__device__ __forceinline__ VRun<S, VW> vldc(const T *base, unsigned i)
+{
+    if constexpr (std::is_same<S, T>::value) {
+        return vld<VW>(base, i);
+    } else {
+        const VRun<T, VW> t = vld<VW>(base, i);
+        VRun<S, VW> r;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) r.v[e] = to_compute(t.v[e]);
+        return r;
+    }
+}
+template <int VW, class S, class T>
+__device__ __forceinline__ void vstc(T *base, unsigned i, const VRun<S, VW> &v)
+{
+    VRun<T, VW> t;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) t.v[e] = StoreAs<T>::from(v.v[e]);
+    *reinterpret_cast<VRun<T, VW> *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(T)) = t;
+}
+
 template <class S, int VW, int RV, int HP>
 struct VecLoads {
     VRun<S, VW> front[RV];
@@ -652,7 +700,7 @@ struct VecLoads {
     VRun<S, VW> halo[HP];
 };
 
-template <class S, int R, int RV, int NT, int HP, int W, int VW>
+template <class S, int R, int RV, int NT, int HP, int W, int VW, class T = S>
 __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
     constexpr int XO = (R + VW - 1) / VW * VW;   // LDS column of x = 0, vector aligned
@@ -675,9 +723,9 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const T *__restrict__ X = reinterpret_cast<const T *>(io.x) + (long)col * io.ldx;
+    const T *__restrict__ O = reinterpret_cast<const T *>(io.old) + (long)col * io.ldo;
+    T *__restrict__ Y = reinterpret_cast<T *>(io.out) + (long)col * io.ldout;
     const bool has_old = io.b != S(0);
 
     const int tid = threadIdx.x;
@@ -727,9 +775,9 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
     auto load_plane = [&](int zz, VRun<S, VW> *dst) {
         if (zz < 0) zz += nz;
         else if (zz >= nz) zz -= nz;
-        const S *src = X + zz * nxny;
+        const T *src = X + zz * nxny;
 #pragma unroll
-        for (int r = 0; r < RV; ++r) dst[r] = vld<VW>(src, qoff[r]);
+        for (int r = 0; r < RV; ++r) dst[r] = vldc<VW, S>(src, qoff[r]);
     };
     auto issue = [&](int zs, VecLoads<S, VW, RV, HP> &L) {
         zs = min(zs, z1 - 1);
@@ -737,11 +785,11 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
         const int zo = zs * nxny;
 #pragma unroll
         for (int r = 0; r < RV; ++r) {
-            L.old[r] = has_old ? vld<VW>(O + zo, qoff[r]) : vzero<S, VW>();
+            L.old[r] = has_old ? vldc<VW, S>(O + zo, qoff[r]) : vzero<S, VW>();
             L.v[r] = vld<VW>(p.veff + zo, qoff[r]);
         }
 #pragma unroll
-        for (int h = 0; h < HP; ++h) L.halo[h] = vld<VW>(X + zo, hsrc[h]);
+        for (int h = 0; h < HP; ++h) L.halo[h] = vldc<VW, S>(X + zo, hsrc[h]);
     };
     // a run, and one wrapped copy of it when it lies within XO of an x face
     // (x + nx to the right of the row, x - nx to the left; nx >= 2 XO, so a
@@ -776,7 +824,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
 #pragma unroll
         for (int h = 0; h < HP; ++h) put(L, hli[h], hx0[h], cur.halo[h]);
         __syncthreads();
-        S *Yz = Y + z * nxny;
+        T *Yz = Y + z * nxny;
 #pragma unroll
         for (int r = 0; r < RV; ++r) {
             S xr[NXV * VW];
@@ -807,7 +855,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
                 res.v[e] = io.a * hval;
                 if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
             }
-            if (qv[r]) *reinterpret_cast<VRun<S, VW> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+            if (qv[r]) vstc<VW>(Yz, qoff[r], res);
         }
 #pragma unroll
         for (int r = 0; r < RV; ++r)
@@ -826,12 +874,12 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
     if (z < z1) step(z, la, lb);
 }
 
-template <class S, int R, int RV, int NT, int HP, int W, int VW>
+template <class S, int R, int RV, int NT, int HP, int W, int VW, class T>
 __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vec_body<S, R, RV, NT, HP, W, VW>(p, step_io(p), w);
+    stencil_vec_body<S, R, RV, NT, HP, W, VW, T>(p, step_io(p), w);
 }
 
 // The same 16-byte-run step with the z-direction accumulated forward
@@ -843,7 +891,7 @@ __global__ void __launch_bounds__(NT) k_stencil_vec(StepArgs<S> p)
 // read those rows as their own newest plane, so they come from L2 instead of
 // being re-fetched R steps later.  Planes outside the chunk only feed the
 // z-terms and skip LDS.
-template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW>
+template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW, class T = S>
 __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const StepIO<S> io, const int w)
 {
     constexpr int XO = (R + VW - 1) / VW * VW;
@@ -866,9 +914,9 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     const int z1 = min(z0 + p.zch, nz);
     const int nxny = nx * ny;
 
-    const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
-    const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
-    S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
+    const T *__restrict__ X = reinterpret_cast<const T *>(io.x) + (long)col * io.ldx;
+    const T *__restrict__ O = reinterpret_cast<const T *>(io.old) + (long)col * io.ldo;
+    T *__restrict__ Y = reinterpret_cast<T *>(io.out) + (long)col * io.ldout;
     const bool has_old = io.b != S(0);
 
     const int tid = threadIdx.x;
@@ -927,19 +975,19 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
         int zz = phys(f);
         if (zz < 0) zz += nz;
         else if (zz >= nz) zz -= nz;
-        const S *Xz = X + zz * nxny;
+        const T *Xz = X + zz * nxny;
 #pragma unroll
-        for (int r = 0; r < RV; ++r) L.front[r] = vld<VW>(Xz, qoff[r]);
+        for (int r = 0; r < RV; ++r) L.front[r] = vldc<VW, S>(Xz, qoff[r]);
         if (f >= z0 && f < z1) {
 #pragma unroll
-            for (int h = 0; h < HP; ++h) L.halo[h] = vld<VW>(Xz, hsrc[h]);
+            for (int h = 0; h < HP; ++h) L.halo[h] = vldc<VW, S>(Xz, hsrc[h]);
 #pragma unroll
             for (int r = 0; r < RV; ++r) L.v[r] = vld<VW>(p.veff + zz * nxny, qoff[r]);
         }
         if (has_old && f - R >= z0) {
             const int zo = phys(f - R) * nxny;
 #pragma unroll
-            for (int r = 0; r < RV; ++r) L.old[r] = vld<VW>(O + zo, qoff[r]);
+            for (int r = 0; r < RV; ++r) L.old[r] = vldc<VW, S>(O + zo, qoff[r]);
         }
     };
     auto put = [&](S *L, int li, int x0, const VRun<S, VW> &v) {   // as in stencil_vec_body
@@ -1008,7 +1056,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
             }
         if (f - R >= z0) {
             const int zp = phys(f - R);
-            S *Yz = Y + zp * nxny;
+            T *Yz = Y + zp * nxny;
 #pragma unroll
             for (int r = 0; r < RV; ++r) {
                 VRun<S, VW> res;
@@ -1017,7 +1065,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
                     res.v[e] = (S(-0.5) * io.a) * acc[r][0].v[e];
                     if (has_old) res.v[e] = fm(io.b, cur.old[r].v[e], res.v[e]);
                 }
-                if (qv[r]) *reinterpret_cast<VRun<S, VW> *>(reinterpret_cast<char *>(Yz) + qoff[r] * (unsigned)sizeof(S)) = res;
+                if (qv[r]) vstc<VW>(Yz, qoff[r], res);
             }
         }
 #pragma unroll
@@ -1053,12 +1101,12 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     }
 }
 
-template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW>
+template <class S, int R, int RV, int NT, int HP, int W, int PD, int VW, class T>
 __global__ void __launch_bounds__(NT) k_stencil_vacc(StepArgs<S> p)
 {
     const int w = xcd_work_item();
     if (w >= p.nwork) return;
-    stencil_vacc_body<S, R, RV, NT, HP, W, PD, VW>(p, step_io(p), w);
+    stencil_vacc_body<S, R, RV, NT, HP, W, PD, VW, T>(p, step_io(p), w);
 }
 
 // --------------------------------------------------------- nonlocal gather --
@@ -1076,8 +1124,8 @@ struct NlArgs {
 // SPLIT = false: one wave per column (four columns per workgroup, index c
 // is the column group); SPLIT = true: the four waves share one column (c)
 // and split the points, for atoms with many sphere points.
-template <class S, int NPM, bool SPLIT>
-__device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ X,
+template <class S, int NPM, bool SPLIT, class T = S>
+__device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &p, const T *__restrict__ X,
                                                long ldx, S *__restrict__ beta, int ncol, int at, int c)
 {
     using A = typename Tr<S>::A;
@@ -1087,13 +1135,13 @@ __device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &
     const int n = SPLIT ? c : c * 4 + wave;
     if (n >= ncol) return;       // uniform over the workgroup when SPLIT
     const int pb = p.atom_pt_off[at], pe = p.atom_pt_off[at + 1];
-    const S *Xn = X + (long)n * ldx;
+    const T *Xn = X + (long)n * ldx;
     S acc[NPM];
 #pragma unroll
     for (int j = 0; j < NPM; ++j) acc[j] = zero<S>();
     constexpr int kStride = SPLIT ? 256 : 64;
     for (int e = pb + (SPLIT ? wave * 64 : 0) + lane; e < pe; e += kStride) {
-        S v = Xn[p.pt_gpos[e]];
+        S v = to_compute(Xn[p.pt_gpos[e]]);
         if constexpr (Tr<S>::C) {
             const int im = p.pt_img[e];
             v = cmul(v, p.phase[2 * im], p.phase[2 * im + 1]);
@@ -1135,11 +1183,11 @@ __device__ __forceinline__ void nl_gather_body(const NlArgs<typename Tr<S>::A> &
     }
 }
 
-template <class S, int NPM, bool SPLIT>
-__global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, const S *__restrict__ X,
+template <class S, int NPM, bool SPLIT, class T>
+__global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, const T *__restrict__ X,
                                                    long ldx, S *__restrict__ beta, int ncol)
 {
-    nl_gather_body<S, NPM, SPLIT>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
+    nl_gather_body<S, NPM, SPLIT, T>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
 }
 
 // atoms with more sphere points than this are gathered by four waves
@@ -1182,9 +1230,9 @@ __device__ __forceinline__ void nl_scatter_body(const NlArgs<typename Tr<S>::A> 
 // Many columns per wave: the point's entry metadata and projector row are
 // loaded once and reused for CPW columns, so per column only the atom's
 // beta row (shared by the wave's neighbouring points) and y itself move.
-template <class S, int NPM, int CPW>
+template <class S, int NPM, int CPW, class T>
 __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A> p, const S *__restrict__ beta,
-                                                       S *__restrict__ Y, long ldy, int ncol,
+                                                       T *__restrict__ Y, long ldy, int ncol,
                                                        typename Tr<S>::A a)
 {
     using A = typename Tr<S>::A;
@@ -1217,7 +1265,7 @@ __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A>
             for (int j = 0; j < NPM; ++j) s = fm(ch[j], bt[j], s);
             if constexpr (Tr<S>::C) s = cmul(s, pr, pi);
             const long gi = (long)c * ldy + g;
-            Y[gi] = fm(a, s, Y[gi]);
+            Y[gi] = StoreAs<T>::from(fm(a, s, to_compute(Y[gi])));
         }
         return;
     }
@@ -1238,7 +1286,7 @@ __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A>
             acc = add(acc, s);
         }
         const long gi = (long)c * ldy + g;
-        Y[gi] = fm(a, acc, Y[gi]);
+        Y[gi] = StoreAs<T>::from(fm(a, acc, to_compute(Y[gi])));
     }
 }
 
@@ -1459,6 +1507,25 @@ __global__ void k_stream3(const double2 *__restrict__ x, const double2 *__restri
     }
 }
 
+// Rounding to and widening from bfloat16 storage (real blocks; FP64 input is
+// rounded through float).
+template <class T>
+__global__ void k_to_bf16_cols(const T *src, long lds, bf16 *dst, long ldd, long n)
+{
+    const T *s = src + (long)blockIdx.y * lds;
+    bf16 *d = dst + (long)blockIdx.y * ldd;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = StoreAs<bf16>::from((float)s[i]);
+}
+template <class D>
+__global__ void k_from_bf16_cols(const bf16 *src, long lds, D *dst, long ldd, long n)
+{
+    const bf16 *s = src + (long)blockIdx.y * lds;
+    D *d = dst + (long)blockIdx.y * ldd;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = (D)to_compute(s[i]);
+}
+
 // ---------------------------------------------------------- dispatch helpers --
 template <class F>
 void dispatch(int e, F &&f)
@@ -1467,7 +1534,8 @@ void dispatch(int e, F &&f)
         case F64: f(double{}); break;
         case C128: f(double2{}); break;
         case F32: f(float{}); break;
-        default: f(float2{}); break;
+        case C64: f(float2{}); break;
+        default: fail(1, "element type not supported by this operation");
     }
 }
 
@@ -1592,7 +1660,7 @@ int vec_width(int nx)
     return 0;
 }
 
-template <class S, int VW, int RV, int NT, int MODE>
+template <class S, int VW, int RV, int NT, int MODE, class T = S>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                      int nzch, bool align, hipStream_t st)
@@ -1600,10 +1668,10 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     constexpr int XO = (kMaxR + VW - 1) / VW * VW;
     constexpr int HS = NT >= 512 ? 1 : 2;      // halo runs per thread
     constexpr int HB = NT >= 512 ? 2 : 3;
-    constexpr uintptr_t AM = sizeof(S) * VW - 1;
+    constexpr uintptr_t AM = sizeof(T) * VW - 1, AV = sizeof(S) * VW - 1;
     auto aligned = [](const void *ptr) { return (reinterpret_cast<uintptr_t>(ptr) & AM) == 0; };
     if (g.nx % VW || (ldx % VW) || (old && ldo % VW) || (ldout % VW) || !aligned(x) || (old && !aligned(old)) ||
-        !aligned(out) || !aligned(veff))
+        !aligned(out) || (reinterpret_cast<uintptr_t>(veff) & AV))
         return false;
     const int runs = g.nx / VW;
     const int lpr = align ? (runs + 15) / 16 * 16 : runs;   // lanes per band row
@@ -1634,19 +1702,19 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), shm, st, p); };
     if constexpr (MODE > 0) {
         if (pitch == 64) {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64, MODE, VW>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64, MODE, VW>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 64, MODE, VW, T>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 64, MODE, VW, T>);
         } else {
-            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128, MODE, VW>);
-            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128, MODE, VW>);
+            if (small_halo) go(k_stencil_vacc<S, kMaxR, RV, NT, HS, 128, MODE, VW, T>);
+            else go(k_stencil_vacc<S, kMaxR, RV, NT, HB, 128, MODE, VW, T>);
         }
     } else {
         if (pitch == 64) {
-            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 64, VW>);
-            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 64, VW>);
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 64, VW, T>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 64, VW, T>);
         } else {
-            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 128, VW>);
-            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 128, VW>);
+            if (small_halo) go(k_stencil_vec<S, kMaxR, RV, NT, HS, 128, VW, T>);
+            else go(k_stencil_vec<S, kMaxR, RV, NT, HB, 128, VW, T>);
         }
     }
     return true;
@@ -1664,6 +1732,11 @@ int stencil_points_per_thread(int e, const StepCfg &cfg)
 
 bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
 {
+    if (e == B16) {
+        // the run-vector kernels with float geometry
+        if (!cfg.vec) return false;
+        e = F32;
+    }
     bool ok = false;
     dispatch(e, [&](auto t) {
         using S = decltype(t);
@@ -1688,10 +1761,52 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
     return ok;
 }
 
+// The run-vector kernels of a real compute type S with storage T: the form
+// cfg.vec (1 ring, 2 forward accumulation, 3 the same two planes ahead), 16-
+// or 8-byte float runs by the row length; false when the shape or the
+// buffers' alignment does not allow it.
+template <class S, class T>
+bool launch_vec_modes(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo, void *out,
+                      long ldout, const void *veff, int ncol, double a, double b, double s, const StepCfg &cfg,
+                      hipStream_t st)
+{
+    const int vw = vec_width<S>(g.nx);
+    auto run = [&](auto rv, auto nt, auto acc) {
+        constexpr int V16 = int(16 / sizeof(S));
+        if constexpr (V16 == 4) {
+            if (vw == 2)
+                return launch_step_vec<S, 2, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value, T>(
+                    g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
+        }
+        return launch_step_vec<S, V16, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value, T>(
+            g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using N256 = std::integral_constant<int, 256>;
+    using N512 = std::integral_constant<int, 512>;
+    auto by_mode = [&](auto mode) {
+        using M = decltype(mode);
+        if (cfg.nt >= 512) return cfg.rp_div >= 2 ? run(I1{}, N512{}, M{}) : run(I2{}, N512{}, M{});
+        return cfg.rp_div >= 2 ? run(I1{}, N256{}, M{}) : run(I2{}, N256{}, M{});
+    };
+    // base RV = 2 runs per thread, RV = 1 for rp_div 2
+    if (cfg.vec == 3) return by_mode(std::integral_constant<int, 2>{});
+    if (cfg.vec == 2) return by_mode(std::integral_constant<int, 1>{});
+    return by_mode(std::integral_constant<int, 0>{});
+}
+
 void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                   void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
                   const StepCfg &cfg, hipStream_t st)
 {
+    if (e == B16) {
+        // bfloat16 storage, float arithmetic: run-vector kernels only
+        if (!g.orth || !cfg.vec ||
+            !launch_vec_modes<float, bf16>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st))
+            fail(8, "the bfloat16 filter needs an orthogonal cell, even row length and 4-byte aligned columns");
+        return;
+    }
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         constexpr int B = base_rp<S>();
@@ -1711,35 +1826,8 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
             return;
         }
         if constexpr (!Tr<S>::C) {
-            // real types: 16-byte runs; base RV = 2 runs, RV = 1 for rp_div 2
-            if (cfg.vec) {
-                bool ok;
-                const int vw = vec_width<S>(g.nx);
-                auto run = [&](auto rv, auto nt, auto acc) {
-                    constexpr int V16 = int(16 / sizeof(S));
-                    if constexpr (V16 == 4) {
-                        if (vw == 2)
-                            return launch_step_vec<S, 2, decltype(rv)::value, decltype(nt)::value,
-                                                   decltype(acc)::value>(g, x, ldx, old, ldo, out, ldout, veff, ncol,
-                                                                         a, b, s, cfg.nzch, cfg.align != 0, st);
-                    }
-                    return launch_step_vec<S, V16, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value>(
-                        g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
-                };
-                using I1 = std::integral_constant<int, 1>;
-                using I2 = std::integral_constant<int, 2>;
-                using N256 = std::integral_constant<int, 256>;
-                using N512 = std::integral_constant<int, 512>;
-                auto by_mode = [&](auto mode) {
-                    using M = decltype(mode);
-                    if (cfg.nt >= 512) return cfg.rp_div >= 2 ? run(I1{}, N512{}, M{}) : run(I2{}, N512{}, M{});
-                    return cfg.rp_div >= 2 ? run(I1{}, N256{}, M{}) : run(I2{}, N256{}, M{});
-                };
-                if (cfg.vec == 3) ok = by_mode(std::integral_constant<int, 2>{});
-                else if (cfg.vec == 2) ok = by_mode(std::integral_constant<int, 1>{});
-                else ok = by_mode(std::integral_constant<int, 0>{});
-                if (ok) return;
-            }
+            if (cfg.vec && launch_vec_modes<S, S>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st))
+                return;
         }
         if (cfg.nt >= 512) {
             if (cfg.rp_div >= 2)
@@ -1755,7 +1843,7 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
     });
 }
 
-template <class S, int NPM>
+template <class S, int NPM, class T = S>
 static void gather_npm(const NlDev &nl, const void *x, long ldx, void *beta, int ncol, hipStream_t st)
 {
     using A = typename Tr<S>::A;
@@ -1763,16 +1851,16 @@ static void gather_npm(const NlDev &nl, const void *x, long ldx, void *beta, int
                 (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
     if (nl.max_atom_pts > kGatherSplitPts) {
         dim3 grid(nl.n_atom, ncol);
-        hipLaunchKernelGGL((k_nl_gather<S, NPM, true>), grid, dim3(256), 0, st, p, (const S *)x, ldx, (S *)beta,
-                           ncol);
+        hipLaunchKernelGGL((k_nl_gather<S, NPM, true, T>), grid, dim3(256), 0, st, p, (const T *)x, ldx,
+                           (S *)beta, ncol);
     } else {
         dim3 grid(nl.n_atom, (ncol + 3) / 4);
-        hipLaunchKernelGGL((k_nl_gather<S, NPM, false>), grid, dim3(256), 0, st, p, (const S *)x, ldx, (S *)beta,
-                           ncol);
+        hipLaunchKernelGGL((k_nl_gather<S, NPM, false, T>), grid, dim3(256), 0, st, p, (const T *)x, ldx,
+                           (S *)beta, ncol);
     }
 }
 
-template <class S, int NPM>
+template <class S, int NPM, class T = S>
 static void scatter_npm(const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
                         hipStream_t st)
 {
@@ -1781,13 +1869,20 @@ static void scatter_npm(const NlDev &nl, const void *beta, void *y, long ldy, in
                 (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
     constexpr int CPW = 4;
     dim3 grid((nl.n_cov + 63) / 64, (ncol + 4 * CPW - 1) / (4 * CPW));
-    hipLaunchKernelGGL((k_nl_scatter_mc<S, NPM, CPW>), grid, dim3(256), 0, st, p, (const S *)beta, (S *)y, ldy,
+    hipLaunchKernelGGL((k_nl_scatter_mc<S, NPM, CPW, T>), grid, dim3(256), 0, st, p, (const S *)beta, (T *)y, ldy,
                        ncol, (A)a);
 }
 
 void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int ncol, hipStream_t st)
 {
     if (nl.n_atom == 0 || nl.n_pts == 0) return;
+    if (e == B16) {
+        // bfloat16 storage: x read as bf16, beta in float
+        if (nl.npm == 8) gather_npm<float, 8, bf16>(nl, x, ldx, beta, ncol, st);
+        else if (nl.npm == 16) gather_npm<float, 16, bf16>(nl, x, ldx, beta, ncol, st);
+        else gather_npm<float, 32, bf16>(nl, x, ldx, beta, ncol, st);
+        return;
+    }
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         if (nl.npm == 8) gather_npm<S, 8>(nl, x, ldx, beta, ncol, st);
@@ -1800,6 +1895,12 @@ void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int
                 hipStream_t st)
 {
     if (nl.n_cov == 0) return;
+    if (e == B16) {
+        if (nl.npm == 8) scatter_npm<float, 8, bf16>(nl, beta, y, ldy, ncol, a, st);
+        else if (nl.npm == 16) scatter_npm<float, 16, bf16>(nl, beta, y, ldy, ncol, a, st);
+        else scatter_npm<float, 32, bf16>(nl, beta, y, ldy, ncol, a, st);
+        return;
+    }
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         if (nl.npm == 8) scatter_npm<S, 8>(nl, beta, y, ldy, ncol, a, st);
@@ -2186,6 +2287,11 @@ void copy_cols(int e, const void *src, long lds, void *dst, long ldd, long nd, i
 {
     if (ncol <= 0 || nd <= 0) return;
     const int nb = (int)std::min<long>(1024, std::max<long>(1, (nd + 255) / 256));
+    if (e == B16) {
+        hipLaunchKernelGGL((k_copy_cols<unsigned short>), dim3(nb, ncol), dim3(256), 0, st,
+                           (const unsigned short *)src, lds, (unsigned short *)dst, ldd, nd);
+        return;
+    }
     dispatch(e, [&](auto t) {
         using S = decltype(t);
         hipLaunchKernelGGL((k_copy_cols<S>), dim3(nb, ncol), dim3(256), 0, st, (const S *)src, lds, (S *)dst,
@@ -2208,6 +2314,24 @@ void convert_cols(int e_src, const void *src, long lds, int e_dst, void *dst, lo
     const int m = elem_complex(e_src) ? 2 : 1;      // scalars per element
     const long n = nd * m;
     const int nb = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    if (e_src == B16 || e_dst == B16) {
+        if (e_src == B16 && e_dst == B16)
+            hipLaunchKernelGGL((k_copy_cols<unsigned short>), dim3(nb, ncol), dim3(256), 0, st,
+                               (const unsigned short *)src, lds, (unsigned short *)dst, ldd, nd);
+        else if (e_dst == B16 && e_src == F64)
+            hipLaunchKernelGGL((k_to_bf16_cols<double>), dim3(nb, ncol), dim3(256), 0, st, (const double *)src, lds,
+                               (bf16 *)dst, ldd, nd);
+        else if (e_dst == B16)
+            hipLaunchKernelGGL((k_to_bf16_cols<float>), dim3(nb, ncol), dim3(256), 0, st, (const float *)src, lds,
+                               (bf16 *)dst, ldd, nd);
+        else if (e_dst == F64)
+            hipLaunchKernelGGL((k_from_bf16_cols<double>), dim3(nb, ncol), dim3(256), 0, st, (const bf16 *)src, lds,
+                               (double *)dst, ldd, nd);
+        else
+            hipLaunchKernelGGL((k_from_bf16_cols<float>), dim3(nb, ncol), dim3(256), 0, st, (const bf16 *)src, lds,
+                               (float *)dst, ldd, nd);
+        return;
+    }
     const bool sd = !elem_single(e_src), dd = !elem_single(e_dst);
     if (sd && !dd)
         hipLaunchKernelGGL((k_convert_cols<float, double>), dim3(nb, ncol), dim3(256), 0, st, (const double *)src,

@@ -8,10 +8,14 @@ namespace lynx {
 namespace gpu {
 
 // Element types of wavefunction blocks; values match enum lynx_dtype.
-enum Elem { F64 = 0, C128 = 1, F32 = 2, C64 = 3 };
-inline size_t elem_size(int e) { return e == F64 ? 8 : e == C128 ? 16 : e == F32 ? 4 : 8; }
+// B16: real bfloat16 storage with float arithmetic (filter steps only;
+// its Veff, projector data and beta are float)
+enum Elem { F64 = 0, C128 = 1, F32 = 2, C64 = 3, B16 = 4 };
+inline size_t elem_size(int e) { return e == F64 ? 8 : e == C128 ? 16 : e == F32 ? 4 : e == B16 ? 2 : 8; }
 inline bool elem_complex(int e) { return e == C128 || e == C64; }
-inline bool elem_single(int e) { return e == F32 || e == C64; }
+inline bool elem_single(int e) { return e == F32 || e == C64 || e == B16; }
+// element type of beta and the other arithmetic-side buffers of e
+inline int elem_compute(int e) { return e == B16 ? F32 : e; }
 
 constexpr int kMaxR = 6;   // stencil radius handled by the fused kernels (FD order 12)
 

@@ -142,7 +142,8 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
         }
         const int passes = scf_iter == 0 ? p.rho_trigger : p.nchefsi;
         const bool single = p.fp64_switch_residual > 0 && (scf_iter == 0 || scf_residual > p.fp64_switch_residual);
-        d->last_precision = single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64;
+        d->last_precision = !single ? LYNX_FILTER_FP64
+                                    : (p.low_precision == LYNX_FILTER_BF16 ? LYNX_FILTER_BF16 : LYNX_FILTER_FP32);
         ok(lynx_set_filter_precision(d->ctx, d->last_precision), d, "lynx_set_filter_precision");
         const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
         for (int pass = 0; pass < passes; ++pass) {
