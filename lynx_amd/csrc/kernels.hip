@@ -1318,7 +1318,9 @@ struct BatchRec {
 };
 
 // KIND: 0 orthogonal (scalar body), 1 non-orthogonal, 2 orthogonal with
-// 16-byte runs and row-aligned lanes (real types, RP runs per thread)
+// 16-byte runs and row-aligned lanes (real types, RP runs per thread; the
+// forward-accumulation body with loads two planes ahead), 3 the same with
+// 8-byte runs of two floats
 template <class S, int R, int RP, int NT, int HP, int W, int KIND>
 __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
                                                       const int *__restrict__ item_unit, int nitems,
@@ -1334,8 +1336,9 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
                        coef[3 * u], coef[3 * u + 1], coef[3 * u + 2]};
     if constexpr (KIND == 1) {
         stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
-    } else if constexpr (KIND == 2) {
-        if constexpr (!Tr<S>::C) stencil_vec_body<S, R, RP, NT, HP, W, int(16 / sizeof(S))>(r.q.b, io, w - r.item0);
+    } else if constexpr (KIND == 2 || KIND == 3) {
+        if constexpr (!Tr<S>::C)
+            stencil_vacc_body<S, R, RP, NT, HP, W, 2, KIND == 3 ? 2 : int(16 / sizeof(S))>(r.q.b, io, w - r.item0);
     } else {
         stencil_step_body<S, R, RP, NT, HP, W>(r.q.b, io, w - r.item0);
     }
@@ -1914,7 +1917,7 @@ struct BatchPlan {
     int e = 0, n = 0;
     void *recs = nullptr;
     struct SGroup {
-        int kind = 0;             // 0 orthogonal, 1 non-orthogonal, 2 orthogonal 16-byte runs
+        int kind = 0;             // 0 orthogonal, 1 non-orthogonal, 2 / 3 orthogonal 16- / 8-byte runs
         int W = 64;
         int *items = nullptr;
         int nitems = 0;
@@ -1969,19 +1972,26 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
         bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
         if constexpr (!Tr<S>::C) {
-            constexpr int VW = int(16 / sizeof(S));
-            constexpr int XO = (kMaxR + VW - 1) / VW * VW;
-            auto al = [](const void *ptr, long ld) {
-                return (reinterpret_cast<uintptr_t>(ptr) & 15) == 0 && ld % VW == 0;
-            };
+            // kind 2: 16-byte runs; kind 3: 8-byte runs of two floats for
+            // rows whose length is not a multiple of four
             const BatchUnit &U = units[u];
-            const int lpr = (g.nx / VW + 15) / 16 * 16;
-            if (g.orth && g.nx % VW == 0 && g.nx + 2 * XO <= 128 && g.nx >= 2 * XO && lpr <= kBatchNT &&
-                al(U.buf[0], U.ld[0]) && al(U.buf[1], U.ld[1]) && al(U.buf[2], U.ld[2]) && al(U.veff, 0)) {
-                kind[u] = 2;
-                W = g.nx + 2 * XO <= 64 ? 64 : 128;
-                sh = 2 * kMaxR * lpr <= kBatchNT * 2;
-            }
+            auto try_vec = [&](int VW, int kd) {
+                const int XO = (kMaxR + VW - 1) / VW * VW;
+                const uintptr_t am = (uintptr_t)(VW * sizeof(S)) - 1;
+                auto al = [&](const void *ptr, long ld) {
+                    return (reinterpret_cast<uintptr_t>(ptr) & am) == 0 && ld % VW == 0;
+                };
+                const int lpr = (g.nx / VW + 15) / 16 * 16;
+                if (g.orth && g.nx % VW == 0 && g.nx + 2 * XO <= 128 && g.nx >= 2 * XO && lpr <= kBatchNT &&
+                    al(U.buf[0], U.ld[0]) && al(U.buf[1], U.ld[1]) && al(U.buf[2], U.ld[2]) && al(U.veff, 0)) {
+                    kind[u] = kd;
+                    W = g.nx + 2 * XO <= 64 ? 64 : 128;
+                    sh = 2 * kMaxR * lpr <= kBatchNT * 2;
+                    return true;
+                }
+                return false;
+            };
+            if (!try_vec(int(16 / sizeof(S)), 2) && sizeof(S) == 4) try_vec(2, 3);
         }
         int k = -1;
         for (size_t i = 0; i < P->sg.size(); ++i)
@@ -2015,9 +2025,9 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         NonorthArgs<A> &q = r.q;
         StepArgs<A> &p = q.b;
         p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = U.ncol;
-        if (kind[u] == 2) {
-            // one 16-byte run per thread, band rows at lane multiples of 16
-            constexpr int VW = int(16 / sizeof(S));
+        if (kind[u] == 2 || kind[u] == 3) {
+            // one run per thread, band rows at lane multiples of 16
+            const int VW = kind[u] == 3 ? 2 : int(16 / sizeof(S));
             p.lpr = (g.nx / VW + 15) / 16 * 16;
             const int rpb = kBatchNT / p.lpr;
             const int nb = (g.ny + rpb - 1) / rpb;
@@ -2103,14 +2113,18 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
     const int blocks = ((G.nitems + 7) / 8) * 8;
     const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
     const A *cf = static_cast<const A *>(coef);
-    if (G.kind == 2) {
+    if (G.kind == 2 || G.kind == 3) {
         if constexpr (!Tr<S>::C) {
-            // vector kind: halo slots counted in runs (2 or 3 per thread)
-            constexpr int VW = int(16 / sizeof(S));
+            // vector kinds: halo slots counted in runs (2 or 3 per thread)
             constexpr int HV = HP == 3 ? 2 : 3;
+            const int VW = G.kind == 3 ? 2 : int(16 / sizeof(S));
             const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT * VW);
-            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>), dim3(blocks), dim3(kBatchNT), shm,
-                               st, recs, G.items, G.nitems, cf, ix, iold, iout);
+            if (G.kind == 2)
+                hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>), dim3(blocks), dim3(kBatchNT),
+                                   shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+            else if constexpr (sizeof(S) == 4)
+                hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 3>), dim3(blocks), dim3(kBatchNT),
+                                   shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
         }
         return;
     }

@@ -55,10 +55,11 @@ def test_kpoint_batch_matches_per_kpoint_oracle(fcc_kbatch, dtype, tol):
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 2e-5)])
 def test_heterogeneous_batch_matches_oracle(dtype, tol):
     # orthogonal and non-orthogonal cells, different grids and band counts,
-    # one without projectors
+    # one without projectors; rows of 20 points (16-byte float runs) and of
+    # 26 (8-byte float runs)
     from lynx_amd import core
     systems = [si8(20), si2_fcc(18), System((20, 22, 24), A_SI, displace(DIAMOND, A_SI, 0.05, seed=4)),
-               si8(16, with_nonlocal=False)]
+               si8(16, with_nonlocal=False), System((26, 20, 22), A_SI, displace(DIAMOND, A_SI, 0.05, seed=5))]
     Hs = []
     for s in systems:
         H = s.hamiltonian()
