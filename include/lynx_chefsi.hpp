@@ -144,12 +144,19 @@ public:
     double lambda_cutoff() const { return lambda_cutoff_; }
     void set_lambda_cutoff(double v) { lambda_cutoff_ = v; }
 
-    // Filter precision of solve/solve_kpt (no reference counterpart): FP32
-    // filter, FP64 Cholesky-QR, projection, eigensolve and rotation.
+    // Precision of solve/solve_kpt (no reference counterpart): FP32 filter,
+    // Gram products and rotations with FP64 reductions, Cholesky and
+    // eigensolve (see lynx_set_filter_precision).
     void set_filter_fp32(bool single)
     {
         if (!H_) throw std::runtime_error("EigenSolver::setup has not been called");
         check(lynx_set_filter_precision(H_->ctx(), single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64), H_->ctx());
+    }
+    // LYNX_FILTER_FP64, LYNX_FILTER_FP32 or LYNX_FILTER_BF16
+    void set_precision(int precision)
+    {
+        if (!H_) throw std::runtime_error("EigenSolver::setup has not been called");
+        check(lynx_set_filter_precision(H_->ctx(), precision), H_->ctx());
     }
 
 private:
