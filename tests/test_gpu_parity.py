@@ -340,3 +340,26 @@ def test_multispecies_large_spheres_apply():
     assert rel(gotc, refc) < 1e-12
     g32 = H._apply(Xc.T.copy(), s.veff, 0.0, np.complex64).reshape(3, -1).T
     assert rel(g32, refc) < 2e-6
+
+
+def test_wide_blocks_nonlocal():
+    # 36-40 columns (ten column groups of the gather and scatter), real and
+    # complex, FP64 and single precision, 8 projectors (Si8) and 32 with
+    # spheres of thousands of points (BaTiO3 stand-in)
+    from systems import batio3
+    for s, ncol in ((si8(16), 40), (batio3(0, n=28), 36)):
+        k = (0.25 * 2 * np.pi / s.a, -0.125 * 2 * np.pi / s.a, 0.375 * 2 * np.pi / s.a)
+        op, H = s.oracle_op(), s.hamiltonian()
+        X = s.random_block(ncol, 43)
+        ref = op.apply(X, s.veff, c=0.05)
+        got = H.apply(H.halo, X.T.copy(), s.veff, 0.05).reshape(ncol, -1).T
+        assert rel(got, ref) < 1e-12
+        g32 = H._apply(X.T.copy(), s.veff, 0.05, np.float32).reshape(ncol, -1).T
+        assert rel(g32, ref) < 2e-6
+        op.set_kpoint(k)
+        Xc = s.random_block(ncol, 44, complex_=True)
+        refc = op.apply(Xc, s.veff)
+        gotc = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k).reshape(ncol, -1).T
+        assert rel(gotc, refc) < 1e-12
+        g64 = H._apply(Xc.T.copy(), s.veff, 0.0, np.complex64).reshape(ncol, -1).T
+        assert rel(g64, refc) < 2e-6
