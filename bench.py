@@ -273,16 +273,18 @@ def run_hetero_batch(steps, warmup):
         bf.close()
         for d in xs + ys:
             d.free()
-    # one frame per call (FP64), the unbatched baseline
+    # one frame per call (FP64), the unbatched baseline; the second round is
+    # timed (the first pays each context's launch-shape tuning)
     xs = [DeviceArray.from_numpy(x) for x in X]
     ys = [DeviceArray(x.size * 8) for x in X]
-    synchronize()
-    t0 = time.perf_counter()
-    for i, (_, s, nb) in enumerate(frames):
-        L.check(lib.lynx_cheb_filter(Hs[i].ctx, L.F64, xs[i].value, s.nd, ys[i].value, s.nd, nb, DEGREE, lc[i], lo[i],
-                                     hi[i], L.MEM_DEVICE), Hs[i].ctx)
-    synchronize()
-    unbatched = 1e3 * (time.perf_counter() - t0)
+    for rep in range(2):
+        synchronize()
+        t0 = time.perf_counter()
+        for i, (_, s, nb) in enumerate(frames):
+            L.check(lib.lynx_cheb_filter(Hs[i].ctx, L.F64, xs[i].value, s.nd, ys[i].value, s.nd, nb, DEGREE, lc[i],
+                                         lo[i], hi[i], L.MEM_DEVICE), Hs[i].ctx)
+        synchronize()
+        unbatched = 1e3 * (time.perf_counter() - t0)
     err = {}
     for i, (cls, _, _) in enumerate(frames):
         a, b = outs[L.F32][i].astype(np.float64), outs[L.F64][i]
