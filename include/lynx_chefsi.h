@@ -201,6 +201,12 @@ lynx_status lynx_psp8_info(const char *path, double *zion, int *lmax, int *lloc,
 lynx_status lynx_last_filter_timing(const lynx_ctx *ctx, double times[4], int launches[4]);
 /* Enable (1) or disable (0) HIP-event timing of filter calls. */
 lynx_status lynx_set_timing(lynx_ctx *ctx, int enable);
+/* Filter calls as HIP graphs (1, the default) or direct launches (0).  With
+ * graphs, the second call with the same buffers, column count and set-up
+ * captures the degree's launches; later calls upload their step scalars to
+ * a device table and replay the graph (no per-launch host work).  No
+ * reference counterpart. */
+lynx_status lynx_set_filter_graphs(lynx_ctx *ctx, int enable);
 /* The practical HBM ceiling for the FP64 stencil step's access pattern on
  * this device: a streaming kernel that reads x and Y_(k-2) and writes y
  * (nd x ncol doubles each, 16-byte accesses) and re-reads one nd-sized Veff

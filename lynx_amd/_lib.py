@@ -53,6 +53,7 @@ SIGNATURES = {
     "lynx_kpoints": (_I, [_I, _I, _I, _DP, _DP, _I, _DP, _DP, _DP, _I, _IP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
     "lynx_set_timing": (_I, [_P, _I]),
+    "lynx_set_filter_graphs": (_I, [_P, _I]),
     "lynx_stream_ceiling": (_I, [_P, _L, _I, _DP]),
     "lynx_mixed_fallbacks": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)]),
     "lynx_set_filter_precision": (_I, [_P, _I]),

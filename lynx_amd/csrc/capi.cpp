@@ -110,6 +110,23 @@ struct lynx_ctx {
     long fp64_overlap_fallbacks = 0;   // FP32 passes redone in FP64
     long cholqr2_passes = 0;           // FP32 passes with a second Cholesky-QR
 
+    // filter graphs: the step scalars come from a device table uploaded per
+    // call, so one captured graph serves every call with the same buffers,
+    // column count and set-up (key); epoch is bumped by each set-up call
+    // whose values the launches capture
+    struct FilterGraph {
+        std::vector<long> key;
+        hipGraphExec_t exec = nullptr;
+        std::vector<std::pair<int, int>> marks;   // timing events recorded by the graph's nodes
+    };
+    std::vector<FilterGraph> graphs;
+    std::vector<std::vector<long>> graph_seen;
+    DevBuf fcoef;                      // kMaxGraphDegree steps of (a, b, shift)
+    void *fcoef_host = nullptr;        // pinned staging of the table
+    hipEvent_t fcoef_copied = nullptr; // the last upload from the staging buffer
+    long epoch = 0;
+    bool use_graphs = true;
+
     bool timing = false;
     double t_last[4] = {0, 0, 0, 0};
     int n_last[4] = {0, 0, 0, 0};
@@ -123,6 +140,11 @@ struct lynx_ctx {
                           &d_cov_gpos, &d_ent_pt, &d_chi64, &d_chi32, &d_gam64, &d_gam32, &d_ph64, &d_ph32,
                           &beta, &wA, &wB, &wC, &lz[0], &lz[1], &lz[2], &s1, &s2, &s3, &gwork, &dvals, &evals,
                           &info, &partials};
+        for (auto &g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        fcoef.release();
+        if (fcoef_copied) (void)hipEventDestroy(fcoef_copied);
+        if (fcoef_host) (void)hipHostFree(fcoef_host);
         for (DevBuf *b : bufs) b->release();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (blas) rocblas_destroy_handle(blas);
@@ -480,7 +502,7 @@ bool bf16_filter_fits(const lynx_ctx *c)
 // ------------------------------------------------------------- operator --
 // y = a * (H - shift) x + b * old, H = -1/2 lap + Veff + V_nl.
 void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long ldo, void *y, long ldy,
-              int ncol, double a, double b, double shift)
+              int ncol, double a, double b, double shift, const void *coef = nullptr)
 {
     const gpu::StepCfg cfg = step_config(c, e, x, ldx, y, ldy, ncol);
     const bool nl = c->has_nl;
@@ -493,17 +515,27 @@ void apply_op(lynx_ctx *c, int e, const void *x, long ldx, const void *old, long
     {
         Marker m(c, 1);
         gpu::stencil_step(e, geom(c), x, ldx, old, ldo, y, ldy, veff_for(c, e), ncol, a, b, shift, cfg,
-                          c->stream);
+                          c->stream, coef);
     }
     if (nl) {
         Marker m(c, 3);
-        gpu::nl_scatter(e, nd, c->beta.p, y, ldy, ncol, a, c->stream);
+        gpu::nl_scatter(e, nd, c->beta.p, y, ldy, ncol, a, c->stream, coef);
     }
     HIP_OK(hipGetLastError());
 }
 
+// filters of higher degree run without a graph
+constexpr int kMaxGraphDegree = 256;
+
 // EigenSolver.cpp:558-619 as a sequence of fused steps.  x is read only; the
 // final block lands in y; w is a work block of the same shape.
+//
+// The first call for a given key (buffers, column count, element type,
+// launch shape and set-up epoch) runs the launches directly (and tunes the
+// launch shape); the second captures them into a HIP graph whose kernels
+// read each step's (a, b, shift) from a device table; later calls upload the
+// table and replay the graph.  With timing on, the graph also holds the
+// event records around each kernel, which every replay records again.
 void filter(lynx_ctx *c, int e, const void *x, long ldx, void *y, long ldy, void *w, long ldw, int ncol,
             int degree, double lambda_c, double eig_min, double eig_max)
 {
@@ -511,28 +543,122 @@ void filter(lynx_ctx *c, int e, const void *x, long ldx, void *y, long ldy, void
     const double ehalf = (eig_max - lambda_c) / 2.0;
     const double cen = (eig_max + lambda_c) / 2.0;
     if (!(ehalf > 0)) fail(LYNX_E_ARG, "eig_max must exceed lambda_cutoff");
+    // (a, b, shift) of every step
+    std::vector<double> sc(3 * degree);
     const double sigma1 = ehalf / (eig_min - cen);
     double sigma = sigma1;
+    sc[0] = sigma / ehalf; sc[1] = 0.0; sc[2] = -cen;
+    for (int k = 2; k <= degree; ++k) {
+        const double sn = 1.0 / (2.0 / sigma1 - sigma);
+        sc[3 * (k - 1)] = 2.0 * sn / ehalf;
+        sc[3 * (k - 1) + 1] = -sigma * sn;
+        sc[3 * (k - 1) + 2] = -cen;
+        sigma = sn;
+    }
     // odd steps write D1, even steps D2; the last step must land in y
     void *D1 = (degree % 2 == 1) ? y : w;
     void *D2 = (degree % 2 == 1) ? w : y;
     const long ld1 = (degree % 2 == 1) ? ldy : ldw;
     const long ld2 = (degree % 2 == 1) ? ldw : ldy;
-    apply_op(c, e, x, ldx, nullptr, 0, D1, ld1, ncol, sigma / ehalf, 0.0, -cen);
-    const void *prev = D1;   // Y_{k-1}
-    long ldp = ld1;
-    const void *pprev = x;   // Y_{k-2}
-    long ldpp = ldx;
-    for (int k = 2; k <= degree; ++k) {
-        const double sn = 1.0 / (2.0 / sigma1 - sigma);
-        const double gam = 2.0 * sn / ehalf;
-        void *dst = (k % 2 == 1) ? D1 : D2;
-        const long ldd = (k % 2 == 1) ? ld1 : ld2;
-        apply_op(c, e, prev, ldp, pprev, ldpp, dst, ldd, ncol, gam, -sigma * sn, -cen);
-        pprev = prev; ldpp = ldp;
-        prev = dst; ldp = ldd;
-        sigma = sn;
+    // table element: the real scalar type of the kernels' (a, b, shift)
+    // (float for single precision and bfloat16 storage, complex included)
+    const size_t ce = gpu::elem_single(e) ? sizeof(float) : sizeof(double);
+    auto run = [&](bool table) {
+        auto cf = [&](int k) -> const void * {
+            return table ? static_cast<const char *>(c->fcoef.p) + (size_t)3 * (k - 1) * ce : nullptr;
+        };
+        apply_op(c, e, x, ldx, nullptr, 0, D1, ld1, ncol, sc[0], sc[1], sc[2], cf(1));
+        const void *prev = D1;   // Y_{k-1}
+        long ldp = ld1;
+        const void *pprev = x;   // Y_{k-2}
+        long ldpp = ldx;
+        for (int k = 2; k <= degree; ++k) {
+            void *dst = (k % 2 == 1) ? D1 : D2;
+            const long ldd = (k % 2 == 1) ? ld1 : ld2;
+            apply_op(c, e, prev, ldp, pprev, ldpp, dst, ldd, ncol, sc[3 * (k - 1)], sc[3 * (k - 1) + 1],
+                     sc[3 * (k - 1) + 2], cf(k));
+            pprev = prev; ldpp = ldp;
+            prev = dst; ldp = ldd;
+        }
+    };
+    if (!c->use_graphs || degree > kMaxGraphDegree) {
+        run(false);
+        return;
     }
+    const gpu::StepCfg cfg = step_config(c, e, x, ldx, D1, ld1, ncol);
+    if (c->has_nl) {
+        // beta sized before the key: the graph's launches hold its address
+        const gpu::NlDev &nd = nl_for(c, e);
+        c->beta.get((size_t)ncol * nd.n_atom * nd.npm * gpu::elem_size(gpu::elem_compute(e)));
+    }
+    // everything a captured launch holds by value: buffers, shapes, the
+    // set-up epoch (grid, atoms, projectors, forced launch shapes), Veff and
+    // beta addresses, the k-point (Bloch phases of the stencil) and the shape
+    long kb[3];
+    std::memcpy(kb, c->kpt, sizeof(kb));
+    std::vector<long> key = {e, (long)(uintptr_t)x, ldx, (long)(uintptr_t)y, ldy, (long)(uintptr_t)w, ldw, ncol,
+                             degree, c->epoch, (long)(uintptr_t)veff_for(c, e), (long)(uintptr_t)c->beta.p,
+                             kb[0], kb[1], kb[2], cfg.rp_div, cfg.nzch, cfg.nt, cfg.vec, cfg.align,
+                             c->timing ? 1L : 0L};
+    lynx_ctx::FilterGraph *g = nullptr;
+    for (auto &fg : c->graphs)
+        if (fg.key == key) g = &fg;
+    if (!g && std::find(c->graph_seen.begin(), c->graph_seen.end(), key) == c->graph_seen.end()) {
+        // first call with this key: direct launches
+        if (c->graph_seen.size() >= 16) c->graph_seen.erase(c->graph_seen.begin());
+        c->graph_seen.push_back(key);
+        run(false);
+        return;
+    }
+    if (!c->fcoef.p) {
+        c->fcoef.get((size_t)3 * kMaxGraphDegree * sizeof(double));
+        HIP_OK(hipHostMalloc(&c->fcoef_host, (size_t)3 * kMaxGraphDegree * sizeof(double)));
+        HIP_OK(hipEventCreateWithFlags(&c->fcoef_copied, hipEventDisableTiming));
+    }
+    if (!g) {
+        // second call: capture (graphs of older set-up epochs are dropped)
+        for (auto &old : c->graphs)
+            if (old.key[9] != c->epoch) {
+                (void)hipGraphExecDestroy(old.exec);
+                old.exec = nullptr;
+            }
+        c->graphs.erase(std::remove_if(c->graphs.begin(), c->graphs.end(),
+                                       [](const lynx_ctx::FilterGraph &fg) { return !fg.exec; }),
+                        c->graphs.end());
+        if (c->graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+        }
+        // timing events are created before the capture (three kernels and
+        // six events per step)
+        if (c->timing) ev_index(c, c->ev_marks.size() * 2 + (size_t)6 * degree + 2);
+        hipGraph_t graph = nullptr;
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run(true);
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIP_OK(hipStreamEndCapture(c->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t r = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_OK(r);
+        c->graphs.push_back({key, exec, c->ev_marks});
+        g = &c->graphs.back();
+    }
+    if (c->timing) c->ev_marks = g->marks;
+    // the table in the arithmetic type, through the pinned staging buffer
+    // (free again once the previous call's upload has run)
+    HIP_OK(hipEventSynchronize(c->fcoef_copied));
+    if (ce == sizeof(double)) std::memcpy(c->fcoef_host, sc.data(), sc.size() * ce);
+    else
+        for (size_t i = 0; i < sc.size(); ++i) static_cast<float *>(c->fcoef_host)[i] = (float)sc[i];
+    HIP_OK(hipMemcpyAsync(c->fcoef.p, c->fcoef_host, sc.size() * ce, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipEventRecord(c->fcoef_copied, c->stream));
+    HIP_OK(hipGraphLaunch(g->exec, c->stream));
 }
 
 long padded_ld(long nd) { return (nd + 63) / 64 * 64; }
@@ -829,6 +955,7 @@ lynx_status lynx_set_grid(lynx_ctx *ctx, int nx, int ny, int nz, const double la
 {
     if (!ctx || !latvec) return LYNX_E_ARG;
     return guarded(ctx, [&] {
+        ++ctx->epoch;   // captured filter graphs hold the old grid / projector data
         if (nx <= 0 || ny <= 0 || nz <= 0) fail(LYNX_E_ARG, "grid dimensions must be positive");
         if ((long)nx * ny * nz > (1L << 31) - 1) fail(LYNX_E_ARG, "grid too large");
         if (cell_type != LYNX_CELL_ORTHOGONAL && cell_type != LYNX_CELL_NONORTHOGONAL)
@@ -876,6 +1003,7 @@ lynx_status lynx_set_atoms(lynx_ctx *ctx, int ntypes, const char *const *paths, 
 {
     if (!ctx) return LYNX_E_ARG;
     return guarded(ctx, [&] {
+        ++ctx->epoch;   // captured filter graphs hold the old grid / projector data
         need_grid(ctx);
         if (ntypes <= 0 || natom < 0 || !paths || (natom > 0 && (!atom_type || !coords)))
             fail(LYNX_E_ARG, "bad atom arguments");
@@ -929,6 +1057,7 @@ lynx_status lynx_set_nonlocal(lynx_ctx *ctx, int n_img, const int *img_atom, con
 {
     if (!ctx) return LYNX_E_ARG;
     return guarded(ctx, [&] {
+        ++ctx->epoch;   // captured filter graphs hold the old grid / projector data
         need_grid(ctx);
         if (n_img < 0 || n_atom < 0) fail(LYNX_E_ARG, "negative sizes");
         NonlocalHost h;
@@ -1489,6 +1618,13 @@ lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
 {
     if (!ctx) return LYNX_E_ARG;
     ctx->timing = enable != 0;
+    return LYNX_OK;
+}
+
+lynx_status lynx_set_filter_graphs(lynx_ctx *ctx, int enable)
+{
+    if (!ctx) return LYNX_E_ARG;
+    ctx->use_graphs = enable != 0;
     return LYNX_OK;
 }
 

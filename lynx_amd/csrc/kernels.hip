@@ -152,6 +152,10 @@ struct StepArgs {
     A c[3][kMaxR + 1];
     A a, b, s;
     A ph[3][2][2];
+    // when set, (a, b, s) are read from this device triple instead (the
+    // filter's steps in a replayed HIP graph take their scalars from a
+    // table uploaded per call)
+    const A *coef = nullptr;
 };
 
 // Per-launch operands of a step (buffers and recurrence scalars), kept
@@ -167,8 +171,9 @@ struct StepIO {
 };
 
 template <class A>
-__host__ __device__ inline StepIO<A> step_io(const StepArgs<A> &p)
+__device__ inline StepIO<A> step_io(const StepArgs<A> &p)
 {
+    if (p.coef) return StepIO<A>{p.x, p.old, p.out, p.ldx, p.ldo, p.ldout, p.coef[0], p.coef[1], p.coef[2]};
     return StepIO<A>{p.x, p.old, p.out, p.ldx, p.ldo, p.ldout, p.a, p.b, p.s};
 }
 
@@ -234,7 +239,7 @@ __device__ __forceinline__ void stencil_step_body(const StepArgs<typename Tr<S>:
     const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
     const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
     S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
-    const bool has_old = io.b != A(0);
+    const bool has_old = io.old != nullptr && io.b != A(0);   // no Y_(k-2) stream without a buffer
 
     const int tid = threadIdx.x;
     const int npts = rows * nx;
@@ -450,7 +455,7 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
     const S *__restrict__ X = reinterpret_cast<const S *>(io.x) + (long)col * io.ldx;
     const S *__restrict__ O = reinterpret_cast<const S *>(io.old) + (long)col * io.ldo;
     S *__restrict__ Y = reinterpret_cast<S *>(io.out) + (long)col * io.ldout;
-    const bool has_old = io.b != A(0);
+    const bool has_old = io.old != nullptr && io.b != A(0);   // no Y_(k-2) stream without a buffer
 
     const int tid = threadIdx.x;
     const int npts = rows * nx;
@@ -726,7 +731,7 @@ __device__ __forceinline__ void stencil_vec_body(const StepArgs<S> &p, const Ste
     const T *__restrict__ X = reinterpret_cast<const T *>(io.x) + (long)col * io.ldx;
     const T *__restrict__ O = reinterpret_cast<const T *>(io.old) + (long)col * io.ldo;
     T *__restrict__ Y = reinterpret_cast<T *>(io.out) + (long)col * io.ldout;
-    const bool has_old = io.b != S(0);
+    const bool has_old = io.old != nullptr && io.b != S(0);   // no Y_(k-2) stream without a buffer
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
@@ -917,7 +922,7 @@ __device__ __forceinline__ void stencil_vacc_body(const StepArgs<S> &p, const St
     const T *__restrict__ X = reinterpret_cast<const T *>(io.x) + (long)col * io.ldx;
     const T *__restrict__ O = reinterpret_cast<const T *>(io.old) + (long)col * io.ldo;
     T *__restrict__ Y = reinterpret_cast<T *>(io.out) + (long)col * io.ldout;
-    const bool has_old = io.b != S(0);
+    const bool has_old = io.old != nullptr && io.b != S(0);   // no Y_(k-2) stream without a buffer
 
     const int tid = threadIdx.x;
     const int dummy = base + tid * VW;
@@ -1233,9 +1238,10 @@ __device__ __forceinline__ void nl_scatter_body(const NlArgs<typename Tr<S>::A> 
 template <class S, int NPM, int CPW, class T>
 __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A> p, const S *__restrict__ beta,
                                                        T *__restrict__ Y, long ldy, int ncol,
-                                                       typename Tr<S>::A a)
+                                                       typename Tr<S>::A a, const typename Tr<S>::A *acoef)
 {
     using A = typename Tr<S>::A;
+    if (acoef) a = acoef[0];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int u = blockIdx.x * 64 + lane;
@@ -1558,7 +1564,7 @@ int nonorth_max_rows(int pitch, int nt)
 template <class S, int RP, int NT>
 void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                     void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
-                    int nzch, hipStream_t st)
+                    int nzch, hipStream_t st, const void *coef = nullptr)
 {
     using A = typename Tr<S>::A;
     constexpr int HS = NT >= 512 ? 2 : 3;      // halo slots: rows up to 64 (256) or 85 (512) points
@@ -1582,6 +1588,7 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k <= kMaxR; ++k) p.c[ax][k] = (A)g.c[ax][k];
     p.a = (A)a; p.b = (A)b; p.s = (A)s;
+    p.coef = static_cast<const A *>(coef);
     for (int ax = 0; ax < 3; ++ax)
         for (int sd = 0; sd < 2; ++sd)
             for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (A)g.phase[ax][sd][q];
@@ -1601,7 +1608,7 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
 template <class S, int RP, int NT>
 void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo, void *out,
                          long ldout, const void *veff, int ncol, double a, double b, double s, int nzch,
-                         hipStream_t st)
+                         hipStream_t st, const void *coef = nullptr)
 {
     using A = typename Tr<S>::A;
     constexpr int HS = NT >= 512 ? 2 : 3;
@@ -1637,6 +1644,7 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     q.ayz = (A)g.alpha[2];
     q.ihz = (A)(1.0 / g.h[2]);
     p.a = (A)a; p.b = (A)b; p.s = (A)s;
+    p.coef = static_cast<const A *>(coef);
     for (int ax = 0; ax < 3; ++ax)
         for (int sd = 0; sd < 2; ++sd)
             for (int k = 0; k < 2; ++k) p.ph[ax][sd][k] = (A)g.phase[ax][sd][k];
@@ -1666,7 +1674,7 @@ int vec_width(int nx)
 template <class S, int VW, int RV, int NT, int MODE, class T = S>
 bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                      void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
-                     int nzch, bool align, hipStream_t st)
+                     int nzch, bool align, hipStream_t st, const void *coef = nullptr)
 {
     constexpr int XO = (kMaxR + VW - 1) / VW * VW;
     constexpr int HS = NT >= 512 ? 1 : 2;      // halo runs per thread
@@ -1695,6 +1703,7 @@ bool launch_step_vec(const StencilGeom &g, const void *x, long ldx, const void *
     for (int ax = 0; ax < 3; ++ax)
         for (int k = 0; k <= kMaxR; ++k) p.c[ax][k] = (S)g.c[ax][k];
     p.a = (S)a; p.b = (S)b; p.s = (S)s;
+    p.coef = static_cast<const S *>(coef);
     for (int ax = 0; ax < 3; ++ax)
         for (int sd = 0; sd < 2; ++sd)
             for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (S)g.phase[ax][sd][q];
@@ -1771,7 +1780,7 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
 template <class S, class T>
 bool launch_vec_modes(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo, void *out,
                       long ldout, const void *veff, int ncol, double a, double b, double s, const StepCfg &cfg,
-                      hipStream_t st)
+                      hipStream_t st, const void *coef)
 {
     const int vw = vec_width<S>(g.nx);
     auto run = [&](auto rv, auto nt, auto acc) {
@@ -1779,10 +1788,10 @@ bool launch_vec_modes(const StencilGeom &g, const void *x, long ldx, const void 
         if constexpr (V16 == 4) {
             if (vw == 2)
                 return launch_step_vec<S, 2, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value, T>(
-                    g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
+                    g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st, coef);
         }
         return launch_step_vec<S, V16, decltype(rv)::value, decltype(nt)::value, decltype(acc)::value, T>(
-            g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st);
+            g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, cfg.align != 0, st, coef);
     };
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
@@ -1801,12 +1810,12 @@ bool launch_vec_modes(const StencilGeom &g, const void *x, long ldx, const void 
 
 void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                   void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
-                  const StepCfg &cfg, hipStream_t st)
+                  const StepCfg &cfg, hipStream_t st, const void *coef)
 {
     if (e == B16) {
         // bfloat16 storage, float arithmetic: run-vector kernels only
         if (!g.orth || !cfg.vec ||
-            !launch_vec_modes<float, bf16>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st))
+            !launch_vec_modes<float, bf16>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st, coef))
             fail(8, "the bfloat16 filter needs an orthogonal cell, even row length and 4-byte aligned columns");
         return;
     }
@@ -1820,28 +1829,28 @@ void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const vo
             constexpr int B2 = B / 4 > 0 ? B / 4 : 1;
             constexpr int B4 = B / 8 > 0 ? B / 8 : 1;
             if (cfg.nt >= 512) {
-                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
-                else launch_step_nonorth<S, B2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
+                else launch_step_nonorth<S, B2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
             } else {
-                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
-                else launch_step_nonorth<S, B2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                if (cfg.rp_div >= 2) launch_step_nonorth<S, B4, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
+                else launch_step_nonorth<S, B2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
             }
             return;
         }
         if constexpr (!Tr<S>::C) {
-            if (cfg.vec && launch_vec_modes<S, S>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st))
+            if (cfg.vec && launch_vec_modes<S, S>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg, st, coef))
                 return;
         }
         if (cfg.nt >= 512) {
             if (cfg.rp_div >= 2)
-                launch_step_rp<S, B / 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                launch_step_rp<S, B / 2, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
             else
-                launch_step_rp<S, B, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                launch_step_rp<S, B, 512>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
         } else {
             if (cfg.rp_div >= 2)
-                launch_step_rp<S, B / 2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                launch_step_rp<S, B / 2, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
             else
-                launch_step_rp<S, B, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st);
+                launch_step_rp<S, B, 256>(g, x, ldx, old, ldo, out, ldout, veff, ncol, a, b, s, cfg.nzch, st, coef);
         }
     });
 }
@@ -1865,7 +1874,7 @@ static void gather_npm(const NlDev &nl, const void *x, long ldx, void *beta, int
 
 template <class S, int NPM, class T = S>
 static void scatter_npm(const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
-                        hipStream_t st)
+                        const void *acoef, hipStream_t st)
 {
     using A = typename Tr<S>::A;
     NlArgs<A> p{nl.atom_pt_off, nl.pt_gpos, nl.pt_img, nl.pt_atom, nl.cov_off, nl.cov_gpos, nl.ent_pt,
@@ -1873,7 +1882,7 @@ static void scatter_npm(const NlDev &nl, const void *beta, void *y, long ldy, in
     constexpr int CPW = 4;
     dim3 grid((nl.n_cov + 63) / 64, (ncol + 4 * CPW - 1) / (4 * CPW));
     hipLaunchKernelGGL((k_nl_scatter_mc<S, NPM, CPW, T>), grid, dim3(256), 0, st, p, (const S *)beta, (T *)y, ldy,
-                       ncol, (A)a);
+                       ncol, (A)a, (const A *)acoef);
 }
 
 void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int ncol, hipStream_t st)
@@ -1895,20 +1904,20 @@ void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int 
 }
 
 void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
-                hipStream_t st)
+                hipStream_t st, const void *acoef)
 {
     if (nl.n_cov == 0) return;
     if (e == B16) {
-        if (nl.npm == 8) scatter_npm<float, 8, bf16>(nl, beta, y, ldy, ncol, a, st);
-        else if (nl.npm == 16) scatter_npm<float, 16, bf16>(nl, beta, y, ldy, ncol, a, st);
-        else scatter_npm<float, 32, bf16>(nl, beta, y, ldy, ncol, a, st);
+        if (nl.npm == 8) scatter_npm<float, 8, bf16>(nl, beta, y, ldy, ncol, a, acoef, st);
+        else if (nl.npm == 16) scatter_npm<float, 16, bf16>(nl, beta, y, ldy, ncol, a, acoef, st);
+        else scatter_npm<float, 32, bf16>(nl, beta, y, ldy, ncol, a, acoef, st);
         return;
     }
     dispatch(e, [&](auto t) {
         using S = decltype(t);
-        if (nl.npm == 8) scatter_npm<S, 8>(nl, beta, y, ldy, ncol, a, st);
-        else if (nl.npm == 16) scatter_npm<S, 16>(nl, beta, y, ldy, ncol, a, st);
-        else scatter_npm<S, 32>(nl, beta, y, ldy, ncol, a, st);
+        if (nl.npm == 8) scatter_npm<S, 8>(nl, beta, y, ldy, ncol, a, acoef, st);
+        else if (nl.npm == 16) scatter_npm<S, 16>(nl, beta, y, ldy, ncol, a, acoef, st);
+        else scatter_npm<S, 32>(nl, beta, y, ldy, ncol, a, acoef, st);
     });
 }
 

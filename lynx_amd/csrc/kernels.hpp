@@ -42,10 +42,12 @@ struct StepCfg {
     int align = 0;     // 16-byte-run kernels: band rows start at lane multiples of 16
 };
 
-// out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out)
+// out = a * ( -1/2 lap x + (veff + s) x ) + b * old    (old may alias out).
+// With coef (device memory, three elements of the arithmetic type), a, b, s
+// are read from coef[0..2] when the kernel runs instead.
 void stencil_step(int e, const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                   void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
-                  const StepCfg &cfg, hipStream_t st);
+                  const StepCfg &cfg, hipStream_t st, const void *coef = nullptr);
 int stencil_points_per_thread(int e, const StepCfg &cfg);
 // True when the shape can run the grid's x rows (halo slots and LDS pitch);
 // for vec shapes, also when nx is a multiple of the 16-byte run.
@@ -74,9 +76,10 @@ struct NlDev {
 // beta[n][atom*npm + j] = dV*Gamma_j * sum_points chi_j * x (with Bloch phase)
 void nl_gather(int e, const NlDev &nl, const void *x, long ldx, void *beta, int ncol,
                hipStream_t st);
-// y[g] += a * sum_entries chi . beta (conjugate Bloch phase)
+// y[g] += a * sum_entries chi . beta (conjugate Bloch phase); with acoef
+// (device memory, arithmetic type) a is read from acoef[0] instead
 void nl_scatter(int e, const NlDev &nl, const void *beta, void *y, long ldy, int ncol, double a,
-                hipStream_t st);
+                hipStream_t st, const void *acoef = nullptr);
 
 // ---- batched filter: many independent units, one launch per kernel ----
 // One unit: an operator (grid, coefficients, Bloch phases, Veff, nonlocal

@@ -4,6 +4,8 @@ Bars: FP64 results agree with the oracle to 1e-11 relative (stencil, nonlocal,
 filter) and 1e-9 (Ritz values of a whole solve, Lanczos bounds); FP32 to 2e-5
 relative.  Every call goes through the C ABI of liblynx_chefsi.so.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -363,3 +365,60 @@ def test_wide_blocks_nonlocal():
         assert rel(gotc, refc) < 1e-12
         g64 = H._apply(Xc.T.copy(), s.veff, 0.0, np.complex64).reshape(ncol, -1).T
         assert rel(g64, refc) < 2e-6
+
+
+def test_filter_graph_replays_match_direct_launches():
+    # the second filter call with the same buffers captures a HIP graph and
+    # later calls replay it with their own (a, b, shift) table: every call,
+    # with other GPU work and synchronisations between them, against the
+    # oracle, and bit-equal to direct launches of the same call; then the
+    # same with HIP-event timing on (its own graph with event records)
+    from lynx_amd import _lib as L
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    s = si8(20)
+    op, H = s.oracle_op(), s.hamiltonian()
+    H.set_veff(s.veff)
+    lib = L.lib()
+    k = (0.25 * 2 * np.pi / A_SI, 0.0, -0.125 * 2 * np.pi / A_SI)
+    cases = ((np.float64, L.F64, 1e-11, None), (np.float32, L.F32, 2e-5, None),
+             (np.complex128, L.C128, 1e-11, k), (np.complex64, L.C64, 2e-5, k))
+    for dt, code, tol, kp in cases:
+        H.set_kpoint(kp if kp is not None else (0.0, 0.0, 0.0))
+        op.set_kpoint(kp if kp is not None else (0.0, 0.0, 0.0))
+        X = s.random_block(6, 51, complex_=kp is not None)
+        xd = DeviceArray.from_numpy(np.ascontiguousarray(X.T.astype(dt)))
+        yd = DeviceArray(X.size * np.dtype(dt).itemsize)
+        params = [(0.30, -0.6, 40.0, 9), (0.45, -0.5, 42.0, 9), (0.35, -0.7, 39.0, 9), (0.30, -0.6, 40.0, 9)]
+        for i, (lc, lo, hi, deg) in enumerate(params):
+            L.check(lib.lynx_cheb_filter(H.ctx, code, xd.value, s.nd, yd.value, s.nd, 6, deg, lc, lo, hi,
+                                         L.MEM_DEVICE), H.ctx)
+            got = np.zeros((6, s.nd), dt)
+            yd.download(got)
+            ref = op.cheb_filter(X, s.veff, lc, lo, hi, deg)
+            assert rel(got.T, ref) < tol, (dt, i)
+            # direct launches of the same call
+            L.check(lib.lynx_set_filter_graphs(H.ctx, 0), H.ctx)
+            L.check(lib.lynx_cheb_filter(H.ctx, code, xd.value, s.nd, yd.value, s.nd, 6, deg, lc, lo, hi,
+                                         L.MEM_DEVICE), H.ctx)
+            L.check(lib.lynx_set_filter_graphs(H.ctx, 1), H.ctx)
+            direct = np.zeros((6, s.nd), dt)
+            yd.download(direct)
+            assert np.array_equal(got, direct), (dt, i)
+            # unrelated GPU work between the calls
+            H._apply(X.T.copy(), s.veff, 0.0, dt)
+            synchronize()
+        L.check(lib.lynx_set_timing(H.ctx, 1), H.ctx)
+        t_d = (ctypes.c_double * 4)()
+        t_n = (ctypes.c_int * 4)()
+        for i in range(3):
+            lc, lo, hi, deg = params[i]
+            L.check(lib.lynx_cheb_filter(H.ctx, code, xd.value, s.nd, yd.value, s.nd, 6, deg, lc, lo, hi,
+                                         L.MEM_DEVICE), H.ctx)
+            L.check(lib.lynx_last_filter_timing(H.ctx, t_d, t_n), H.ctx)
+            assert t_n[1] == deg and t_d[1] > 0, (dt, i)
+            got = np.zeros((6, s.nd), dt)
+            yd.download(got)
+            assert rel(got.T, op.cheb_filter(X, s.veff, lc, lo, hi, deg)) < tol, (dt, "timed", i)
+        L.check(lib.lynx_set_timing(H.ctx, 0), H.ctx)
+        xd.free()
+        yd.free()
