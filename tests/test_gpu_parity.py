@@ -422,3 +422,24 @@ def test_filter_graph_replays_match_direct_launches():
         L.check(lib.lynx_set_timing(H.ctx, 0), H.ctx)
         xd.free()
         yd.free()
+
+
+@pytest.mark.parametrize("n", [(10, 9, 8), (14, 7, 13)])
+def test_small_grids_wrap_within_the_stencil_radius(n):
+    # grids no wider than 2R = 12 points along some axis: the halo rows and
+    # planes wrap onto the block's own rows (R = 6 < every dimension)
+    s = System(n, A_SI, DIAMOND, kpt=(0.3, -0.2, 0.1))
+    s.latvec = np.diag([A_SI * n[0] / 16, A_SI * n[1] / 16, A_SI * n[2] / 16])
+    op, H = s.oracle_op(), s.hamiltonian()
+    X = s.random_block(3, 61)
+    op.set_kpoint((0, 0, 0))
+    ref = op.apply(X, s.veff, c=0.2)
+    H.set_kpoint((0, 0, 0))
+    got = H.apply(H.halo, X.T.copy(), s.veff, 0.2).reshape(3, -1).T
+    assert rel(got, ref) < 1e-12
+    k = (0.3, -0.2, 0.1)
+    op.set_kpoint(k)
+    Xc = s.random_block(3, 62, complex_=True)
+    refc = op.apply(Xc, s.veff)
+    gotc = H.apply_kpt(H.halo, Xc.T.copy(), s.veff, k).reshape(3, -1).T
+    assert rel(gotc, refc) < 1e-12
