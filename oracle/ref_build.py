@@ -1,7 +1,7 @@
 """Compiles the buildable subset of the reference (read in place under
 /root/reference, never copied) with oracle/ref_driver.cpp into
 oracle/_ref/libref.so.  Test infrastructure only; a missing reference tree is
-not an error (the GPU box has none)."""
+not an error (machines that only run the built library have none)."""
 import os
 import subprocess
 import sys
