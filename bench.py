@@ -275,8 +275,8 @@ def run_hetero_batch(steps, warmup):
         bf.close()
         for d in xs + ys:
             d.free()
-    # one frame per call (FP64), the unbatched baseline; the second round is
-    # timed (the first pays each context's launch-shape tuning)
+    # one frame per call (FP64), the unbatched baseline; the second sweep over
+    # the frames is timed (the first pays each context's launch-shape tuning)
     xs = [DeviceArray.from_numpy(x) for x in X]
     ys = [DeviceArray(x.size * 8) for x in X]
     for rep in range(2):

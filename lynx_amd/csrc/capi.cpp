@@ -449,7 +449,7 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
         // the timed step reads the destination as its Y_{k-2} stream (in
         // place, as the recurrence does), so all three streams are measured.
         // The whole block is timed (the tail of the last wave of workgroups
-        // depends on the column count).  Two rounds: every candidate once
+        // depends on the column count).  Two timing passes: every candidate once
         // after an untimed launch, then the six fastest three more times;
         // each candidate's time is its minimum.
         auto time_one = [&](const gpu::StepCfg &k) {
