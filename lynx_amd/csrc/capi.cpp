@@ -803,7 +803,12 @@ double triangular_diag_ratio(lynx_ctx *c, int ew, int n)
 {
     const size_t es = gpu::elem_size(ew);
     std::vector<double> d(ew == gpu::C128 ? 2 * n : n);
-    HIP_OK(hipMemcpy2DAsync(d.data(), es, c->s1.p, (n + 1) * es, es, n, hipMemcpyDeviceToHost, c->stream));
+    // gather the diagonal on the device (column j's element j: n columns of
+    // one element at leading dimension n + 1), then one contiguous copy; a
+    // strided 2D copy of n single elements runs as n separate transfers
+    void *diag = c->evals.get((size_t)n * es);
+    gpu::copy_cols(ew, c->s1.p, n + 1, diag, 1, 1, n, c->stream);
+    HIP_OK(hipMemcpyAsync(d.data(), diag, (size_t)n * es, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     double lo = 1e300, hi = 0;
     for (int i = 0; i < n; ++i) {
