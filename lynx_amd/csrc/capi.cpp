@@ -1133,17 +1133,25 @@ lynx_status lynx_set_veff(lynx_ctx *ctx, const double *veff, int mem)
         need_device(ctx);
         need_grid(ctx);
         const size_t n = ctx->nd;
+        // the device copy is overwritten before the check: a refused Veff
+        // leaves the context without one until the next successful call
+        ctx->has_veff = false;
         ctx->veff64.get(n * sizeof(double));
         HIP_OK(hipMemcpyAsync(ctx->veff64.p, veff, n * sizeof(double),
                               mem == LYNX_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               ctx->stream));
-        std::vector<double> hv(n);
-        HIP_OK(hipMemcpyAsync(hv.data(), ctx->veff64.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_OK(hipStreamSynchronize(ctx->stream));
-        for (double v : hv)
-            if (!std::isfinite(v)) fail(LYNX_E_NONFINITE, "non-finite effective potential");
-        std::vector<float> hf(hv.begin(), hv.end());
-        upload(ctx->veff32, hf, ctx->stream);
+        // finiteness on the device: 0*v is 0 for a finite v and NaN for an
+        // infinite or NaN one, so sum_i v_i (0 v_i) is finite exactly when
+        // every entry is (and cannot overflow)
+        void *zv = ctx->lz[0].get(n * sizeof(double));
+        ctx->partials.get(gpu::kPartialBlocks * sizeof(double));
+        gpu::scale_copy(gpu::F64, ctx->veff64.p, zv, 0.0, (long)n, ctx->stream);
+        const double chk = host_sum(
+            ctx, gpu::dot_partials(gpu::F64, ctx->veff64.p, zv, (long)n, ctx->partials.as<double>(), ctx->stream));
+        if (!std::isfinite(chk)) fail(LYNX_E_NONFINITE, "non-finite effective potential");
+        // the FP32 copy for the single-precision kernels (round to nearest)
+        ctx->veff32.get(n * sizeof(float));
+        gpu::convert_cols(gpu::F64, ctx->veff64.p, (long)n, gpu::F32, ctx->veff32.p, (long)n, (long)n, 1, ctx->stream);
         HIP_OK(hipStreamSynchronize(ctx->stream));
         ctx->has_veff = true;
     });

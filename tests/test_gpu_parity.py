@@ -118,6 +118,39 @@ def test_filter_fp32(si8_24):
     assert rel(Y, ref) < 2e-5
 
 
+def test_set_veff_refuses_non_finite_values(si8_24):
+    """lynx_set_veff checks finiteness on the device: a NaN or infinite entry
+    is refused with LYNX_E_NONFINITE and leaves the context without a Veff
+    (LYNX_E_STATE until the next good call); a huge finite entry is accepted.
+    After a good Veff the FP32 filter, which reads the FP32 copy made on the
+    device, matches the oracle again."""
+    s, op, H = si8_24
+    from lynx_amd import _lib as L
+    lib = L.lib()
+    X = s.random_block(4, 8)
+    X32 = np.asfortranarray(X, np.float32)
+    Y = np.zeros_like(X32)
+
+    def run():
+        return lib.lynx_cheb_filter(H.ctx, L.F32, L.vptr(X32), X.shape[0], L.vptr(Y), X.shape[0], 4, 10,
+                                    0.6, -0.4, 45.0, L.MEM_HOST)
+
+    for pos, bad in ((len(s.veff) // 3, np.nan), (0, np.inf), (len(s.veff) - 1, -np.inf)):
+        v = s.veff.copy()
+        v[pos] = bad
+        assert lib.lynx_set_veff(H.ctx, L.dptr(v), L.MEM_HOST) == L.E_NONFINITE, bad
+        assert run() == L.E_STATE, bad
+    v = s.veff.copy()
+    v[5] = 1e300
+    assert lib.lynx_set_veff(H.ctx, L.dptr(v), L.MEM_HOST) == L.OK
+    # through the C ABI: the wrapper's set_veff skips an array it uploaded last
+    good = np.ascontiguousarray(s.veff, np.float64)
+    assert lib.lynx_set_veff(H.ctx, L.dptr(good), L.MEM_HOST) == L.OK
+    H._veff_key = None
+    L.check(run(), H.ctx)
+    assert rel(Y, op.cheb_filter(X, s.veff, 0.6, -0.4, 45.0, 10)) < 2e-5
+
+
 def _subspace_dist(A, B, dV):
     # distance between the spans: || P_A - P_B || via orthonormal bases
     qa, _ = np.linalg.qr(A)
