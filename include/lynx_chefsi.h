@@ -119,8 +119,11 @@ lynx_status lynx_get_nonlocal(const lynx_ctx *ctx, int *img_atom, int *img_ndc,
 /* ------------------------------------------------- potential, k-point */
 
 /* Effective potential, Nd doubles (host or device pointer); the context
- * keeps its own device copy.  Replaces upload_Veff_sync
- * (EigenSolver.cu:1028-1033). */
+ * keeps its own device copy (FP64, and an FP32 copy rounded to nearest for
+ * the single-precision paths).  A NaN or infinite entry returns
+ * LYNX_E_NONFINITE and leaves the context without a potential: calls that
+ * need one return LYNX_E_STATE until the next successful call.  Replaces
+ * upload_Veff_sync (EigenSolver.cu:1028-1033). */
 lynx_status lynx_set_veff(lynx_ctx *ctx, const double *veff, int mem);
 /* k-point in the lattice-aligned reciprocal basis (k_i = 2 pi k_red_i / L_i),
  * as passed to Hamiltonian::apply_kpt (Hamiltonian.hpp:78-80) and
