@@ -229,6 +229,9 @@ class Hamiltonian:
             raise ValueError("Veff must have Nd entries")
         key = (v.ctypes.data, v.size, hash(v[:: max(1, v.size // 64)].tobytes()))
         if key != self._veff_key:
+            # a refused potential leaves the context without one: forget the
+            # previous upload first so that it is sent again next time
+            self._veff_key = None
             L.check(L.lib().lynx_set_veff(self.ctx, L.dptr(v), L.MEM_HOST), self.ctx)
             self._veff_key = key
 

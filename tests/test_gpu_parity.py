@@ -148,7 +148,20 @@ def test_set_veff_refuses_non_finite_values(si8_24):
     assert lib.lynx_set_veff(H.ctx, L.dptr(good), L.MEM_HOST) == L.OK
     H._veff_key = None
     L.check(run(), H.ctx)
-    assert rel(Y, op.cheb_filter(X, s.veff, 0.6, -0.4, 45.0, 10)) < 2e-5
+    ref = op.cheb_filter(X, s.veff, 0.6, -0.4, 45.0, 10)
+    assert rel(Y, ref) < 2e-5
+    # the wrapper: a refused potential raises, and the array uploaded before
+    # it is sent again by the next call
+    H.set_veff(s.veff)
+    v = s.veff.copy()
+    v[7] = np.nan
+    with pytest.raises(L.LynxError):
+        H.set_veff(v)
+    assert run() == L.E_STATE
+    H.set_veff(s.veff)
+    Y[:] = 0
+    L.check(run(), H.ctx)
+    assert rel(Y, ref) < 2e-5
 
 
 def _subspace_dist(A, B, dV):
