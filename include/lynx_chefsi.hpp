@@ -83,6 +83,9 @@ public:
     void set_veff(const double *Veff) const
     {
         if (Veff != last_veff_) {
+            // a refused potential leaves the context without one: forget the
+            // previous pointer first so that it is uploaded again next time
+            last_veff_ = nullptr;
             check(lynx_set_veff(ctx_, Veff, LYNX_MEM_HOST), ctx_);
             last_veff_ = Veff;
         }

@@ -6,6 +6,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
+#include <stdexcept>
 #include <vector>
 
 static std::vector<double> read_doubles(const char *path, size_t n)
@@ -58,5 +60,19 @@ int main(int argc, char **argv)
     std::printf("stage %.15e", ef);
     for (double e : e2) std::printf(" %.15e", e);
     std::printf("\n");
+
+    // a refused potential throws and leaves the context without one; the
+    // array set before it is then uploaded again by the next call
+    std::vector<double> bad = veff;
+    bad[3] = std::numeric_limits<double>::quiet_NaN();
+    int threw = 0;
+    try {
+        H.set_veff(bad.data());
+    } catch (const std::runtime_error &) {
+        threw = 1;
+    }
+    double lo2 = 0, hi2 = 0;
+    es.lanczos_bounds(veff.data(), nd, lo2, hi2);
+    std::printf("refused %d %.15e %.15e\n", threw, lo2, hi2);
     return 0;
 }

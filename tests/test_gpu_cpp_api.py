@@ -40,3 +40,6 @@ def test_cpp_classes_match_python_api(tmp_path):
     e2, _, ef, _ = stage.step(0, 1.0, s.veff, [X.copy(order="F")])
     assert abs(lines["stage"][0] - ef) < 1e-10
     assert np.abs(lines["stage"][1:] - e2).max() < 1e-10
+    # a NaN Veff is refused with an exception; the earlier array then works again
+    assert lines["refused"][0] == 1
+    assert np.allclose(lines["refused"][1:], [lo, hi], rtol=1e-12, atol=1e-12)
