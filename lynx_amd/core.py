@@ -227,13 +227,16 @@ class Hamiltonian:
         v = np.ascontiguousarray(Veff, np.float64).ravel()
         if v.size != self.grid.Nd():
             raise ValueError("Veff must have Nd entries")
-        key = (v.ctypes.data, v.size, hash(v[:: max(1, v.size // 64)].tobytes()))
-        if key != self._veff_key:
-            # a refused potential leaves the context without one: forget the
-            # previous upload first so that it is sent again next time
-            self._veff_key = None
-            L.check(L.lib().lynx_set_veff(self.ctx, L.dptr(v), L.MEM_HOST), self.ctx)
-            self._veff_key = key
+        # skip the upload only when the values are bit-identical to the last
+        # accepted ones (the caller may change an array in place)
+        last = self._veff_key
+        if last is not None and last.size == v.size and np.array_equal(last.view(np.int64), v.view(np.int64)):
+            return
+        # a refused potential leaves the context without one: forget the
+        # previous upload first so that it is sent again next time
+        self._veff_key = None
+        L.check(L.lib().lynx_set_veff(self.ctx, L.dptr(v), L.MEM_HOST), self.ctx)
+        self._veff_key = v.copy()
 
     def set_kpoint(self, kpt_cart):
         k = np.ascontiguousarray(kpt_cart, np.float64)

@@ -164,6 +164,19 @@ def test_set_veff_refuses_non_finite_values(si8_24):
     assert rel(Y, ref) < 2e-5
 
 
+def test_set_veff_in_place_change_is_uploaded(si8_24):
+    """The wrapper skips an upload only for bit-identical values: a change
+    made in place to one entry of the same array reaches the device."""
+    s, op, H = si8_24
+    X = s.random_block(3, 4)
+    v = s.veff.copy()
+    H.set_veff(v)
+    v[1] += 0.75
+    got = H.apply(H.halo, X.T.copy().ravel(), v, 0.0).reshape(3, -1).T
+    assert rel(got, op.apply(X, v)) < 1e-11
+    H.set_veff(s.veff)
+
+
 def _subspace_dist(A, B, dV):
     # distance between the spans: || P_A - P_B || via orthonormal bases
     qa, _ = np.linalg.qr(A)
