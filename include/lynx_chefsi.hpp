@@ -13,6 +13,7 @@
 #include "lynx_chefsi.h"
 
 #include <complex>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -50,6 +51,7 @@ public:
         check(lynx_set_grid(ctx_, nx, ny, nz, latvec, orthogonal ? LYNX_CELL_ORTHOGONAL : LYNX_CELL_NONORTHOGONAL,
                             fd_order), ctx_);
         nd_ = nx * ny * nz;
+        last_veff_.clear();
     }
     // Nonlocal projectors from psp8 files (Crystal::setup + NonlocalProjector::setup).
     void set_atoms(const std::vector<std::string> &psp8, const std::vector<int> &type,
@@ -80,23 +82,26 @@ public:
     lynx_ctx *ctx() const { return ctx_; }
     int Nd() const { return nd_; }
 
+    // Uploads Veff unless its Nd_d values are bit-identical to the last
+    // accepted ones (the caller may change an array in place between calls).
     void set_veff(const double *Veff) const
     {
-        if (Veff != last_veff_) {
-            // a refused potential leaves the context without one: forget the
-            // previous pointer first so that it is uploaded again next time
-            last_veff_ = nullptr;
-            check(lynx_set_veff(ctx_, Veff, LYNX_MEM_HOST), ctx_);
-            last_veff_ = Veff;
-        }
+        const size_t bytes = sizeof(double) * static_cast<size_t>(nd_);
+        if (!last_veff_.empty() && std::memcmp(last_veff_.data(), Veff, bytes) == 0)
+            return;
+        // a refused potential leaves the context without one: forget the
+        // previous values first so that they are uploaded again next time
+        last_veff_.clear();
+        check(lynx_set_veff(ctx_, Veff, LYNX_MEM_HOST), ctx_);
+        last_veff_.assign(Veff, Veff + nd_);
     }
-    // Forget the cached potential pointer (call after changing its contents in place).
-    void invalidate_veff() const { last_veff_ = nullptr; }
+    // Forget the cached potential so that the next call uploads it.
+    void invalidate_veff() const { last_veff_.clear(); }
 
 private:
     lynx_ctx *ctx_ = nullptr;
     int nd_ = 0;
-    mutable const double *last_veff_ = nullptr;
+    mutable std::vector<double> last_veff_;
 };
 
 // EigenSolver with the reference's solve/solve_kpt/lanczos_bounds signatures

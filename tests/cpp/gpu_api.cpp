@@ -4,6 +4,7 @@
 // test can compare them with the same calls made through the Python API.
 #include "lynx_chefsi.hpp"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
@@ -74,5 +75,23 @@ int main(int argc, char **argv)
     double lo2 = 0, hi2 = 0;
     es.lanczos_bounds(veff.data(), nd, lo2, hi2);
     std::printf("refused %d %.15e %.15e\n", threw, lo2, hi2);
+
+    // a change made in place to one entry of the same array reaches the
+    // device: H(v + d e_j) x - H(v) x = d x_j e_j
+    std::vector<double> x(psi2.begin(), psi2.begin() + nd), y0(nd), y1(nd);
+    int j = 0;
+    for (int i = 1; i < nd; ++i)
+        if (std::fabs(x[i]) > std::fabs(x[j])) j = i;
+    const double d = 0.75;
+    H.apply(x.data(), veff.data(), y0.data(), 1);
+    veff[j] += d;
+    H.apply(x.data(), veff.data(), y1.data(), 1);
+    double err = 0, ymax = 0;
+    for (int i = 0; i < nd; ++i) {
+        const double want = (i == j) ? d * x[i] : 0.0;
+        err = std::fmax(err, std::fabs(y1[i] - y0[i] - want));
+        ymax = std::fmax(ymax, std::fabs(y0[i]));
+    }
+    std::printf("inplace %.6e %.6e %.6e\n", err, ymax, std::fabs(d * x[j]));
     return 0;
 }

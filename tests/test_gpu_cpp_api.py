@@ -43,3 +43,7 @@ def test_cpp_classes_match_python_api(tmp_path):
     # a NaN Veff is refused with an exception; the earlier array then works again
     assert lines["refused"][0] == 1
     assert np.allclose(lines["refused"][1:], [lo, hi], rtol=1e-12, atol=1e-12)
+    # a change made in place to one Veff entry reaches the device
+    err, ymax, step = lines["inplace"]
+    assert step > 1e-3
+    assert err < 1e-12 * max(1.0, ymax)
