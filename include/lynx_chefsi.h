@@ -181,6 +181,20 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
 lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, double *eig_max,
                                 double tol, int max_iter, unsigned seed, int *iters);
 
+/* ------------------------------------------------- initial psi (a17) */
+
+/* Initial wavefunctions: nband columns of nd values (leading dimension ld,
+ * 0 means nd), uniform in [-0.5, 0.5] from glibc's srand(seed)/rand()
+ * stream, column by column; complex entries draw the real part, then the
+ * imaginary part.  LYNX_F32 / LYNX_C64 blocks receive the same values
+ * rounded.  Replaces Wavefunction::randomize / randomize_kpt
+ * (Wavefunction.cpp:53-76); the reference's SCF seeds spin s with
+ * s*1000 + 1 (SCF.cpp:863-881).  The stream is drawn from a private state
+ * (the caller's rand() sequence is not advanced).  ctx may be NULL for host
+ * blocks; device blocks are uploaded on the context's stream. */
+lynx_status lynx_randomize(lynx_ctx *ctx, int dtype, void *psi, int nd, int ld, int nband, unsigned seed,
+                           int mem);
+
 /* ------------------------------------------------------ diagnostics */
 
 /* Host-only helpers, usable without a GPU (no context needed). */
@@ -231,6 +245,16 @@ lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps);
  * FP64 only (EigenSolver.cpp:558-681). */
 enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1, LYNX_FILTER_BF16 = 2 };
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
+lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision);
+/* Number of lynx_set_veff calls made on the context so far (accepted or
+ * refused).  A caller that skips re-uploading an unchanged potential keeps
+ * the generation of its own last upload and uploads again when another
+ * caller (the eigen stage, another wrapper) has set a potential since.  No
+ * reference counterpart. */
+lynx_status lynx_get_veff_generation(const lynx_ctx *ctx, long *generation);
+/* Number of lynx_chefsi_solve calls completed on the context (each CheFSI
+ * pass over one block).  No reference counterpart (a diagnostic). */
+lynx_status lynx_solve_count(const lynx_ctx *ctx, long *count);
 /* Counters of the FP32 passes of this context.  *count: passes redone in
  * FP64 from the unchanged psi, because the filtered block was too
  * ill-conditioned for single precision (Cholesky factor diagonal spread above
@@ -305,6 +329,13 @@ const char *lynx_scf_eig_last_error(const lynx_scf_eig *driver);
 lynx_status lynx_scf_eig_step(lynx_scf_eig *driver, int scf_iter, double scf_residual, const double *veff,
                               void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
                               int *precision_used);
+/* Fermi level and occupations of nspin x nkpt x nband eigenvalues with
+ * k-point weights (summing to 1) and the spin factor 2 for nspin = 1:
+ * the root of sum w f(e - Ef) = nelectron, f Fermi-Dirac (smearing 0) or
+ * 0.5 erfc(beta x) (1).  Host-only; replaces Occupation::compute
+ * (Occupation.cpp:86-176).  occ (may be NULL) receives f(e - Ef). */
+lynx_status lynx_fermi_level(int nspin, int nkpt, int nband, const double *eig, const double *kpt_weights,
+                             double nelectron, double beta, int smearing, double *occ, double *fermi);
 /* Current lambda_cutoff and per-spin eig_min / eig_max (nspin each). */
 lynx_status lynx_scf_eig_state(const lynx_scf_eig *driver, double *lambda_cutoff, double *eig_min,
                                double *eig_max);

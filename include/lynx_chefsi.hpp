@@ -3,9 +3,17 @@
 //
 // A caller of LYNX's src/solvers/EigenSolver.hpp and
 // src/operators/Hamiltonian.hpp switches to the MI355X library by building
-// these objects instead: the argument lists, their meaning (column-major psi
-// with leading dimension ld, Nd_d grid points, eigvals ascending, Veff of
-// Nd_d doubles) and the side effect on lambda_cutoff() are the same.
+// these objects instead.  The work calls keep the reference's argument
+// lists: EigenSolver::solve / solve_kpt / lanczos_bounds[_kpt] and
+// Hamiltonian::apply / apply_kpt, with the same meaning (column-major psi with
+// leading dimension ld, Nd_d grid points, eigvals ascending, Veff of Nd_d
+// doubles) and the same side effect on lambda_cutoff().  The set-up calls do
+// not: the reference's EigenSolver::setup(const LynxContext&, const
+// Hamiltonian&) (EigenSolver.hpp:32) and Hamiltonian::setup(stencil, domain,
+// grid, halo, vnl) (Hamiltonian.hpp:49-53) take LYNX's own grid objects,
+// which this library does not have; here Hamiltonian::setup takes the grid
+// as point counts, lattice rows and FD order, set_atoms the psp8 files and
+// positions, and EigenSolver::setup the Hamiltonian alone.
 // Failures throw std::runtime_error, as the reference's solve does on a
 // Cholesky or eigensolver failure (EigenSolver.cpp:633-635, 667-669).
 #pragma once
@@ -82,18 +90,23 @@ public:
     lynx_ctx *ctx() const { return ctx_; }
     int Nd() const { return nd_; }
 
-    // Uploads Veff unless its Nd_d values are bit-identical to the last
-    // accepted ones (the caller may change an array in place between calls).
+    // Uploads Veff unless its Nd_d values are bit-identical to the last ones
+    // this object uploaded (the caller may change an array in place between
+    // calls) and no other caller has set a potential on the context since
+    // (ScfEigenStage sets each spin's Veff itself).
     void set_veff(const double *Veff) const
     {
         const size_t bytes = sizeof(double) * static_cast<size_t>(nd_);
-        if (!last_veff_.empty() && std::memcmp(last_veff_.data(), Veff, bytes) == 0)
+        long gen = -1;
+        check(lynx_get_veff_generation(ctx_, &gen), ctx_);
+        if (!last_veff_.empty() && gen == last_gen_ && std::memcmp(last_veff_.data(), Veff, bytes) == 0)
             return;
         // a refused potential leaves the context without one: forget the
         // previous values first so that they are uploaded again next time
         last_veff_.clear();
         check(lynx_set_veff(ctx_, Veff, LYNX_MEM_HOST), ctx_);
         last_veff_.assign(Veff, Veff + nd_);
+        check(lynx_get_veff_generation(ctx_, &last_gen_), ctx_);
     }
     // Forget the cached potential so that the next call uploads it.
     void invalidate_veff() const { last_veff_.clear(); }
@@ -102,6 +115,7 @@ private:
     lynx_ctx *ctx_ = nullptr;
     int nd_ = 0;
     mutable std::vector<double> last_veff_;
+    mutable long last_gen_ = -1;
 };
 
 // EigenSolver with the reference's solve/solve_kpt/lanczos_bounds signatures

@@ -49,6 +49,7 @@ SIGNATURES = {
     "lynx_chefsi_solve": (_I, [_P, _I, _P, _I, _I, _DP, _D, _D, _D, _I, _DP, _I]),
     "lynx_lanczos_bounds": (_I, [_P, _I, _DP, _DP, _D, _I, ctypes.c_uint, _IP]),
     "lynx_fd_weights": (_I, [_I, _DP, _DP]),
+    "lynx_randomize": (_I, [_P, _I, _P, _I, _I, _I, ctypes.c_uint, _I]),
     "lynx_psp8_info": (_I, [ctypes.c_char_p, _DP, _IP, _IP, _IP, _DP]),
     "lynx_kpoints": (_I, [_I, _I, _I, _DP, _DP, _I, _DP, _DP, _DP, _I, _IP]),
     "lynx_last_filter_timing": (_I, [_P, _DP, _IP]),
@@ -57,6 +58,10 @@ SIGNATURES = {
     "lynx_stream_ceiling": (_I, [_P, _L, _I, _DP]),
     "lynx_mixed_fallbacks": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)]),
     "lynx_set_filter_precision": (_I, [_P, _I]),
+    "lynx_get_filter_precision": (_I, [_P, _IP]),
+    "lynx_get_veff_generation": (_I, [_P, ctypes.POINTER(_L)]),
+    "lynx_solve_count": (_I, [_P, ctypes.POINTER(_L)]),
+    "lynx_fermi_level": (_I, [_I, _I, _I, _DP, _DP, _D, _D, _I, _DP, _DP]),
     "lynx_get_grid_dims": (_I, [_P, _IP]),
     "lynx_scf_eig_create": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _DP]),
     "lynx_scf_eig_destroy": (_I, [_P]),

@@ -204,6 +204,7 @@ class Hamiltonian:
     def __init__(self, device_id=0):
         self._c = _Context(device_id)
         self._veff_key = None
+        self._veff_gen = -1
         self.stencil = None
         self._kpt = (0.0, 0.0, 0.0)
 
@@ -228,15 +229,28 @@ class Hamiltonian:
         if v.size != self.grid.Nd():
             raise ValueError("Veff must have Nd entries")
         # skip the upload only when the values are bit-identical to the last
-        # accepted ones (the caller may change an array in place)
+        # ones this object uploaded (the caller may change an array in place)
+        # and no other caller has set a potential on the context since (the
+        # eigen stage sets each spin's Veff itself)
         last = self._veff_key
-        if last is not None and last.size == v.size and np.array_equal(last.view(np.int64), v.view(np.int64)):
+        if (last is not None and self._veff_gen == self.veff_generation() and last.size == v.size
+                and np.array_equal(last.view(np.int64), v.view(np.int64))):
             return
         # a refused potential leaves the context without one: forget the
         # previous upload first so that it is sent again next time
         self._veff_key = None
         L.check(L.lib().lynx_set_veff(self.ctx, L.dptr(v), L.MEM_HOST), self.ctx)
         self._veff_key = v.copy()
+        self._veff_gen = self.veff_generation()
+
+    def veff_generation(self):
+        g = ctypes.c_long()
+        L.check(L.lib().lynx_get_veff_generation(self.ctx, ctypes.byref(g)), self.ctx)
+        return g.value
+
+    def invalidate_veff(self):
+        """Forget the cached potential so that the next call uploads it."""
+        self._veff_key = None
 
     def set_kpoint(self, kpt_cart):
         k = np.ascontiguousarray(kpt_cart, np.float64)
@@ -266,6 +280,52 @@ class Hamiltonian:
     def apply_kpt(self, halo, psi, Veff, kpt_cart, c=0.0):
         self.set_kpoint(kpt_cart)
         return self._apply(psi, Veff, c, np.complex128)
+
+
+class Wavefunction:
+    """Orbitals, eigenvalues and occupations per (spin, k-point), with the
+    pylynx Wavefunction interface (bind_electronic.cpp:15-44).  ``psi`` blocks
+    are (Nd_d, Nband) Fortran-ordered arrays; ``randomize`` fills one from
+    glibc's srand(seed)/rand() stream through lynx_randomize
+    (Wavefunction.cpp:53-76)."""
+
+    def __init__(self):
+        self._nd = self._nband = self._nspin = self._nkpts = 0
+        self._complex = False
+        self._psi, self._eig, self._occ = [], [], []
+
+    def allocate(self, Nd_d, Nband, Nspin, Nkpts, is_complex=False, Nspinor=1):
+        if Nspinor != 1:
+            raise ValueError("spinor wavefunctions are outside the CheFSI path")
+        self._nd, self._nband, self._nspin, self._nkpts = int(Nd_d), int(Nband), int(Nspin), int(Nkpts)
+        self._complex = bool(is_complex)
+        dt = np.complex128 if self._complex else np.float64
+        n = self._nspin * self._nkpts
+        self._psi = [np.zeros((self._nd, self._nband), dt, order="F") for _ in range(n)]
+        self._eig = [np.zeros(self._nband) for _ in range(n)]
+        self._occ = [np.zeros(self._nband) for _ in range(n)]
+        return self
+
+    def Nd_d(self): return self._nd
+    def Nband(self): return self._nband
+    def Nband_global(self): return self._nband
+    def Nspin(self): return self._nspin
+    def Nkpts(self): return self._nkpts
+    def is_complex(self): return self._complex
+
+    def _i(self, spin, kpt):
+        if not (0 <= spin < self._nspin and 0 <= kpt < self._nkpts):
+            raise IndexError("spin or k-point out of range")
+        return spin * self._nkpts + kpt
+
+    def randomize(self, spin, kpt, seed=42):
+        X = self._psi[self._i(spin, kpt)]
+        L.check(L.lib().lynx_randomize(None, L.dtype_code(X.dtype), L.vptr(X), self._nd, self._nd, self._nband,
+                                       int(seed), L.MEM_HOST))
+
+    def psi(self, spin=0, kpt=0): return self._psi[self._i(spin, kpt)]
+    def eigenvalues(self, spin=0, kpt=0): return self._eig[self._i(spin, kpt)]
+    def occupations(self, spin=0, kpt=0): return self._occ[self._i(spin, kpt)]
 
 
 class LynxContext:

@@ -83,6 +83,7 @@ struct lynx_ctx {
     int nd = 0;
 
     bool has_veff = false;
+    long veff_gen = 0;                             // bumped by every lynx_set_veff call
     DevBuf veff64, veff32;
     double kpt[3] = {0, 0, 0};
 
@@ -107,6 +108,7 @@ struct lynx_ctx {
 
     // precision of the filter inside lynx_chefsi_solve (LYNX_FILTER_*)
     int filter_precision = 0;
+    long solves = 0;                   // completed lynx_chefsi_solve calls
     long fp64_overlap_fallbacks = 0;   // FP32 passes redone in FP64
     long cholqr2_passes = 0;           // FP32 passes with a second Cholesky-QR
 
@@ -813,6 +815,7 @@ double triangular_diag_ratio(lynx_ctx *c, int ew, int n)
     double lo = 1e300, hi = 0;
     for (int i = 0; i < n; ++i) {
         const double v = ew == gpu::C128 ? std::hypot(d[2 * i], d[2 * i + 1]) : std::fabs(d[i]);
+        if (!std::isfinite(v)) return 1e300;   // a non-finite factor: the pass is redone in FP64
         lo = std::min(lo, v);
         hi = std::max(hi, v);
     }
@@ -1135,6 +1138,7 @@ lynx_status lynx_set_veff(lynx_ctx *ctx, const double *veff, int mem)
         const size_t n = ctx->nd;
         // the device copy is overwritten before the check: a refused Veff
         // leaves the context without one until the next successful call
+        ++ctx->veff_gen;
         ctx->has_veff = false;
         ctx->veff64.get(n * sizeof(double));
         HIP_OK(hipMemcpyAsync(ctx->veff64.p, veff, n * sizeof(double),
@@ -1382,8 +1386,14 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
                 void *H = ctx->s2.get((size_t)nband * nband * es);
                 gram(ctx, es1, Cs, ldw, Ws, ldw, nband, ctx->st.dV, H);
                 gpu::hermitian_part(dtype, H, nband, ctx->stream);
-                // a11 in FP64
-                subspace_eig(ctx, dtype, nband, eigvals);
+                // a11 in FP64 (a failed or non-finite eigensolve of the
+                // single-precision block also sends the pass to FP64)
+                try {
+                    subspace_eig(ctx, dtype, nband, eigvals);
+                } catch (const Error &err) {
+                    if (err.status != LYNX_E_EIG && err.status != LYNX_E_NONFINITE) throw;
+                    return false;
+                }
                 if (!(eigvals[0] >= eig_min - 0.05 * (eig_max - eig_min)) || !(eigvals[nband - 1] <= eig_max))
                     return false;
                 // a12: psi = Cs Q
@@ -1398,6 +1408,7 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
                                             ctx->stream));
                 HIP_OK(hipStreamSynchronize(ctx->stream));
                 if (lambda_c_out) *lambda_c_out = eigvals[nband - 1] + 0.1;
+                ++ctx->solves;
                 return;
             }
             ++ctx->fp64_overlap_fallbacks;
@@ -1422,6 +1433,7 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
             HIP_OK(hipMemcpy2DAsync(psi, ldp * es, P, nd * es, nd * es, nband, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
         if (lambda_c_out) *lambda_c_out = eigvals[nband - 1] + 0.1;
+        ++ctx->solves;
     });
 }
 
@@ -1493,6 +1505,59 @@ lynx_status lynx_lanczos_bounds(lynx_ctx *ctx, int dtype, double *eig_min, doubl
         *eig_min = emin - 0.1;
         *eig_max = emax * 1.01;
         if (iters) *iters = j;
+    });
+}
+
+lynx_status lynx_randomize(lynx_ctx *ctx, int dtype, void *psi, int nd_, int ld, int nband, unsigned seed, int mem)
+{
+    if (!ctx && mem == LYNX_MEM_DEVICE) return LYNX_E_ARG;
+    lynx_ctx host_only;
+    host_only.device = -1;
+    lynx_ctx *c = ctx ? ctx : &host_only;
+    return guarded(c, [&] {
+        check_dtype(dtype);
+        if (dtype == LYNX_BF16) fail(LYNX_E_UNSUPPORTED, "initial wavefunctions are FP64 or FP32 blocks");
+        if (nband <= 0 || nd_ <= 0 || !psi) fail(LYNX_E_ARG, "bad block arguments");
+        if (mem == LYNX_MEM_DEVICE) need_device(c);
+        const long nd = nd_, ldp = ld ? ld : nd;
+        if (ldp < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const bool cplx = gpu::elem_complex(dtype), single = gpu::elem_single(dtype);
+        // glibc's rand() stream after srand(seed) (random() with its default
+        // 128-byte state), drawn from a private state so that the caller's
+        // own rand() sequence is left alone
+        char state[128];
+        random_data rd;
+        std::memset(&rd, 0, sizeof(rd));
+        if (initstate_r(seed, state, sizeof(state), &rd) != 0) fail(LYNX_E_ARG, "initstate_r failed");
+        auto draw = [&] {
+            int32_t r = 0;
+            random_r(&rd, &r);
+            return -0.5 + 1.0 * ((double)r / RAND_MAX);
+        };
+        // Wavefunction.cpp:53-76: column by column; complex entries draw the
+        // real part, then the imaginary part
+        const long vals = cplx ? 2 * nd : nd;
+        const size_t es = gpu::elem_size(dtype);
+        // device blocks: staged 32 columns at a time, one 2D copy each
+        const int chunk = mem == LYNX_MEM_DEVICE ? 32 : 1;
+        std::vector<double> buf((size_t)vals * chunk);
+        std::vector<float> buff(single ? (size_t)vals * chunk : 0);
+        for (int j0 = 0; j0 < nband; j0 += chunk) {
+            const int nc = std::min(chunk, nband - j0);
+            for (long i = 0; i < vals * nc; ++i) buf[i] = draw();
+            const void *src = buf.data();
+            if (single) {
+                for (long i = 0; i < vals * nc; ++i) buff[i] = (float)buf[i];
+                src = buff.data();
+            }
+            char *dst = static_cast<char *>(psi) + (size_t)j0 * ldp * es;
+            if (mem == LYNX_MEM_DEVICE) {
+                HIP_OK(hipMemcpy2DAsync(dst, ldp * es, src, nd * es, nd * es, nc, hipMemcpyHostToDevice, c->stream));
+                HIP_OK(hipStreamSynchronize(c->stream));
+            } else {
+                std::memcpy(dst, src, nd * es);
+            }
+        }
     });
 }
 
@@ -1627,6 +1692,27 @@ lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision)
     return LYNX_OK;
 }
 
+lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision)
+{
+    if (!ctx || !precision) return LYNX_E_ARG;
+    *precision = ctx->filter_precision;
+    return LYNX_OK;
+}
+
+lynx_status lynx_solve_count(const lynx_ctx *ctx, long *count)
+{
+    if (!ctx || !count) return LYNX_E_ARG;
+    *count = ctx->solves;
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_veff_generation(const lynx_ctx *ctx, long *generation)
+{
+    if (!ctx || !generation) return LYNX_E_ARG;
+    *generation = ctx->veff_gen;
+    return LYNX_OK;
+}
+
 lynx_status lynx_set_timing(lynx_ctx *ctx, int enable)
 {
     if (!ctx) return LYNX_E_ARG;
@@ -1705,6 +1791,7 @@ struct lynx_batch {
         double kpt[3];
         DevBuf ph64, ph32;     // image phases e^{-i k.R} of this unit's k-point
         DevBuf work, dx, dy;   // recurrence work block; staging for host buffers
+        long epoch = -1;       // the context's set-up epoch the phases were built for
     };
     std::vector<Unit> units;
     // plan and graph of the last call; rebuilt when buffers, shapes or dtype change
@@ -1731,6 +1818,24 @@ struct lynx_batch {
 };
 
 namespace {
+// Bloch factors e^{-i k.R_img} of the unit's images at its own k-point,
+// from the context's current projector set-up (rebuilt when a set-up call
+// on the context has changed its images).
+void unit_phases(lynx_batch::Unit &U, hipStream_t st)
+{
+    const NonlocalHost &h = U.ctx->nlh;
+    std::vector<double> ph(2 * std::max(1, h.n_img));
+    for (int i = 0; i < h.n_img; ++i) {
+        const double *sh = &h.img_shift[3 * i];
+        const double th = -(U.kpt[0] * sh[0] + U.kpt[1] * sh[1] + U.kpt[2] * sh[2]);
+        ph[2 * i] = std::cos(th);
+        ph[2 * i + 1] = std::sin(th);
+    }
+    upload(U.ph64, ph, st);
+    upload(U.ph32, std::vector<float>(ph.begin(), ph.end()), st);
+    U.epoch = U.ctx->epoch;
+}
+
 template <class F>
 lynx_status bguarded(lynx_batch *b, F &&f)
 {
@@ -1770,16 +1875,7 @@ lynx_status lynx_batch_create(lynx_batch **out, int nunit, lynx_ctx *const *ctxs
             lynx_batch::Unit &U = b->units[u];
             U.ctx = ctxs[u];
             for (int a = 0; a < 3; ++a) U.kpt[a] = kpts ? kpts[3 * u + a] : 0.0;
-            const NonlocalHost &h = U.ctx->nlh;
-            std::vector<double> ph(2 * std::max(1, h.n_img));
-            for (int i = 0; i < h.n_img; ++i) {
-                const double *sh = &h.img_shift[3 * i];
-                const double th = -(U.kpt[0] * sh[0] + U.kpt[1] * sh[1] + U.kpt[2] * sh[2]);
-                ph[2 * i] = std::cos(th);
-                ph[2 * i + 1] = std::sin(th);
-            }
-            upload(U.ph64, ph, b->stream);
-            upload(U.ph32, std::vector<float>(ph.begin(), ph.end()), b->stream);
+            unit_phases(U, b->stream);
         }
         HIP_OK(hipStreamSynchronize(b->stream));
     });
@@ -1830,8 +1926,17 @@ lynx_status lynx_batch_filter(lynx_batch *b, int dtype, void *const *x, const lo
         const size_t es = gpu::elem_size(dtype);
         const bool single = gpu::elem_single(dtype);
         const bool cplx = gpu::elem_complex(dtype);
-        // the contexts' own set-up work (Veff, projectors) must be complete
-        for (auto &U : b->units) HIP_OK(hipStreamSynchronize(U.ctx->stream));
+        // the contexts' own set-up work (Veff, projectors) must be complete;
+        // a context whose grid or projectors were set up again since the
+        // last call gets new phase tables, and its new set-up epoch and Veff
+        // buffers enter the key, so the plan and graph are rebuilt
+        for (auto &U : b->units) {
+            HIP_OK(hipStreamSynchronize(U.ctx->stream));
+            need_gpu_stencil(U.ctx);
+            need_veff(U.ctx);
+            if (U.ctx->epoch != U.epoch) unit_phases(U, b->stream);
+        }
+        HIP_OK(hipStreamSynchronize(b->stream));
         std::vector<gpu::BatchUnit> bu(n);
         std::vector<long> key = {dtype, degree % 2, mem};
         for (int u = 0; u < n; ++u) {
@@ -1875,6 +1980,11 @@ lynx_status lynx_batch_filter(lynx_batch *b, int dtype, void *const *x, const lo
             key.push_back(lX);
             key.push_back(lY);
             key.push_back(ncol[u]);
+            key.push_back(c->epoch);
+            key.push_back(reinterpret_cast<long>(c->veff64.p));
+            key.push_back(reinterpret_cast<long>(c->veff32.p));
+            key.push_back(reinterpret_cast<long>(U.ph64.p));
+            key.push_back(reinterpret_cast<long>(U.ph32.p));
         }
         if (key != b->key || !b->plan) {
             if (b->graph) (void)hipGraphExecDestroy(b->graph);

@@ -125,6 +125,15 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
                               int *precision_used)
 {
     if (!d || !veff || !psi || !eig) return LYNX_E_ARG;
+    // the context's filter precision is the caller's setting: the step sets
+    // its own and restores the caller's on every return
+    int caller_precision = LYNX_FILTER_FP64;
+    if (lynx_get_filter_precision(d->ctx, &caller_precision) != LYNX_OK) return LYNX_E_ARG;
+    struct Restore {
+        lynx_ctx *c;
+        int p;
+        ~Restore() { (void)lynx_set_filter_precision(c, p); }
+    } restore{d->ctx, caller_precision};
     try {
         const lynx_scf_eig_params &p = d->p;
         const int dtype = p.complex_kpt ? LYNX_C128 : LYNX_F64;
@@ -181,6 +190,30 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
         return LYNX_OK;
     } catch (const Fail &f) {
         d->err = f.msg;
+        return f.st;
+    }
+}
+
+lynx_status lynx_fermi_level(int nspin, int nkpt, int nband, const double *eig, const double *kpt_weights,
+                             double nelectron, double beta, int smearing, double *occ, double *fermi)
+{
+    if (nspin < 1 || nspin > 2 || nkpt < 1 || nband < 1 || !eig || !kpt_weights || !fermi || !(beta > 0) ||
+        (smearing != 0 && smearing != 1))
+        return LYNX_E_ARG;
+    try {
+        const double spin_fac = nspin == 1 ? 2.0 : 1.0;
+        std::vector<double> e, w;
+        for (int s = 0; s < nspin; ++s)
+            for (int k = 0; k < nkpt; ++k)
+                for (int n = 0; n < nband; ++n) {
+                    e.push_back(eig[((long)s * nkpt + k) * nband + n]);
+                    w.push_back(kpt_weights[k] * spin_fac);
+                }
+        *fermi = fermi_level(e, w, nelectron, beta, smearing);
+        if (occ)
+            for (size_t i = 0; i < e.size(); ++i) occ[i] = smear(e[i] - *fermi, beta, smearing);
+        return LYNX_OK;
+    } catch (const Fail &f) {
         return f.st;
     }
 }

@@ -90,23 +90,3 @@ class DeviceArray:
 
 def synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
-
-
-def _copy_probe(nbytes):
-    """Device-to-device copy of nbytes, repeated 5 times: the reference
-    traffic for calibrating HBM byte counters of the profiler."""
-    a = DeviceArray(nbytes)
-    b = DeviceArray(nbytes)
-    a.fill_bytes(1)
-    for _ in range(5):
-        _ok(hip().hipMemcpy(b.ptr, a.ptr, nbytes, D2D), "hipMemcpy D2D")
-    synchronize()
-    print(f"copied {nbytes} bytes x5")
-
-
-if __name__ == "__main__":
-    import sys
-    if len(sys.argv) == 3 and sys.argv[1] == "copy":
-        _copy_probe(int(sys.argv[2]))
-    else:
-        print("usage: python -m lynx_amd.hipmem copy NBYTES")

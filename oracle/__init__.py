@@ -44,7 +44,16 @@ PINNING = {
     "ylm_real": "orthonormality on the sphere and the addition theorem",
     "chi": "addition theorem makes V_nl independent of the m convention; sphere membership by brute force",
     "cheb_filter": "closed form p_m(lambda) = T_m((l-c)/e)/T_m((lmin-c)/e) on eigenvectors",
-    "lanczos": "bounds bracket the dense spectrum; glibc rand() start vector",
+    "lanczos": "the reference's own lanczos_bounds<T> (Lanczos.hpp, LAPACK dsterf_) run on this oracle's H.v: "
+               "same step count and bounds; bounds bracket the dense spectrum; glibc rand() start vector",
+    "rand_uniform": "the reference's Wavefunction::randomize[_kpt], compiled from source, bitwise",
+    "fermi_level": "the reference's Occupation::compute, compiled from source; closed forms (symmetric levels, "
+                   "one half-filled level, sum w f = Ne)",
+    "ScfEigStage": "composed of the pinned solve, Lanczos and Fermi level; its bookkeeping by properties a "
+                   "plausible mistake fails: a constant Veff shift of one spin shifts that spin's Lanczos bounds "
+                   "exactly and makes it set lambda_c; eig_min kept at the Lanczos value in iteration 0 and "
+                   "taken from eig(s, k0)[0] afterwards; eig_max fixed; solve count rho_trigger / nchefsi "
+                   "passes x spins x k-points; sum w f = Ne",
     "solve": "dense eigh of the explicit H on a small grid (subspace converges to lowest N)",
     "kpoints": "MP 6x6x6 -> 112 after time reversal (KPoints.cpp:55-70); weights sum",
 }
@@ -748,40 +757,64 @@ def fermi_level(eigs, weights, nelectron, beta, kind=0):
 
 
 class ScfEigStage:
-    """SCF::estimate_spectral_bounds + SCF::solve_eigenproblem for one spin
-    and the k-points of one operator, FP64 throughout."""
+    """SCF::estimate_spectral_bounds (SCF.cpp:883-935) + SCF::solve_eigenproblem
+    (SCF.cpp:310-467) for nspin spins and the k-points of one operator, FP64
+    throughout.  veff holds nspin x Nd values (spin-major); psis holds the
+    nspin x nkpt blocks in the order s * nkpt + k."""
 
     def __init__(self, op, nband, nelectron, beta, kpts=None, kpt_weights=None, degree=20, rho_trigger=4,
-                 nchefsi=1, kind=0):
+                 nchefsi=1, kind=0, nspin=1):
         self.op, self.nband, self.ne, self.beta, self.kind = op, nband, nelectron, beta, kind
         self.kpts = [None] if kpts is None else list(kpts)
         nk = len(self.kpts)
         self.kw = np.full(nk, 1.0 / nk) if kpt_weights is None else np.asarray(kpt_weights, float)
         self.degree, self.rho_trigger, self.nchefsi = degree, rho_trigger, nchefsi
+        self.nspin = nspin
         self.bounds = None
+        self.solves = 0
 
     def step(self, scf_iter, veff, psis):
         op = self.op
+        nd = op.grid.nd
+        veff = np.asarray(veff, float).reshape(self.nspin, nd)
+        nk = len(self.kpts)
         if self.bounds is None:
-            if self.kpts[0] is not None:
-                op.set_kpoint(self.kpts[0])
-            mn, mx, _ = op.lanczos_bounds(veff)
-            self.eig_min, self.eig_max = mn, mx
-            self.lambda_c = 0.5 * (mn + mx)
+            # SCF.cpp:912-929: Lanczos per spin at the first k-point
+            self.eig_min = np.zeros(self.nspin)
+            self.eig_max = np.zeros(self.nspin)
+            for s in range(self.nspin):
+                if self.kpts[0] is not None:
+                    op.set_kpoint(self.kpts[0])
+                self.eig_min[s], self.eig_max[s], _ = op.lanczos_bounds(veff[s])
+            self.lambda_c = 0.5 * (self.eig_min[0] + self.eig_max[0])
             self.bounds = True
+        # SCF.cpp:323: rho_trigger passes in the first iteration
         passes = self.rho_trigger if scf_iter == 0 else self.nchefsi
-        eigs = [None] * len(self.kpts)
+        eigs = [None] * (self.nspin * nk)
         for _ in range(passes):
-            for k, kp in enumerate(self.kpts):
-                if kp is not None:
-                    op.set_kpoint(kp)
-                psis[k], eigs[k], _ = op.solve(psis[k], veff, self.lambda_c, self.eig_min, self.eig_max,
-                                               self.degree)
-            if scf_iter > 0:
-                self.eig_min = eigs[0][0]
-            self.lambda_c = eigs[0][-1] + 0.1
+            for s in range(self.nspin):
+                for k, kp in enumerate(self.kpts):
+                    if kp is not None:
+                        op.set_kpoint(kp)
+                    i = s * nk + k
+                    psis[i], eigs[i], _ = op.solve(psis[i], veff[s], self.lambda_c, self.eig_min[s],
+                                                   self.eig_max[s], self.degree)
+                    self.solves += 1
+            # SCF.cpp:436-449: eig_min per spin from k-point 0 (iterations
+            # after the first), lambda_c from the largest last eigenvalue
+            last_max = -1e30
+            for s in range(self.nspin):
+                e0 = eigs[s * nk]
+                if scf_iter > 0:
+                    self.eig_min[s] = e0[0]
+                last_max = max(last_max, e0[-1])
+            self.lambda_c = last_max + 0.1
+            # Occupation.cpp:86-176: spin factor 2 for one spin
+            spin_fac = 2.0 if self.nspin == 1 else 1.0
             e = np.concatenate(eigs)
-            w = np.concatenate([np.full(self.nband, 2.0 * self.kw[k]) for k in range(len(self.kpts))])
+            w = np.concatenate([np.full(self.nband, spin_fac * self.kw[k]) for s in range(self.nspin)
+                                for k in range(nk)])
             self.fermi = fermi_level(e, w, self.ne, self.beta, self.kind)
             self.occ = smearing(e - self.fermi, self.beta, self.kind)
+            self.weights = w
         return psis, np.concatenate(eigs)

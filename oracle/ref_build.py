@@ -11,7 +11,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "_ref")
 SOURCES = ["core/Lattice.cpp", "core/FDGrid.cpp", "core/Domain.cpp", "core/KPoints.cpp",
            "core/NumericalMethods.cpp", "operators/FDStencil.cpp", "parallel/HaloExchange.cpp",
-           "atoms/Pseudopotential.cpp"]
+           "atoms/Pseudopotential.cpp", "electronic/Wavefunction.cpp", "electronic/Occupation.cpp",
+           "parallel/MPIComm.cpp"]
+# MPICH (Occupation's communicator type; no MPI call is made with null
+# communicators) and the MKL runtime (LAPACK dsterf_ for the Lanczos
+# template) from the container's conda tree
+CONDA = "/opt/conda"
 
 
 def main():
@@ -20,7 +25,12 @@ def main():
         return 0
     os.makedirs(OUT, exist_ok=True)
     srcs = [os.path.join(REF, s) for s in SOURCES] + [os.path.join(HERE, "ref_driver.cpp")]
-    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I", REF, "-o", os.path.join(OUT, "libref.so")] + srcs
+    cmd = (["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I", REF, "-I", os.path.join(CONDA, "include"),
+            "-o", os.path.join(OUT, "libref.so")] + srcs +
+           ["-L", os.path.join(CONDA, "lib"), "-lmkl_rt", "-lmpi",
+            # the system's libstdc++ (the one g++ compiled against) is found
+            # before the conda tree's older copy
+            "-Wl,-rpath,/usr/lib/x86_64-linux-gnu", "-Wl,-rpath," + os.path.join(CONDA, "lib")])
     return subprocess.run(cmd).returncode
 
 

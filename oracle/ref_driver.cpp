@@ -4,8 +4,10 @@
 // oracle's restatement (and the library's host set-up) against the
 // reference's own code for the leaf steps that build without MPI, LAPACK or
 // the rest of the program: FD stencil, lattice/grid, halo wrap, psp8 reader
-// and splines, k-points, and the start vector of the Lanczos bound
-// estimate (only its header-inline random fill is instantiated).
+// and splines, k-points, the Lanczos bound estimate (its template, with the
+// caller's matvec and the LAPACK dsterf_ of the MKL runtime), the random
+// initial wavefunctions, and the occupations and Fermi level (Occupation
+// with null communicators, linked against MPICH without MPI_Init).
 #include "core/Domain.hpp"
 #include "core/FDGrid.hpp"
 #include "core/KPoints.hpp"
@@ -15,6 +17,8 @@
 #include "operators/FDStencil.hpp"
 #include "parallel/HaloExchange.hpp"
 #include "solvers/Lanczos.hpp"
+#include "electronic/Occupation.hpp"
+#include "electronic/Wavefunction.hpp"
 
 #include <complex>
 #include <cstring>
@@ -158,6 +162,70 @@ void ref_lanczos_start(int complex_, int n, unsigned seed, double *out)
     } else {
         lanczos_detail::random_init(out, n);
     }
+}
+
+// lynx::lanczos_bounds<T> (Lanczos.hpp:104-198) on the caller's operator:
+// matvec(x, y) with n doubles (real) or n interleaved (re, im) pairs.
+// *nmatvec receives the number of operator applications.
+typedef void (*ref_matvec_fn)(const double *x, double *y);
+void ref_lanczos(int complex_, int n, ref_matvec_fn mv, double tol, int max_iter, int seed, double *emin,
+                 double *emax, int *nmatvec)
+{
+    int calls = 0;
+    if (complex_) {
+        std::function<void(const std::complex<double> *, std::complex<double> *)> f =
+            [&](const std::complex<double> *x, std::complex<double> *y) {
+                ++calls;
+                mv(reinterpret_cast<const double *>(x), reinterpret_cast<double *>(y));
+            };
+        lanczos_bounds<std::complex<double>>(f, n, *emin, *emax, tol, max_iter, seed);
+    } else {
+        std::function<void(const double *, double *)> f = [&](const double *x, double *y) {
+            ++calls;
+            mv(x, y);
+        };
+        lanczos_bounds<double>(f, n, *emin, *emax, tol, max_iter, seed);
+    }
+    *nmatvec = calls;
+}
+
+// Wavefunction::randomize / randomize_kpt (Wavefunction.cpp:53-76) of one
+// (spin, k-point) block of nd x nband: out holds nd*nband doubles (real) or
+// interleaved (re, im) pairs, column-major.
+void ref_randomize(int complex_, int nd, int nband, unsigned seed, double *out)
+{
+    Wavefunction w;
+    w.allocate(nd, nband, 1, 1, complex_ != 0);
+    if (complex_) {
+        w.randomize_kpt(0, 0, seed);
+        for (int j = 0; j < nband; ++j)
+            std::memcpy(out + 2L * nd * j, w.psi_kpt(0, 0).col(j), sizeof(double) * 2 * nd);
+    } else {
+        w.randomize(0, 0, seed);
+        for (int j = 0; j < nband; ++j) std::memcpy(out + (long)nd * j, w.psi(0, 0).col(j), sizeof(double) * nd);
+    }
+}
+
+// Occupation::compute (Occupation.cpp:86-176) with null communicators:
+// eigs holds nspin x nkpt x nband values; smearing 0 Fermi-Dirac, 1
+// Gaussian (this library's numbering); returns the Fermi level and fills
+// occ in the same layout.
+double ref_occupation(int nspin, int nkpt, int nband, const double *eigs, const double *kpt_weights,
+                      double nelectron, double beta, int smearing, double *occ)
+{
+    Wavefunction w;
+    w.allocate(1, nband, nspin, nkpt, false);
+    for (int s = 0; s < nspin; ++s)
+        for (int k = 0; k < nkpt; ++k)
+            for (int n = 0; n < nband; ++n) w.eigenvalues(s, k)(n) = eigs[((long)s * nkpt + k) * nband + n];
+    std::vector<double> kw(kpt_weights, kpt_weights + nkpt);
+    MPIComm none;
+    const SmearingType st = smearing == 1 ? SmearingType::GaussianSmearing : SmearingType::FermiDirac;
+    const double ef = Occupation::compute(w, nelectron, beta, st, kw, none, none, 0);
+    for (int s = 0; s < nspin; ++s)
+        for (int k = 0; k < nkpt; ++k)
+            for (int n = 0; n < nband; ++n) occ[((long)s * nkpt + k) * nband + n] = w.occupations(s, k)(n);
+    return ef;
 }
 
 }  // extern "C"

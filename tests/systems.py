@@ -184,3 +184,48 @@ def h2o(seed, n=50, box=20.0, sigma=0.05):
 def si2_fcc(n=24, **kw):
     """Si2 in the FCC primitive cell (non-orthogonal), a = 10.26 Bohr."""
     return System(n, A_SI, SI2, latvec=A_SI * FCC_PRIMITIVE, **kw)
+
+
+# -- the measured workloads ------------------------------------------------------
+
+A_SI64, A_SI216 = 2 * A_SI, 30.78
+
+
+def si64_filter_bench():
+    """Si64 filter-only operator workload: the 2x2x2 diamond supercell
+    (a = 20.52 Bohr), atoms displaced by sigma = 0.05 Bohr (seed 1), 60^3 grid."""
+    return System(60, A_SI64, displace(supercell(2), A_SI64, 0.05, seed=1))
+
+
+def si216_mixed():
+    """Si216: the 3x3x3 supercell (a = 30.78 Bohr), sigma = 0.03 Bohr (seed 3), 90^3 grid."""
+    return System(90, A_SI216, displace(supercell(3), A_SI216, 0.03, seed=3))
+
+
+def hetero_frames():
+    """The 64-frame heterogeneous batch: (class, system, orbitals) for 32 Si8
+    (30^3, 24), 16 Si16 (30x30x60, 40), 8 BaTiO3 (28^3, 30), 8 H2O (50^3, 8),
+    sigma = 0.05 Bohr (seeds 4 + frame)."""
+    frames = [("Si8", System(30, A_SI, displace(DIAMOND, A_SI, 0.05, seed=4 + i)), 24) for i in range(32)]
+    frames += [("Si16", si16(36 + i), 40) for i in range(16)]
+    frames += [("BaTiO3", batio3(52 + i), 30) for i in range(8)]
+    frames += [("H2O", h2o(60 + i), 8) for i in range(8)]
+    return frames
+
+
+def fcc_kpoints(s, k=(6, 6, 6)):
+    """Monkhorst-Pack k-points of a cell, reduced by time reversal
+    (lynx_kpoints), in the lattice-aligned basis the library takes."""
+    import ctypes
+    from lynx_amd import _lib as L
+    lv = np.ascontiguousarray(s.latvec, float)
+    cell = L.CELL_ORTHOGONAL if s.orth else L.CELL_NONORTHOGONAL
+    n = ctypes.c_int()
+    sh = np.zeros(3)
+    L.check(L.lib().lynx_kpoints(k[0], k[1], k[2], L.dptr(sh), L.dptr(lv), cell, None, None, None, 0,
+                                 ctypes.byref(n)))
+    kc = np.zeros(3 * n.value)
+    w = np.zeros(n.value)
+    L.check(L.lib().lynx_kpoints(k[0], k[1], k[2], L.dptr(sh), L.dptr(lv), cell, L.dptr(kc), None, L.dptr(w),
+                                 n.value, ctypes.byref(n)))
+    return kc.reshape(-1, 3), w

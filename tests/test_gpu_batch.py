@@ -106,3 +106,36 @@ def test_batch_errors():
     with pytest.raises(L.LynxError):
         bf.filter([s.random_block(2, 1)], 5, [50.0], [-1.0], [40.0])   # eig_max below lambda_cutoff
     bf.close()
+
+
+def test_batch_follows_a_reconfigured_context():
+    # a member context set up again after the batch was made (other atoms on
+    # the same grid, then a larger grid): the batch rebuilds its plan, phase
+    # tables and graph and matches the oracle of the new set-up
+    from lynx_amd import core as C
+    s1 = System(18, A_SI, displace(DIAMOND, A_SI, 0.05, seed=21))
+    H = s1.hamiltonian()
+    H.set_veff(s1.veff)
+    k = kvec(s1, (0.25, -0.125, 0.0))
+    bf = C.BatchFilter([(H, k), (H, None)])
+    lc, lo, hi = zip(*[_bounds(i) for i in range(2)])
+    systems = (s1, System(18, A_SI, displace(DIAMOND, A_SI, 0.08, seed=22)),
+               System(20, A_SI, displace(DIAMOND, A_SI, 0.05, seed=23)))
+    for rep, s in enumerate(systems):
+        if rep > 0:
+            lat = C.Lattice(s.latvec, C.CellType.Orthogonal)
+            grid = C.FDGrid(*s.n, lat)
+            st = C.FDStencil(12, grid, lat)
+            dom = C.Domain(grid)
+            vnl = C.NonlocalProjector().setup_atoms(s.psp_files, s.types, s.frac, True)
+            H.setup(st, dom, grid, C.HaloExchange(dom, st.FDn()), vnl)
+            H.set_veff(s.veff)
+        blocks = [s.random_block(3, 90 + rep, complex_=True), s.random_block(2, 95 + rep, complex_=True)]
+        op = s.oracle_op()
+        for graph_pass in range(2):
+            got = bf.filter(blocks, 6, lc, lo, hi, dtype=np.complex128)
+            for i, kk in enumerate((k, (0.0, 0.0, 0.0))):
+                op.set_kpoint(kk)
+                ref = op.cheb_filter(blocks[i], s.veff, lc[i], lo[i], hi[i], 6)
+                assert rel(got[i], ref) < 1e-11, (rep, graph_pass, i)
+    bf.close()

@@ -34,7 +34,7 @@ def test_eigen_stage_matches_oracle_fp64(si8_16):
         assert prec == 0
         assert np.abs(e - e_ref).max() < 1e-9 * max(1, np.abs(e_ref).max()), it
         lc, mn, mx = drv.state()
-        assert abs(lc - ref.lambda_c) < 1e-9 and abs(mn[0] - ref.eig_min) < 1e-9 and abs(mx[0] - ref.eig_max) < 1e-9
+        assert abs(lc - ref.lambda_c) < 1e-9 and abs(mn[0] - ref.eig_min[0]) < 1e-9 and abs(mx[0] - ref.eig_max[0]) < 1e-9
         assert abs(ef - ref.fermi) < 1e-9
         assert np.abs(occ - ref.occ).max() < 1e-7
 
@@ -77,3 +77,61 @@ def test_eigen_stage_fp32_complex_kpoints(si8_16):
         assert prmx == 1
         assert np.abs(emx - e64).max() < 1e-5, it
         assert abs(efmx - ef64) < 1e-5
+
+
+def test_eigen_stage_two_spins_matches_oracle(si8_16):
+    # nspin = 2 with spin 1's Veff shifted by a constant: per-spin Lanczos
+    # bounds, lambda_c from the larger last eigenvalue over spins, eig_min
+    # updates after the first iteration, the pass count (rho_trigger, then
+    # nchefsi, over both spins) through the context's solve counter
+    import ctypes
+    from lynx_amd import core, _lib as L
+    s = si8_16
+    op, H = s.oracle_op(), s.hamiltonian()
+    veff = np.concatenate([s.veff, s.veff + 0.05])
+    X0 = s.random_block(NB, 84)
+    ref = oracle.ScfEigStage(op, NB, NE, BETA, nspin=2, rho_trigger=3, nchefsi=2)
+    drv = core.ScfEigenStage(H, NB, NE, BETA, nspin=2, rho_trigger=3, nchefsi=2)
+    psi_ref = [X0.copy(order="F"), X0.copy(order="F")]
+    psi = [X0.copy(order="F"), X0.copy(order="F")]
+    cnt = ctypes.c_long()
+    for it in range(3):
+        psi_ref, e_ref = ref.step(it, veff, psi_ref)
+        e, occ, ef, prec = drv.step(it, 1.0, veff, psi)
+        assert np.abs(e - e_ref).max() < 1e-9 * max(1, np.abs(e_ref).max()), it
+        lc, mn, mx = drv.state()
+        assert abs(lc - ref.lambda_c) < 1e-9
+        assert np.abs(mn - ref.eig_min).max() < 1e-9 and np.abs(mx - ref.eig_max).max() < 1e-9
+        assert abs(ef - ref.fermi) < 1e-9 and np.abs(occ - ref.occ).max() < 1e-7
+        assert abs((ref.weights * occ).sum() - NE) < 1e-9
+        L.check(L.lib().lynx_solve_count(H.ctx, ctypes.byref(cnt)), H.ctx)
+        assert cnt.value == ref.solves == 6 + 4 * it
+
+
+def test_wrapper_uploads_veff_again_after_the_stage(si8_16):
+    # the eigen stage sets each spin's Veff on the context itself; a later
+    # H.apply with the array the wrapper uploaded before must not run on the
+    # stage's last (spin-1) potential, and the stage leaves the context's
+    # filter precision as the caller set it
+    import ctypes
+    from lynx_amd import core, _lib as L
+    s = si8_16
+    op, H = s.oracle_op(), s.hamiltonian()
+    vup = s.veff.copy()
+    vdn = s.veff + 0.3
+    X = s.random_block(3, 85)
+    y0 = H.apply(H.halo, X.T.copy(), vup).reshape(3, -1).T          # caches vup in the wrapper
+    L.check(L.lib().lynx_set_filter_precision(H.ctx, L.FILTER_BF16), H.ctx)
+    drv = core.ScfEigenStage(H, NB, NE, BETA, nspin=2, rho_trigger=1, fp64_switch_residual=1e-4)
+    psi = [s.random_block(NB, 86), s.random_block(NB, 87)]
+    drv.step(0, 1.0, np.concatenate([vup, vdn]), psi)
+    p = ctypes.c_int()
+    L.check(L.lib().lynx_get_filter_precision(H.ctx, ctypes.byref(p)), H.ctx)
+    assert p.value == L.FILTER_BF16
+    y1 = H.apply(H.halo, X.T.copy(), vup).reshape(3, -1).T
+    ref = op.apply(X, vup)
+    assert rel(y1, ref) < 1e-12 and rel(y0, ref) < 1e-12
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))

@@ -493,3 +493,48 @@ def test_fermi_level_closed_forms(kind):
     w = np.full(17, 2.0)
     ef = oracle.fermi_level(e, w, 19.0, 50.0, kind)
     assert abs((w * oracle.smearing(e - ef, 50.0, kind)).sum() - 19.0) < 1e-11
+
+
+# ------------------------------------------------------------ eigen stage
+
+
+def test_eigen_stage_bookkeeping_two_spins():
+    # SCF::estimate_spectral_bounds + solve_eigenproblem (SCF.cpp:310-467,
+    # 883-935) with two spins whose potentials differ by a constant c:
+    # - H + c has the same Krylov space, so its Lanczos run takes the same
+    #   steps with every eigenvalue estimate shifted by c (closed form);
+    # - spin 1's last eigenvalue is then the larger one and must set lambda_c
+    #   (a lambda_c taken from spin 0 alone fails);
+    # - eig_min keeps the Lanczos value in the first SCF iteration and takes
+    #   eig(s, k0)[0] afterwards; eig_max never changes;
+    # - rho_trigger passes in iteration 0, nchefsi later, each over every spin;
+    # - the occupations hold Ne electrons;
+    # - once converged, spin 1's eigenvalues are spin 0's plus c.
+    g, ps, op = _si8(12)
+    c = 0.05
+    v0 = np.random.default_rng(7).uniform(-1, 0, g.nd)
+    veff = np.concatenate([v0, v0 + c])
+    nb, ne, beta = 8, 10.0, 1.0 / (3.166811563e-6 * 1000.0)
+    mn0, mx0, it0 = op.lanczos_bounds(v0)
+    mn1, mx1, it1 = op.lanczos_bounds(v0 + c)
+    assert it1 == it0
+    assert abs((mn1 + 0.1) - (mn0 + 0.1) - c) < 1e-9
+    assert abs(mx1 / 1.01 - mx0 / 1.01 - c) < 1e-9
+    st = oracle.ScfEigStage(op, nb, ne, beta, nspin=2, rho_trigger=3, nchefsi=2)
+    X = np.asfortranarray(np.random.default_rng(8).uniform(-0.5, 0.5, (g.nd, nb)))
+    psis = [X.copy(order="F"), X.copy(order="F")]
+    psis, e = st.step(0, veff, psis)
+    assert st.solves == 3 * 2
+    assert list(st.eig_min) == [mn0, mn1] and list(st.eig_max) == [mx0, mx1]
+    e = e.reshape(2, nb)
+    assert e[1, -1] > e[0, -1] + 0.01
+    assert abs(st.lambda_c - (e[1, -1] + 0.1)) < 1e-15
+    assert abs((st.weights * st.occ).sum() - ne) < 1e-10
+    for it in range(1, 6):
+        psis, e = st.step(it, veff, psis)
+        e = e.reshape(2, nb)
+        assert st.solves == 6 + 4 * it
+        assert list(st.eig_min) == [e[0, 0], e[1, 0]]
+        assert list(st.eig_max) == [mx0, mx1]
+        assert abs(st.lambda_c - (max(e[0, -1], e[1, -1]) + 0.1)) < 1e-15
+    assert np.abs(e[1, :5] - e[0, :5] - c).max() < 1e-6

@@ -134,3 +134,84 @@ def test_lanczos_start_vector_matches_reference(ref, cplx):
     want = out[0::2] + 1j * out[1::2] if cplx else out
     got = oracle.lanczos_start(cplx, n, 1)
     assert np.array_equal(got, want)
+
+
+_MATVEC = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_lanczos_matches_reference(ref, cplx):
+    # the reference's own lanczos_bounds<T> (Lanczos.hpp:104-198, LAPACK
+    # dsterf_ from the MKL runtime) run with the oracle's H.v as its matvec:
+    # the oracle's Lanczos (its own recurrence, start vector and tridiagonal
+    # extremes) must stop after the same number of steps with the same bounds
+    from systems import si8
+    k = (0.25 * 2 * np.pi / A_SI, -0.125 * 2 * np.pi / A_SI, 0.375 * 2 * np.pi / A_SI) if cplx else (0, 0, 0)
+    s = si8(14, kpt=k)
+    op = s.oracle_op()
+    nd = s.nd
+
+    def mv(x, y):
+        n = 2 * nd if cplx else nd
+        xa = np.ctypeslib.as_array(x, (n,))
+        ya = np.ctypeslib.as_array(y, (n,))
+        v = xa.view(np.complex128) if cplx else xa
+        out = op.apply(v.reshape(nd, 1), s.veff, complex_=bool(cplx)).ravel()
+        ya[:] = out.view(np.float64) if cplx else out
+
+    cb = _MATVEC(mv)
+    for tol, max_iter in ((1e-2, 1000), (1e-6, 1000), (0.0, 7)):
+        mn, mx, nmv = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        ref.ref_lanczos(cplx, nd, cb, ctypes.c_double(tol), max_iter, 1, ctypes.byref(mn), ctypes.byref(mx),
+                        ctypes.byref(nmv))
+        omn, omx, it = op.lanczos_bounds(s.veff, tol=tol, max_iter=max_iter)
+        assert it == nmv.value - 1, (tol, it, nmv.value)
+        assert abs(omn - mn.value) <= 1e-10 * max(1.0, abs(mn.value)), (tol, omn, mn.value)
+        assert abs(omx - mx.value) <= 1e-10 * abs(mx.value), (tol, omx, mx.value)
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_randomize_matches_reference(ref, cplx):
+    # Wavefunction::randomize / randomize_kpt (Wavefunction.cpp:53-76) against
+    # the library's lynx_randomize (through the pylynx-style Wavefunction)
+    # and the oracle's glibc stream, bitwise
+    from lynx_amd import core
+    nd, nb, seed = 1013, 5, 1001
+    out = np.zeros(2 * nd * nb if cplx else nd * nb)
+    ref.ref_randomize(cplx, nd, nb, ctypes.c_uint(seed), P(out))
+    want = (out[0::2] + 1j * out[1::2] if cplx else out).reshape(nb, nd).T
+    w = core.Wavefunction().allocate(nd, nb, 2, 1, bool(cplx))
+    w.randomize(1, 0, seed)
+    assert np.array_equal(w.psi(1, 0), want)
+    assert not np.any(w.psi(0, 0))
+    stream = oracle.rand_uniform(seed, -0.5, 0.5, out.size)
+    assert np.array_equal(stream, out)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("nspin,nkpt", [(1, 1), (2, 3)])
+def test_occupation_matches_reference(ref, kind, nspin, nkpt):
+    # Occupation::compute (Occupation.cpp:86-176) against the oracle's Fermi
+    # level and the library's host lynx_fermi_level
+    from lynx_amd import _lib as L
+    nb = 13
+    rng = np.random.default_rng(11 + kind + nspin)
+    eig = np.sort(rng.uniform(-0.4, 0.3, (nspin, nkpt, nb)), axis=2).ravel()
+    kw = rng.uniform(0.5, 1.5, nkpt)
+    kw /= kw.sum()
+    ne, beta = 15.0, 1.0 / (3.166811563e-6 * 3000.0)
+    rocc = np.zeros_like(eig)
+    ref.ref_occupation.restype = ctypes.c_double
+    ref_ef = ref.ref_occupation(nspin, nkpt, nb, P(eig), P(kw), ctypes.c_double(ne), ctypes.c_double(beta), kind,
+                                P(rocc))
+    spin_fac = 2.0 if nspin == 1 else 1.0
+    w = np.concatenate([np.full(nb, spin_fac * kw[k]) for s in range(nspin) for k in range(nkpt)])
+    ef = oracle.fermi_level(eig, w, ne, beta, kind)
+    assert abs(ef - ref_ef) < 1e-12
+    assert np.abs(oracle.smearing(eig - ef, beta, kind) - rocc).max() < 1e-10
+    occ = np.zeros_like(eig)
+    lef = ctypes.c_double()
+    L.check(L.lib().lynx_fermi_level(nspin, nkpt, nb, L.dptr(eig), L.dptr(kw), ne, beta, kind, L.dptr(occ),
+                                     ctypes.byref(lef)))
+    assert abs(lef.value - ref_ef) < 1e-12
+    assert np.abs(occ - rocc).max() < 1e-10
