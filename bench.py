@@ -59,9 +59,8 @@ def profiled_traffic(shape, kernel_prefix):
 
 
 def build_system():
-    from systems import System, displace, supercell
-    frac = displace(supercell(2), A, 0.05, seed=1)
-    return System(N, A, frac)
+    from systems import si64_filter_bench
+    return si64_filter_bench()
 
 
 def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
@@ -111,6 +110,7 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
     Y = np.zeros((NCOL, nd), dtype=Xh.dtype)
     Yd.download(Y)
     finite = bool(np.all(np.isfinite(Y)))
+    Y = Y.T      # (nd, ncol) view of the column-major block
     esize = Xh.dtype.itemsize
     B = filter_bytes(nd, NCOL, DEGREE, esize)
     ms_step = 1e3 * elapsed / steps
@@ -131,6 +131,7 @@ def run_gpu(sysm, dtype_code, steps, warmup, X64, shape=None):
                         "vector_runs": vr.value,
                         "tuner_ms": [round(tms[i], 4) for i in range(nt.value)]},
         finite=finite,
+        Y=Y,
     )
     Xd.free(); Yd.free()
     return res
@@ -238,14 +239,11 @@ def run_hetero_batch(steps, warmup):
     stand-ins), 8 H2O frames (50^3, 8; O psp8, H in Veff only); sigma = 0.05
     Bohr (seed 4 + frame).  Filtered together in FP64 and FP32; per-class
     FP32-vs-FP64 error of the filtered blocks."""
-    from systems import System, A_SI, DIAMOND, displace, si16, batio3, h2o
+    from systems import hetero_frames
     from lynx_amd import _lib as L, core
     from lynx_amd.hipmem import DeviceArray, synchronize
     lib = L.lib()
-    frames = [("Si8", System(30, A_SI, displace(DIAMOND, A_SI, 0.05, seed=4 + i)), 24) for i in range(32)]
-    frames += [("Si16", si16(36 + i), 40) for i in range(16)]
-    frames += [("BaTiO3", batio3(52 + i), 30) for i in range(8)]
-    frames += [("H2O", h2o(60 + i), 8) for i in range(8)]
+    frames = hetero_frames()
     t0 = time.perf_counter()
     Hs, bnds = [], []
     for _, s, nb in frames:
@@ -310,11 +308,10 @@ def run_mixed_si216():
     the same with the filter's blocks stored in bfloat16.  The band-energy
     difference per atom uses the occupations of the FP64 run (Fermi-Dirac,
     1000 K).  The first stage also pays the launch-shape tuning."""
-    from systems import System, displace, supercell
+    from systems import si216_mixed
     from lynx_amd import _lib as L, core
     from lynx_amd.hipmem import DeviceArray, synchronize
-    a = 30.78
-    s = System(90, a, displace(supercell(3), a, 0.03, seed=3))
+    s = si216_mixed()
     nb, ne, beta = 480, 4.0 * 216, 1.0 / (3.166811563e-6 * 1000.0)
     rng = np.random.default_rng(7)
     X0 = rng.uniform(-0.5, 0.5, (nb, s.nd))
@@ -366,19 +363,38 @@ def stream_ceiling(sysm):
     return round(g.value, 1)
 
 
-def cpu_baseline(sysm, X64, lam, lo, hi):
-    """The oracle (single-threaded C restatement) on the same workload: the
-    same grid, atoms, degree and all 160 columns (about 10 s of one core)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sysm, X64, lam, lo, hi, Y_gpu):
+    """The oracle's C restatement of the filter on the same workload (same
+    grid, atoms, degree, bounds and all 160 columns), its columns shared among
+    OpenMP threads on the host's cores (OMP_NUM_THREADS, else every core).
+    Its output is also the check of the GPU run: the largest per-column
+    relative difference over all 160 columns."""
     ncol = NCOL
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     op = sysm.oracle_op()
     Xs = np.asfortranarray(X64[:, :ncol])
     t0 = time.perf_counter()
-    op.cheb_filter(Xs, sysm.veff, lam, lo, hi, DEGREE)
+    ref = op.cheb_filter(Xs, sysm.veff, lam, lo, hi, DEGREE, threads=threads)
     dt = time.perf_counter() - t0
     B = filter_bytes(sysm.nd, ncol, DEGREE, 8)
-    return {"value": B / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle.c FP64 filter, degree {DEGREE}, {ncol} of {NCOL} columns on the same 60^3 Si64 "
-                      f"grid with nonlocal projectors, one call; {dt:.2f} s"}
+    err = float((np.linalg.norm(Y_gpu - ref, axis=0) / np.linalg.norm(ref, axis=0)).max())
+    cpu = {"value": B / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+           "cpu_model": _cpu_model(),
+           "sample": f"oracle.c FP64 filter (CPU restatement, not the reference), degree {DEGREE}, all {ncol} "
+                     f"columns on the same 60^3 Si64 grid with nonlocal projectors, one call on {threads} "
+                     f"OpenMP threads; {dt:.2f} s"}
+    return cpu, err
 
 
 def main():
@@ -431,9 +447,12 @@ def main():
         print(f"[bench] hetero batch: {variants['hetero_batch']}", file=sys.stderr)
 
     cpu = None
+    parity = {"finite": r64["finite"], "max_col_rel_err_vs_oracle": None}
     if not args.no_cpu_baseline:
         lo, hi, _ = r64["lanczos"]
-        cpu = cpu_baseline(sysm, X64, 0.5 * (lo + hi), lo, hi)
+        cpu, err = cpu_baseline(sysm, X64, 0.5 * (lo + hi), lo, hi, r64["Y"])
+        parity.update(max_col_rel_err_vs_oracle=err, columns_checked=NCOL, tolerance=1e-11)
+    r64.pop("Y", None)
 
     achieved = r64["stencil_gbs"]
     ceiling = stream_ceiling(sysm)
@@ -477,6 +496,7 @@ def main():
             "frac_of_stream_ceiling": round(achieved / ceiling, 4),
         },
         "cpu_baseline": cpu,
+        "parity": parity,
         "variants": variants,
         "breakdown_ms_per_step": {"stencil": round(r64["stencil_ms_per_step"], 4),
                                   "nonlocal_gather": round(r64["gather_ms_per_step"], 4),

@@ -656,8 +656,9 @@ class Operator:
                                    ctypes.c_double(c), int(nonlocal_ and bool(self.images)))
         return y
 
-    def cheb_filter(self, x, veff, lambda_c, eig_min, eig_max, degree, nonlocal_=True):
-        """EigenSolver.cpp:558-619."""
+    def cheb_filter(self, x, veff, lambda_c, eig_min, eig_max, degree, nonlocal_=True, threads=1):
+        """EigenSolver.cpp:558-619.  threads > 1 (real blocks) shares the
+        columns among OpenMP threads; the result is the same bit for bit."""
         complex_ = np.iscomplexobj(x) or self.is_complex
         nd = self.grid.nd
         ncol = x.size // nd
@@ -672,9 +673,14 @@ class Operator:
         else:
             xx = np.asfortranarray(x, np.float64)
             y = np.zeros_like(xx)
-            lib().orc_cheb_filter_real(ctypes.byref(self.o), _p(xx), _p(y), _p(veff), ncol,
-                                       dbl(lambda_c), dbl(eig_min), dbl(eig_max), degree,
-                                       int(nonlocal_ and bool(self.images)))
+            if threads > 1:
+                lib().orc_cheb_filter_real_par(ctypes.byref(self.o), _p(xx), _p(y), _p(veff), ncol,
+                                               dbl(lambda_c), dbl(eig_min), dbl(eig_max), degree,
+                                               int(nonlocal_ and bool(self.images)), int(threads))
+            else:
+                lib().orc_cheb_filter_real(ctypes.byref(self.o), _p(xx), _p(y), _p(veff), ncol,
+                                           dbl(lambda_c), dbl(eig_min), dbl(eig_max), degree,
+                                           int(nonlocal_ and bool(self.images)))
         return y
 
     def lanczos_bounds(self, veff, tol=1e-2, max_iter=1000, seed=1, nonlocal_=True):

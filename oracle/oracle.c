@@ -371,6 +371,22 @@ void orc_cheb_filter_cplx(const orc_op *op, const zd *x, zd *y, const double *ve
     FILTER_BODY(zd, orc_h_apply_cplx)
 }
 
+/* The same filter with the columns shared among nthreads OpenMP threads,
+ * one column per call of the routine above: columns are independent in
+ * every step (halo, stencil, nonlocal, recurrence), so the result is the
+ * same, bit for bit, as one call over all columns.  The reference runs its
+ * CPU filter with OpenMP over columns and planes (Hamiltonian.cpp:145-147). */
+void orc_cheb_filter_real_par(const orc_op *op, const double *x, double *y, const double *veff,
+                              int ncol, double lambda_c, double eig_min, double eig_max,
+                              int degree, int with_nonlocal, int nthreads)
+{
+    const long nd = (long)op->nx * op->ny * op->nz;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int n = 0; n < ncol; ++n)
+        orc_cheb_filter_real(op, x + n * nd, y + n * nd, veff, 1, lambda_c, eig_min, eig_max, degree,
+                             with_nonlocal);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Lanczos bounds: Lanczos.hpp:104-198.                                     */
 /* The tridiagonal eigenvalue problem (the reference calls LAPACK dsterf)  */
