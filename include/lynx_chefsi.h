@@ -139,6 +139,22 @@ lynx_status lynx_set_kpoint(lynx_ctx *ctx, const double kpt[3]);
 lynx_status lynx_h_apply(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, void *y,
                          int ld_y, int ncol, double c, int mem);
 
+/* lynx_h_apply on device buffers without allocating: no hipMalloc, hipFree
+ * or allocating library call, for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid it).  No
+ * counterpart in the reference.  psi and y are device pointers; the work is
+ * enqueued on the context's stream and the call returns without waiting
+ * (lynx_ctx_synchronize waits).  The nonlocal projector coefficients go to a
+ * 32 MiB scratch array that each GPU holds once for the process: ncol times
+ * atoms times padded projectors (8, 16 or 32) times 8 bytes (16 complex; 4
+ * and 8 in single precision) must fit, else LYNX_E_ARG before any launch.
+ * The array is shared by every call on one GPU, from every context:
+ * overlapping calls on one GPU (two streams, two host threads) race.  The
+ * launch shape is the forced one, or the one lynx_h_apply or the filter
+ * tuned for this grid, dtype and column count, else a default shape. */
+lynx_status lynx_h_apply_noalloc(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, void *y, int ld_y,
+                                 int ncol, double c);
+
 /* ------------------------------------------------------ filter (a8) */
 
 /* Degree-m scaled Chebyshev filter Y = p_m(H) X on [lambda_c, eig_max],

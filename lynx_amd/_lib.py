@@ -43,6 +43,7 @@ SIGNATURES = {
     "lynx_set_veff": (_I, [_P, _DP, _I]),
     "lynx_set_kpoint": (_I, [_P, _DP]),
     "lynx_h_apply": (_I, [_P, _I, _P, _I, _P, _I, _I, _D, _I]),
+    "lynx_h_apply_noalloc": (_I, [_P, _I, _P, _I, _P, _I, _I, _D]),
     "lynx_cheb_filter": (_I, [_P, _I, _P, _I, _P, _I, _I, _I, _D, _D, _D, _I]),
     "lynx_orthogonalize": (_I, [_P, _I, _P, _I, _I, _I]),
     "lynx_project_hamiltonian": (_I, [_P, _I, _P, _I, _I, _P, _I]),

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -67,6 +68,31 @@ void upload(DevBuf &b, const std::vector<T> &v, hipStream_t st)
 {
     b.get(std::max<size_t>(1, v.size()) * sizeof(T));
     if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// Scratch of lynx_h_apply_noalloc: the nonlocal projector coefficients
+// beta (columns x atoms x padded projectors, compute type) of one call.  A
+// file-scope device array: every GPU the library runs on holds its own copy
+// until the process ends.  Host code never uses the array's name as an
+// address; it takes the per-device address from hipGetSymbolAddress
+// (scratch_address) and passes that.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+// Address of this device's copy of the scratch array, looked up once per
+// device with that device current (the caller has made it current).
+void *scratch_address(int device)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(LYNX_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    addr[device] = p;
+    return p;
 }
 
 }  // namespace
@@ -128,6 +154,8 @@ struct lynx_ctx {
     hipEvent_t fcoef_copied = nullptr; // the last upload from the staging buffer
     long epoch = 0;
     bool use_graphs = true;
+
+    void *scratch = nullptr;           // this device's scratch array (lynx_h_apply_noalloc)
 
     bool timing = false;
     double t_last[4] = {0, 0, 0, 0};
@@ -927,6 +955,7 @@ lynx_status lynx_ctx_create(lynx_ctx **out, int device_id)
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         BLAS_OK(rocblas_create_handle(&c->blas));
         BLAS_OK(rocblas_set_stream(c->blas, c->stream));
+        c->scratch = scratch_address(device_id);
     });
     if (s != LYNX_OK) {
         delete c;
@@ -1195,6 +1224,48 @@ lynx_status lynx_h_apply(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, 
         }
         HIP_OK(hipStreamSynchronize(ctx->stream));
         timing_end(ctx);
+    });
+}
+
+lynx_status lynx_h_apply_noalloc(lynx_ctx *ctx, int dtype, const void *psi, int ld_psi, void *y, int ld_y,
+                                 int ncol, double c)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        check_dtype(dtype);
+        need_gpu_stencil(ctx);
+        need_veff(ctx);
+        if (ncol <= 0 || !psi || !y) fail(LYNX_E_ARG, "bad block arguments");
+        if (!ctx->scratch) fail(LYNX_E_STATE, "no scratch array on this context's device");
+        const long nd = ctx->nd;
+        const long ldp = ld_psi ? ld_psi : nd, ldy = ld_y ? ld_y : nd;
+        if (ldp < nd || ldy < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+        const int e = dtype;
+        const bool nl = ctx->has_nl;
+        const gpu::NlDev &nld = nl_for(ctx, e);
+        if (nl) {
+            const size_t per_col = (size_t)nld.n_atom * nld.npm * gpu::elem_size(gpu::elem_compute(e));
+            if ((size_t)ncol * per_col > kScratchBytes)
+                fail(LYNX_E_ARG, "the projector coefficients of " + std::to_string(ncol) +
+                                     " columns exceed the 32 MiB scratch array: at most " +
+                                     std::to_string(kScratchBytes / per_col) + " columns per call");
+        }
+        // launch shape: the forced one, or the one already tuned for this
+        // grid, dtype and column count, else the default shape (tuning
+        // creates timing events, which this entry point does not)
+        gpu::StepCfg cfg{};
+        if (ctx->has_forced) {
+            cfg = ctx->forced;
+        } else {
+            const std::vector<long> key = {ctx->st.nx, ctx->st.ny, ctx->st.nz, e, ncol};
+            auto it = ctx->tuned.find(key);
+            if (it != ctx->tuned.end()) cfg = it->second;
+        }
+        if (nl) gpu::nl_gather(e, nld, psi, ldp, ctx->scratch, ncol, ctx->stream);
+        gpu::stencil_step(e, geom(ctx), psi, ldp, nullptr, 0, y, ldy, veff_for(ctx, e), ncol, 1.0, 0.0, c, cfg,
+                          ctx->stream);
+        if (nl) gpu::nl_scatter(e, nld, ctx->scratch, y, ldy, ncol, 1.0, ctx->stream);
+        HIP_OK(hipGetLastError());
     });
 }
 
