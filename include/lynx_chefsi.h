@@ -110,6 +110,19 @@ lynx_status lynx_set_nonlocal(lynx_ctx *ctx, int n_img, const int *img_atom,
                               const int *img_ndc, const int *img_nproj, const int *gpos,
                               const double *chi, const double *img_shift, int n_atom,
                               const int *ip_displ, const double *gamma);
+/* Local-pseudopotential set-up of the atoms of lynx_set_atoms, built on the
+ * host on first use: pseudocharge b = -1/(4 pi) lap V_J summed over atom
+ * images, reference charge b_ref of the smooth -Z/r potential, correction
+ * potential Vc = V_ref - V_J, the isolated-atom valence densities scaled to
+ * Nelectron, and the model core density (zero without NLCC), each Nd
+ * doubles (any may be NULL); scalars[6] = Eself, Ec, integral of b,
+ * Nelectron, has-NLCC flag, number of atom boxes.  Replaces
+ * Electrostatics::compute_pseudocharge / compute_Vloc / compute_Ec /
+ * compute_atomic_density / compute_core_density (Electrostatics.cpp:139-520)
+ * with the boxes of Crystal::compute_atom_influence (Crystal.cpp:45-134).
+ * Works on host-only contexts. */
+lynx_status lynx_get_electrostatics(lynx_ctx *ctx, double *b, double *b_ref, double *vc, double *rho_atom,
+                                    double *rho_core, double *scalars);
 lynx_status lynx_get_nonlocal_sizes(const lynx_ctx *ctx, int *n_img, long *n_gpos,
                                     long *n_chi, int *n_atom, int *n_proj);
 lynx_status lynx_get_nonlocal(const lynx_ctx *ctx, int *img_atom, int *img_ndc,

@@ -38,6 +38,7 @@ SIGNATURES = {
     "lynx_get_stencil": (_I, [_P, _DP, _DP, _DP]),
     "lynx_set_atoms": (_I, [_P, _I, ctypes.POINTER(ctypes.c_char_p), _I, _IP, _DP, _I]),
     "lynx_set_nonlocal": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP, _DP, _I, _IP, _DP]),
+    "lynx_get_electrostatics": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _DP]),
     "lynx_get_nonlocal_sizes": (_I, [_P, _IP, ctypes.POINTER(_L), ctypes.POINTER(_L), _IP, _IP]),
     "lynx_get_nonlocal": (_I, [_P, _IP, _IP, _IP, _IP, _DP, _DP, _IP, _DP]),
     "lynx_set_veff": (_I, [_P, _DP, _I]),

@@ -5,6 +5,7 @@
 #include "lynx_chefsi.h"
 
 #include "common.hpp"
+#include "electro.hpp"
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -115,6 +116,14 @@ struct lynx_ctx {
 
     bool has_nl = false;
     NonlocalHost nlh;
+    // atoms of lynx_set_atoms (psp8 data, types, lattice-aligned positions)
+    // and their local-pseudopotential set-up, built on first use
+    bool has_atoms = false;
+    std::vector<Psp8> psps;
+    std::vector<int> atom_types;
+    std::vector<double> atom_pos;
+    bool has_elec = false;
+    ElecHost elec;
     gpu::NlDev nl64, nl32;                         // views for double / float kernels
     DevBuf d_atom_pt_off, d_pt_gpos, d_pt_img, d_pt_atom, d_cov_off, d_cov_gpos, d_ent_pt;
     DevBuf d_chi64, d_chi32, d_gam64, d_gam32, d_ph64, d_ph32;
@@ -1008,6 +1017,7 @@ lynx_status lynx_set_grid(lynx_ctx *ctx, int nx, int ny, int nz, const double la
         ctx->has_veff = false;
         ctx->has_nl = false;
         ctx->nlh = NonlocalHost{};
+        ctx->has_atoms = ctx->has_elec = false;
     });
 }
 
@@ -1081,6 +1091,11 @@ lynx_status lynx_set_atoms(lynx_ctx *ctx, int ntypes, const char *const *paths, 
             }
         }
         build_nonlocal(ctx->lat, ctx->st, psps, types, pos, ctx->nlh);
+        ctx->psps = std::move(psps);
+        ctx->atom_types = std::move(types);
+        ctx->atom_pos = std::move(pos);
+        ctx->has_atoms = true;
+        ctx->has_elec = false;
         if (ctx->device >= 0) {
             upload_nonlocal(ctx);
             upload_phases(ctx);
@@ -1125,9 +1140,39 @@ lynx_status lynx_set_nonlocal(lynx_ctx *ctx, int n_img, const int *img_atom, con
         for (int g : h.gpos)
             if (g < 0 || g >= ctx->nd) fail(LYNX_E_ARG, "nonlocal grid index out of range");
         ctx->nlh = std::move(h);
+        ctx->has_atoms = ctx->has_elec = false;
         if (ctx->device >= 0) {
             upload_nonlocal(ctx);
             upload_phases(ctx);
+        }
+    });
+}
+
+lynx_status lynx_get_electrostatics(lynx_ctx *ctx, double *b, double *b_ref, double *vc, double *rho_atom,
+                                    double *rho_core, double *scalars)
+{
+    if (!ctx) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_grid(ctx);
+        if (!ctx->has_atoms) fail(LYNX_E_STATE, "lynx_set_atoms has not been called");
+        if (!ctx->has_elec) {
+            build_electrostatics(ctx->lat, ctx->st, ctx->psps, ctx->atom_types, ctx->atom_pos, ctx->elec);
+            ctx->has_elec = true;
+        }
+        const ElecHost &E = ctx->elec;
+        const size_t bytes = E.b.size() * sizeof(double);
+        if (b) std::memcpy(b, E.b.data(), bytes);
+        if (b_ref) std::memcpy(b_ref, E.b_ref.data(), bytes);
+        if (vc) std::memcpy(vc, E.vc.data(), bytes);
+        if (rho_atom) std::memcpy(rho_atom, E.rho_at.data(), bytes);
+        if (rho_core) std::memcpy(rho_core, E.rho_core.data(), bytes);
+        if (scalars) {
+            scalars[0] = E.eself;
+            scalars[1] = E.ec;
+            scalars[2] = E.int_b;
+            scalars[3] = E.nelectron;
+            scalars[4] = E.has_nlcc ? 1.0 : 0.0;
+            scalars[5] = E.n_boxes;
         }
     });
 }

@@ -54,9 +54,19 @@ struct Psp8 {
     std::vector<std::vector<std::vector<double>>> udv, udv_d;   // [l][p][i]
     std::vector<std::vector<double>> gamma;          // [l][p]
     std::vector<double> rc;                          // per l
+    // local part and densities (Pseudopotential.cpp:133-305)
+    double vloc0 = 0;                                // Vloc(r = 0) from the file
+    std::vector<double> rvloc, rvloc_d;              // r Vloc(r) and its spline derivative
+    double fchrg = 0;                                // model core charge switch
+    std::vector<double> rho_c, rho_c_d;              // core density / (4 pi), when fchrg > 0
+    std::vector<double> rho_iso, rho_iso_d;          // isolated-atom valence density / (4 pi)
     int nproj_per_atom() const;
     double rc_max() const;
 };
+// The reference's single-point Hermite evaluation (NumericalMethods.cpp:194-225):
+// the interval from the uniform-grid guess, checked, else bisection.
+double spline_eval1(const std::vector<double> &x, const std::vector<double> &y, const std::vector<double> &yd,
+                    double xi);
 void psp8_load(const std::string &path, Psp8 &out);
 void spline_deriv(const double *x, const double *y, int n, double *yd);
 void spline_eval(const double *x, const double *y, const double *yd, int n,

@@ -204,14 +204,19 @@ void psp8_load(const std::string &path, Psp8 &ps)
     ps.lloc = (int)h3[3];
     ps.mmax = (int)h3[4];
     if (ps.lmax < 0 || ps.lmax > 3 || ps.mmax < 3) fail(8, "unsupported psp8 (lmax > 3 or tiny mesh): " + path);
-    next();                                 // rchrg fchrg qchrg
+    auto h4 = next(3);                      // rchrg fchrg qchrg
+    ps.fchrg = h4.size() >= 2 ? h4[1] : 0.0;
     auto np = next(ps.lmax + 1);
     if ((int)np.size() < ps.lmax + 1) fail(6, "bad nproj line: " + path);
     ps.ppl.resize(ps.lmax + 1);
     for (int l = 0; l <= ps.lmax; ++l) ps.ppl[l] = (int)np[l];
     auto ext = next(1);
     const int extension = ext.empty() ? 0 : (int)ext[0];
-    if (extension >= 2) ++pos;              // nprojso
+    std::vector<int> nprojso(ps.lmax + 1, 0);
+    if (extension >= 2) {                   // spin-orbit projector counts for l = 1..lmax
+        auto so = next(ps.lmax + 1);
+        for (int l = 1; l <= ps.lmax && l - 1 < (int)so.size(); ++l) nprojso[l] = (int)so[l - 1];
+    }
     const int mm = ps.mmax;
     ps.r.assign(mm, 0.0);
     ps.udv.assign(ps.lmax + 1, {});
@@ -246,8 +251,49 @@ void psp8_load(const std::string &path, Psp8 &ps)
         } else if (l == ps.lloc) {
             std::vector<std::vector<double>> cols;
             read_rows(1, cols);
+            ps.vloc0 = cols[0][0];
+            ps.rvloc.resize(mm);
+            for (int i = 0; i < mm; ++i) ps.rvloc[i] = ps.r[i] * cols[0][i];
         }
     }
+    if (ps.lloc > ps.lmax) {                // the local channel follows the nonlocal ones
+        next();
+        std::vector<std::vector<double>> cols;
+        read_rows(1, cols);
+        ps.vloc0 = cols[0][0];
+        ps.rvloc.resize(mm);
+        for (int i = 0; i < mm; ++i) ps.rvloc[i] = ps.r[i] * cols[0][i];
+    }
+    if (ps.rvloc.empty()) fail(6, "psp8 file without a local potential: " + path);
+    if (ps.vloc0 == 0.0 && mm > 1) ps.vloc0 = ps.r[0] < 1e-14 ? ps.rvloc[1] / ps.r[1] : ps.rvloc[0] / ps.r[0];
+    if (extension >= 2 && ps.lmax >= 1)     // spin-orbit projectors: skipped
+        for (int l = 1; l <= ps.lmax; ++l)
+            if (nprojso[l] > 0) {
+                next();
+                for (int i = 0; i < mm; ++i) next();
+            }
+    // model core charge, then the isolated-atom valence density, each
+    // "i r value ..." rows stored as value / (4 pi)
+    auto density_rows = [&](std::vector<double> &out, bool required) {
+        out.assign(mm, 0.0);
+        for (int i = 0; i < mm; ++i) {
+            if (pos >= lines.size()) {
+                if (required) fail(6, "truncated core density: " + path);
+                if (i == 0) out.clear();
+                return;
+            }
+            auto v = numbers(lines[pos], 3);
+            if (v.size() < 3) {
+                if (required) fail(6, "short core density row: " + path);
+                if (i == 0) out.clear();
+                return;
+            }
+            ++pos;
+            out[i] = v[2] / (4.0 * M_PI);
+        }
+    };
+    if (ps.fchrg > 1e-10) density_rows(ps.rho_c, true);
+    density_rows(ps.rho_iso, false);
     ps.rc.assign(ps.lmax + 1, 0.0);
     for (int l = 0; l <= ps.lmax; ++l) {
         if (ps.ppl[l] > 0 && l != ps.lloc) {
@@ -263,6 +309,37 @@ void psp8_load(const std::string &path, Psp8 &ps)
             spline_deriv(ps.r.data(), ps.udv[l][p].data(), mm, ps.udv_d[l][p].data());
         }
     }
+    auto deriv = [&](const std::vector<double> &y, std::vector<double> &d) {
+        d.assign(y.size(), 0.0);
+        if (!y.empty()) spline_deriv(ps.r.data(), y.data(), mm, d.data());
+    };
+    deriv(ps.rvloc, ps.rvloc_d);
+    deriv(ps.rho_c, ps.rho_c_d);
+    deriv(ps.rho_iso, ps.rho_iso_d);
+}
+
+double spline_eval1(const std::vector<double> &x, const std::vector<double> &y, const std::vector<double> &yd,
+                    double xi)
+{
+    const int n = (int)x.size();
+    if (xi <= x[0]) return y[0];
+    if (xi >= x[n - 1]) return y[n - 1];
+    int j = (int)((xi - x[0]) / (x[1] - x[0]));
+    if (j >= n - 1) j = n - 2;
+    if (xi < x[j] || xi > x[j + 1]) {
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (x[mid] > xi) hi = mid; else lo = mid;
+        }
+        j = lo;
+    }
+    const double w = 1.0 / (x[j + 1] - x[j]);
+    const double s = (y[j + 1] - y[j]) * w;
+    const double a2 = w * (3.0 * s - 2.0 * yd[j] - yd[j + 1]);
+    const double a3 = w * w * (-2.0 * s + yd[j] + yd[j + 1]);
+    const double t = xi - x[j];
+    return ((a3 * t + a2) * t + yd[j]) * t + y[j];
 }
 
 // ------------------------------------------------------------------ splines

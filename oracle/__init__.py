@@ -199,6 +199,10 @@ class Psp8:
     gamma: dict = field(default_factory=dict)     # (l, p) -> ekb
     rc: list = field(default_factory=list)
     r_vloc: np.ndarray | None = None
+    vloc0: float = 0.0                             # Vloc(r = 0) as listed in the file
+    fchrg: float = 0.0                             # model core charge switch
+    rho_c: np.ndarray | None = None                # core density / (4 pi)
+    rho_iso: np.ndarray | None = None              # isolated-atom valence density / (4 pi)
 
     @property
     def nproj_per_atom(self):
@@ -233,14 +237,19 @@ def load_psp8(path):
     pspcod, pspxc, lmax, lloc, mmax = (int(t[0]), int(t[1]), int(t[2]), int(t[3]), int(t[4]))
     if pspcod != 8:
         raise ValueError("only psp8 is supported")
-    pos += 1                                    # rchrg fchrg qchrg
+    fchrg = _floats(lines[pos])[1]; pos += 1    # rchrg fchrg qchrg
     nums = lines[pos].split(); pos += 1
     ppl = [int(nums[l]) for l in range(lmax + 1)]
     ext = int(lines[pos].split()[0]); pos += 1
+    nprojso = [0] * (lmax + 1)
     if ext >= 2:
-        pos += 1                                # nprojso line
+        so = _floats(lines[pos]); pos += 1       # nprojso line (l = 1..lmax)
+        for l in range(1, lmax + 1):
+            if l - 1 < len(so):
+                nprojso[l] = int(so[l - 1])
     r = np.zeros(mmax)
     psp = Psp8(zion, pspxc, lmax, lloc, mmax, ppl, r)
+    psp.fchrg = fchrg
     have_r = False
     for l in range(lmax + 1):
         hdr = _floats(lines[pos]); pos += 1
@@ -264,6 +273,7 @@ def load_psp8(path):
             if not have_r:
                 r[:] = rows[:, 1]
             psp.r_vloc = rows[:, 1] * rows[:, 2]
+            psp.vloc0 = rows[0, 2]
             have_r = True
     if lloc > lmax:
         pos += 1
@@ -271,6 +281,33 @@ def load_psp8(path):
         if not have_r:
             r[:] = rows[:, 1]
         psp.r_vloc = rows[:, 1] * rows[:, 2]
+        psp.vloc0 = rows[0, 2]
+    if psp.vloc0 == 0.0 and mmax > 1:
+        psp.vloc0 = psp.r_vloc[1] / r[1] if r[0] < 1e-14 else psp.r_vloc[0] / r[0]
+    # spin-orbit projectors (skipped), model core charge, isolated-atom
+    # valence density: "i r value ..." rows, value / (4 pi)
+    if ext >= 2 and lmax >= 1:
+        for l in range(1, lmax + 1):
+            if nprojso[l] > 0:
+                pos += 1 + mmax
+
+    def density_block():
+        nonlocal pos
+        vals = []
+        while len(vals) < mmax and pos < len(lines):
+            f = _floats(lines[pos])
+            if len(f) < 3:
+                break
+            vals.append(f[2] / (4.0 * math.pi))
+            pos += 1
+        return np.array(vals + [0.0] * (mmax - len(vals))) if vals else None
+
+    if fchrg > 1e-10:
+        psp.rho_c = density_block()
+    psp.rho_iso = density_block()
+    psp.r_vloc_d = spline_deriv(r, psp.r_vloc)
+    psp.rho_c_d = None if psp.rho_c is None else spline_deriv(r, psp.rho_c)
+    psp.rho_iso_d = None if psp.rho_iso is None else spline_deriv(r, psp.rho_iso)
     for l in range(lmax + 1):
         if ppl[l] > 0 and l != lloc:
             u = psp.udv[(l, 0)]

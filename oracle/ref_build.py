@@ -12,7 +12,8 @@ OUT = os.path.join(HERE, "_ref")
 SOURCES = ["core/Lattice.cpp", "core/FDGrid.cpp", "core/Domain.cpp", "core/KPoints.cpp",
            "core/NumericalMethods.cpp", "operators/FDStencil.cpp", "parallel/HaloExchange.cpp",
            "atoms/Pseudopotential.cpp", "electronic/Wavefunction.cpp", "electronic/Occupation.cpp",
-           "parallel/MPIComm.cpp"]
+           "parallel/MPIComm.cpp", "atoms/AtomType.cpp", "atoms/Crystal.cpp", "core/ParameterDefaults.cpp",
+           "physics/Electrostatics.cpp"]
 # MPICH (Occupation's communicator type; no MPI call is made with null
 # communicators) and the MKL runtime (LAPACK dsterf_ for the Lanczos
 # template) from the container's conda tree

@@ -6,8 +6,9 @@
 // the rest of the program: FD stencil, lattice/grid, halo wrap, psp8 reader
 // and splines, k-points, the Lanczos bound estimate (its template, with the
 // caller's matvec and the LAPACK dsterf_ of the MKL runtime), the random
-// initial wavefunctions, and the occupations and Fermi level (Occupation
-// with null communicators, linked against MPICH without MPI_Init).
+// initial wavefunctions, the occupations and Fermi level (Occupation with
+// null communicators, linked against MPICH without MPI_Init), and the
+// local-pseudopotential set-up (Crystal's atom boxes and Electrostatics).
 #include "core/Domain.hpp"
 #include "core/FDGrid.hpp"
 #include "core/KPoints.hpp"
@@ -19,6 +20,9 @@
 #include "solvers/Lanczos.hpp"
 #include "electronic/Occupation.hpp"
 #include "electronic/Wavefunction.hpp"
+#include "atoms/AtomType.hpp"
+#include "atoms/Crystal.hpp"
+#include "physics/Electrostatics.hpp"
 
 #include <complex>
 #include <cstring>
@@ -226,6 +230,82 @@ double ref_occupation(int nspin, int nkpt, int nband, const double *eigs, const 
         for (int k = 0; k < nkpt; ++k)
             for (int n = 0; n < nband; ++n) occ[((long)s * nkpt + k) * nband + n] = w.occupations(s, k)(n);
     return ef;
+}
+
+// Crystal::compute_atom_influence + Electrostatics (pseudocharge, Vc, Ec,
+// atomic and core densities) for natom atoms at fractional coordinates of
+// ntype psp8 files (type-major), with the rc_max of Crystal::setup
+// (Crystal.cpp:327-336).  out_* hold nd doubles; scalars[5] = Eself, Ec,
+// int_b, number of boxes, has-NLCC.
+int ref_electrostatics(int nx, int ny, int nz, const double *lv, int orth, int order, int ntype,
+                       const char *const *paths, int natom, const int *types, const double *frac, double *out_b,
+                       double *out_bref, double *out_vc, double *out_rho, double *out_core, double *scalars)
+{
+    try {
+        Lattice lat = make_lattice(lv, orth);
+        FDGrid g(nx, ny, nz, lat, BCType::Periodic, BCType::Periodic, BCType::Periodic);
+        DomainVertices v{0, nx - 1, 0, ny - 1, 0, nz - 1};
+        Domain d(g, v);
+        FDStencil st(order, g, lat);
+        std::vector<AtomType> at;
+        std::vector<Vec3> pos;
+        std::vector<int> ti;
+        int nelec = 0;
+        for (int t = 0; t < ntype; ++t) {
+            int n = 0;
+            for (int i = 0; i < natom; ++i) n += types[i] == t;
+            Pseudopotential tmp;
+            tmp.load_psp8(paths[t]);
+            AtomType a("X", 1.0, tmp.Zval(), n);
+            a.psd().load_psp8(paths[t]);
+            for (int i = 0; i < natom; ++i) {
+                if (types[i] != t) continue;
+                Vec3 f(frac[3 * i], frac[3 * i + 1], frac[3 * i + 2]);
+                Vec3 p;
+                if (orth) {
+                    p = lat.frac_to_cart(f);
+                } else {
+                    Vec3 L = lat.lengths();
+                    p = Vec3(f.x * L.x, f.y * L.y, f.z * L.z);
+                }
+                pos.push_back(p);
+                ti.push_back(t);
+            }
+            nelec += (int)tmp.Zval() * n;
+            at.push_back(std::move(a));
+        }
+        Crystal cr(std::move(at), pos, ti, lat);
+        double rc_max = 0.0;
+        for (int t = 0; t < cr.n_types(); ++t) {
+            const auto &ps = cr.types()[t].psd();
+            for (auto rc : ps.rc()) rc_max = std::max(rc_max, rc);
+            if (!ps.radial_grid().empty()) rc_max = std::max(rc_max, ps.radial_grid().back());
+        }
+        rc_max += 8.0 * std::max({g.dx(), g.dy(), g.dz()});
+        std::vector<AtomInfluence> inf;
+        cr.compute_atom_influence(d, rc_max, inf);
+        Electrostatics el;
+        el.compute_pseudocharge(cr, inf, d, g, st);
+        const int nd = nx * ny * nz;
+        std::vector<double> vloc(nd, 0.0);
+        el.compute_Vloc(cr, inf, d, g, vloc.data());
+        el.compute_Ec(vloc.data(), nd, g.dV());
+        el.compute_atomic_density(cr, inf, d, g, out_rho, nelec);
+        const bool nlcc = el.compute_core_density(cr, inf, d, g, out_core);
+        std::memcpy(out_b, el.pseudocharge().data(), nd * sizeof(double));
+        std::memcpy(out_bref, el.pseudocharge_ref().data(), nd * sizeof(double));
+        std::memcpy(out_vc, vloc.data(), nd * sizeof(double));
+        int nbox = 0;
+        for (auto &x : inf) nbox += x.n_atom;
+        scalars[0] = el.Eself();
+        scalars[1] = el.Ec();
+        scalars[2] = el.int_b();
+        scalars[3] = nbox;
+        scalars[4] = nlcc ? 1.0 : 0.0;
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
 }
 
 }  // extern "C"

@@ -215,3 +215,63 @@ def test_occupation_matches_reference(ref, kind, nspin, nkpt):
                                      ctypes.byref(lef)))
     assert abs(lef.value - ref_ef) < 1e-12
     assert np.abs(occ - rocc).max() < 1e-10
+
+
+@pytest.mark.parametrize("case", ["si8_orth", "si2_fcc", "becu_two_species"])
+def test_electrostatics_matches_reference(ref, case):
+    # Crystal::compute_atom_influence + Electrostatics (pseudocharge, b_ref,
+    # Vc, Eself, Ec, atomic density; Electrostatics.cpp:139-463) compiled
+    # from source, against the oracle and the library's host set-up
+    import oracle.scf as osc
+    from lynx_amd import _lib as L
+    from systems import DIAMOND
+    becu = "/root/reference/test_suite/systems/BeCu_bulk"
+    if case == "si8_orth":
+        n, lv, orth, files, types, frac = (16, 16, 18), np.diag([A_SI, A_SI, 1.1 * A_SI]), 1, [psp("C.pot")], \
+            [0] * 8, DIAMOND
+    elif case == "si2_fcc":
+        n, lv, orth, files, types, frac = (15, 15, 15), A_SI * np.array([[0, .5, .5], [.5, 0, .5], [.5, .5, 0]]), 0, \
+            [psp("C.pot")], [0, 0], np.array([[0, 0, 0], [.25, .25, .25]])
+    else:
+        n, lv, orth, files, types, frac = (21, 21, 21), np.diag([5.080050872061664] * 3), 1, \
+            [os.path.join(becu, "Be.pot"), os.path.join(becu, "Cu.pot")], [0, 1], np.array([[0, 0, 0], [.5, .5, .5]])
+        if not os.path.isdir(becu):
+            pytest.skip("reference test_suite absent")
+    nd = int(np.prod(n))
+    lv = np.ascontiguousarray(lv, float)
+    fr = np.ascontiguousarray(frac, float)
+    ty = np.ascontiguousarray(types, np.int32)
+    arr = (ctypes.c_char_p * len(files))(*[f.encode() for f in files])
+    out = [np.zeros(nd) for _ in range(5)]
+    sc = np.zeros(5)
+    assert ref.ref_electrostatics(n[0], n[1], n[2], P(lv), orth, 12, len(files), arr, len(ty), P(ty), P(fr),
+                                  *[P(o) for o in out], P(sc)) == 0
+    rb, rbr, rvc, rrho, rcore = out
+    lat = oracle.Lattice(lv, bool(orth))
+    g = oracle.Grid(*n, lat)
+    E = osc.Electrostatics(g, [oracle.load_psp8(f) for f in files], list(types), frac)
+    scale = np.abs(rb).max()
+    assert len(E.boxes) == int(sc[3])
+    assert np.abs(E.b - rb).max() < 1e-12 * scale and np.abs(E.b_ref - rbr).max() < 1e-12 * np.abs(rbr).max()
+    assert np.abs(E.vc - rvc).max() < 1e-12 * np.abs(rvc).max()
+    assert np.abs(E.rho_at - rrho).max() < 1e-12 * np.abs(rrho).max()
+    assert abs(E.eself - sc[0]) < 1e-10 * abs(sc[0]) and abs(E.ec - sc[1]) < 1e-10 * abs(sc[1])
+    assert abs(E.int_b - sc[2]) < 1e-9
+    # the pseudocharge holds the ionic charge (closed form: the flux of
+    # -Z/r through the box boundary)
+    assert abs(E.int_b + E.nelectron) < 1e-6
+    # the library's host set-up on a host-only context
+    ctx = ctypes.c_void_p()
+    L.check(L.lib().lynx_ctx_create(ctypes.byref(ctx), -1))
+    try:
+        L.check(L.lib().lynx_set_grid(ctx, n[0], n[1], n[2], L.dptr(lv), 0 if orth else 1, 12), ctx)
+        L.check(L.lib().lynx_set_atoms(ctx, len(files), arr, len(ty), L.iptr(ty), L.dptr(fr), 1), ctx)
+        lo = [np.zeros(nd) for _ in range(5)]
+        ls = np.zeros(6)
+        L.check(L.lib().lynx_get_electrostatics(ctx, *[L.dptr(o) for o in lo], L.dptr(ls)), ctx)
+        for mine, theirs in zip(lo, out):
+            assert np.abs(mine - theirs).max() <= 1e-12 * max(np.abs(theirs).max(), 1e-300)
+        assert abs(ls[0] - sc[0]) < 1e-10 * abs(sc[0]) and abs(ls[1] - sc[1]) < 1e-10 * abs(sc[1])
+        assert int(ls[5]) == int(sc[3]) and int(ls[3]) == E.nelectron
+    finally:
+        L.lib().lynx_ctx_destroy(ctx)
