@@ -365,9 +365,87 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *driver, int scf_iter, double scf_res
  * (Occupation.cpp:86-176).  occ (may be NULL) receives f(e - Ef). */
 lynx_status lynx_fermi_level(int nspin, int nkpt, int nband, const double *eig, const double *kpt_weights,
                              double nelectron, double beta, int smearing, double *occ, double *fermi);
+/* lynx_scf_eig_step with Veff in host (LYNX_MEM_HOST) or device
+ * (LYNX_MEM_DEVICE) memory. */
+lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *driver, int scf_iter, double scf_residual, const double *veff,
+                                   int veff_mem, void *const *psi, int ld, int mem, double *eig, double *occ,
+                                   double *fermi, int *precision_used);
 /* Current lambda_cutoff and per-spin eig_min / eig_max (nspin each). */
 lynx_status lynx_scf_eig_state(const lynx_scf_eig *driver, double *lambda_cutoff, double *eig_min,
                                double *eig_max);
+
+/* ---- SCF ------------------------------------------------------------------
+ * The self-consistent field loop around the CheFSI eigen stage, on the
+ * GPU in FP64: electron density from the orbital blocks, exchange-
+ * correlation, the Poisson solve for the electrostatic potential, Pulay
+ * mixing of the density with the Kerker preconditioner, and the total
+ * energy.  Replaces SCF::run (SCF.cpp:110-291) with initialize_scf
+ * (SCF.cpp:937-1028), compute_new_density, compute_scf_energy,
+ * check_convergence and mix_and_update (SCF.cpp:469-791);
+ * ElectronDensity::compute (ElectronDensity.cpp:55-140);
+ * EffectivePotential::compute (EffectivePotential.cpp:44-176);
+ * PoissonSolver::solve (PoissonSolver.cpp:43-166); Mixer::mix
+ * (Mixer.cpp:116-266) with KerkerPreconditioner::apply
+ * (KerkerPreconditioner.cpp:37-95); Energy::compute_all (Energy.cpp:140-200).
+ * The context needs lynx_set_grid and lynx_set_atoms (psp8 files); the
+ * driver owns the orbitals (device), drawn as Wavefunction::randomize does
+ * with seed spin * 1000 + 1, and the density starts from the isolated-atom
+ * densities.  Density mixing only; collinear spin; LDA (PZ81 or PW92) or
+ * GGA PBE, written from the published forms (the reference calls libxc). */
+enum lynx_xc { LYNX_XC_LDA_PZ = 0, LYNX_XC_LDA_PW = 1, LYNX_XC_GGA_PBE = 2 };
+typedef struct lynx_scf lynx_scf;
+typedef struct {
+    int nspin;                    /* 1 or 2 */
+    int nkpt;                     /* k-points (kpts and weights given at create) */
+    int complex_kpt;              /* 1: complex Bloch blocks; 0: real (Gamma point, nkpt 1) */
+    int nband;
+    int cheb_degree;
+    int rho_trigger;              /* CheFSI passes in the first iteration (4 in the reference) */
+    int nchefsi;                  /* passes per later iteration (1) */
+    double beta;                  /* 1 / (k_B T) in 1/Ha */
+    int smearing;                 /* 0 Fermi-Dirac, 1 Gaussian */
+    int xc;                       /* LYNX_XC_* */
+    double tol;                   /* density residual ||rho_out - rho_in|| / ||rho_out|| */
+    int max_iter, min_iter;       /* 100 and 2 in the reference */
+    double poisson_tol;           /* <= 0: tol / 100 (ParameterDefaults.cpp:41-43) */
+    double precond_tol;           /* <= 0: h_eff^2 1e-3 (ParameterDefaults.cpp:45-47) */
+    int mixing_history;           /* Pulay history (7) */
+    double mixing_param;          /* 0.3 */
+    int kerker;                   /* 1: Kerker preconditioner (the reference's default), 0: none */
+    double fp64_switch_residual;  /* as lynx_scf_eig_params: reduced-precision passes above it */
+    int low_precision;            /* LYNX_FILTER_FP32 or LYNX_FILTER_BF16 */
+} lynx_scf_params;
+typedef struct {
+    double etotal, eband, exc, ehart, eself, ec, entropy;   /* Ha (Energy::compute_all) */
+    double fermi;                 /* Ha */
+    double scf_error;             /* last density residual */
+    int iterations;
+    int converged;
+    int natom;
+    double seconds;               /* host wall time of lynx_scf_run */
+} lynx_scf_result;
+lynx_status lynx_scf_create(lynx_scf **scf, lynx_ctx *ctx, const lynx_scf_params *params, const double *kpts,
+                            const double *kpt_weights);
+lynx_status lynx_scf_destroy(lynx_scf *scf);
+const char *lynx_scf_last_error(const lynx_scf *scf);
+/* Runs the SCF loop from the atomic densities and random orbitals to
+ * convergence (or max_iter). */
+lynx_status lynx_scf_run(lynx_scf *scf, lynx_scf_result *result);
+/* Per-iteration record of the last run (capacity cap): total energy, density
+ * residual, wall seconds, filter precision used; *n the iteration count. */
+lynx_status lynx_scf_history(const lynx_scf *scf, int cap, double *etotal, double *error, double *seconds,
+                             int *precision, int *n);
+/* Eigenvalues and occupations (nspin x nkpt x nband), the input density of
+ * the last iteration (nspin x Nd: total for one spin, up and down for two)
+ * and Veff (nspin x Nd); any pointer may be NULL. */
+lynx_status lynx_scf_get(const lynx_scf *scf, double *eig, double *occ, double *rho, double *veff);
+/* The electrostatic potential phi and the XC energy density of the last
+ * potential build (Nd each); any pointer may be NULL. */
+lynx_status lynx_scf_get_fields(const lynx_scf *scf, double *phi, double *exc, double *vxc);
+/* One potential build from a given density (nspin x Nd as lynx_scf_get),
+ * without the eigen stage: XC, Poisson (warm start from the previous phi)
+ * and Veff; for testing the SCF tail alone. */
+lynx_status lynx_scf_build_potential(lynx_scf *scf, const double *rho);
 
 /* ---- Batched filter ------------------------------------------------------
  * A batch filters many independent units together: unit u applies the

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "liblynx_chefsi.so")
 OK, E_ARG, E_HIP, E_CHOLESKY, E_EIG, E_NONFINITE, E_IO, E_STATE, E_UNSUPPORTED = range(9)
 F64, C128, F32, C64, BF16 = 0, 1, 2, 3, 4
 FILTER_FP64, FILTER_FP32, FILTER_BF16 = 0, 1, 2
+XC_LDA_PZ, XC_LDA_PW, XC_GGA_PBE = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 CELL_ORTHOGONAL, CELL_NONORTHOGONAL = 0, 1
 
@@ -69,7 +70,16 @@ SIGNATURES = {
     "lynx_scf_eig_destroy": (_I, [_P]),
     "lynx_scf_eig_last_error": (ctypes.c_char_p, [_P]),
     "lynx_scf_eig_step": (_I, [_P, _I, _D, _DP, ctypes.POINTER(_P), _I, _I, _DP, _DP, _DP, _IP]),
+    "lynx_scf_eig_step_veff": (_I, [_P, _I, _D, _DP, _I, ctypes.POINTER(_P), _I, _I, _DP, _DP, _DP, _IP]),
     "lynx_scf_eig_state": (_I, [_P, _DP, _DP, _DP]),
+    "lynx_scf_create": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _DP]),
+    "lynx_scf_destroy": (_I, [_P]),
+    "lynx_scf_last_error": (ctypes.c_char_p, [_P]),
+    "lynx_scf_run": (_I, [_P, _P]),
+    "lynx_scf_history": (_I, [_P, _I, _DP, _DP, _DP, _IP, _IP]),
+    "lynx_scf_get": (_I, [_P, _DP, _DP, _DP, _DP]),
+    "lynx_scf_get_fields": (_I, [_P, _DP, _DP, _DP]),
+    "lynx_scf_build_potential": (_I, [_P, _DP]),
     "lynx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), _DP]),
     "lynx_batch_destroy": (_I, [_P]),
     "lynx_batch_last_error": (ctypes.c_char_p, [_P]),

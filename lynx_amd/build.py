@@ -26,6 +26,8 @@ SOURCES = [
     ("scf_eig.cpp", "host"),
     ("capi.cpp", "hip"),
     ("kernels.hip", "hip"),
+    ("kernels_scf.hip", "hip"),
+    ("scf.cpp", "hip"),
 ]
 
 

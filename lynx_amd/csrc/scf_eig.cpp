@@ -49,8 +49,63 @@ double smear(double x, double beta, int type)
     return 1.0 / (1.0 + std::exp(a));
 }
 
-// Fermi level: the root of sum_i w_i f(e_i - Ef) = Ne, bracketed as the
-// reference does and found by bisection to the reference's 1e-14 tolerance.
+// Brent's root finder (Brent 1973, in the form of NumericalMethods.cpp:14-78):
+// inverse quadratic interpolation or secant steps kept inside the bracket,
+// bisection otherwise.  Which point of a flat stretch it returns (the
+// Fermi level of a gapped spectrum) depends on these steps, so they are
+// kept as the reference takes them.
+template <class F>
+double brent(F &&f, double a, double b, double tol, int max_iter)
+{
+    double fa = f(a), fb = f(b);
+    double c = a, fc = fa, d = b - a, e = d;
+    for (int it = 0; it < max_iter; ++it) {
+        if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) {
+            c = a; fc = fa;
+            d = e = b - a;
+        }
+        if (std::fabs(fc) < std::fabs(fb)) {
+            a = b; b = c; c = a;
+            fa = fb; fb = fc; fc = fa;
+        }
+        const double tol1 = 2.0 * 2.220446049250313e-16 * std::fabs(b) + 0.5 * tol;
+        const double xm = 0.5 * (c - b);
+        if (std::fabs(xm) <= tol1 || fb == 0.0) return b;
+        if (std::fabs(e) >= tol1 && std::fabs(fa) > std::fabs(fb)) {
+            const double s = fb / fa;
+            double p, q;
+            if (a == c) {
+                p = 2.0 * xm * s;
+                q = 1.0 - s;
+            } else {
+                const double qq = fa / fc, r = fb / fc;
+                p = s * (2.0 * xm * qq * (qq - r) - (b - a) * (r - 1.0));
+                q = (qq - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0) q = -q;
+            p = std::fabs(p);
+            if (2.0 * p < std::min(3.0 * xm * q - std::fabs(tol1 * q), std::fabs(e * q))) {
+                e = d;
+                d = p / q;
+            } else {
+                d = xm;
+                e = d;
+            }
+        } else {
+            d = xm;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        b += std::fabs(d) > tol1 ? d : (xm >= 0 ? tol1 : -tol1);
+        fb = f(b);
+    }
+    return b;
+}
+
+// Fermi level: the root of sum_i w_i f(e_i - Ef) = Ne on the reference's
+// bracket (10/beta beyond the extreme levels, else 100/beta), by Brent to
+// 1e-14 in at most 200 steps (Occupation.cpp:53-84).
 double fermi_level(const std::vector<double> &e, const std::vector<double> &w, double ne, double beta, int type)
 {
     auto occ = [&](double ef) {
@@ -61,26 +116,12 @@ double fermi_level(const std::vector<double> &e, const std::vector<double> &w, d
     const double emin = *std::min_element(e.begin(), e.end());
     const double emax = *std::max_element(e.begin(), e.end());
     double a = emin - 10.0 / beta, b = emax + 10.0 / beta;
-    double fa = occ(a), fb = occ(b);
-    if (fa * fb > 0) {
+    if (occ(a) * occ(b) > 0) {
         a = emin - 100.0 / beta;
         b = emax + 100.0 / beta;
-        fa = occ(a);
-        fb = occ(b);
-        if (fa * fb > 0) throw Fail{LYNX_E_ARG, "cannot bracket the Fermi level"};
+        if (occ(a) * occ(b) > 0) throw Fail{LYNX_E_ARG, "cannot bracket the Fermi level"};
     }
-    for (int it = 0; it < 400 && b - a > 1e-14; ++it) {
-        const double m = 0.5 * (a + b);
-        const double fm = occ(m);
-        if (fm == 0.0) return m;
-        if ((fm < 0) == (fa < 0)) {
-            a = m;
-            fa = fm;
-        } else {
-            b = m;
-        }
-    }
-    return 0.5 * (a + b);
+    return brent(occ, a, b, 1e-14, 200);
 }
 
 }  // namespace
@@ -124,6 +165,14 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
                               void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
                               int *precision_used)
 {
+    return lynx_scf_eig_step_veff(d, scf_iter, scf_residual, veff, LYNX_MEM_HOST, psi, ld, mem, eig, occ, fermi,
+                                  precision_used);
+}
+
+lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_residual, const double *veff,
+                                   int veff_mem, void *const *psi, int ld, int mem, double *eig, double *occ,
+                                   double *fermi, int *precision_used)
+{
     if (!d || !veff || !psi || !eig) return LYNX_E_ARG;
     // the context's filter precision is the caller's setting: the step sets
     // its own and restores the caller's on every return
@@ -141,7 +190,7 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
         // spectral bounds once, per spin, at the first k-point (SCF.cpp:912-929)
         if (!d->have_bounds) {
             for (int s = 0; s < p.nspin; ++s) {
-                ok(lynx_set_veff(d->ctx, veff + s * d->nd, LYNX_MEM_HOST), d, "lynx_set_veff");
+                ok(lynx_set_veff(d->ctx, veff + s * d->nd, veff_mem), d, "lynx_set_veff");
                 if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[0]), d, "lynx_set_kpoint");
                 ok(lynx_lanczos_bounds(d->ctx, dtype, &d->eig_min[s], &d->eig_max[s], 1e-2, 1000, 1, nullptr), d,
                    "lynx_lanczos_bounds");
@@ -157,7 +206,7 @@ lynx_status lynx_scf_eig_step(lynx_scf_eig *d, int scf_iter, double scf_residual
         const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
         for (int pass = 0; pass < passes; ++pass) {
             for (int s = 0; s < p.nspin; ++s) {
-                ok(lynx_set_veff(d->ctx, veff + s * d->nd, LYNX_MEM_HOST), d, "lynx_set_veff");
+                ok(lynx_set_veff(d->ctx, veff + s * d->nd, veff_mem), d, "lynx_set_veff");
                 for (int k = 0; k < p.nkpt; ++k) {
                     if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[3 * k]), d, "lynx_set_kpoint");
                     ok(lynx_chefsi_solve(d->ctx, dtype, psi[s * p.nkpt + k], ld, N, eig + (long)(s * p.nkpt + k) * N,

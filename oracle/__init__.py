@@ -720,11 +720,14 @@ class Operator:
                                            int(nonlocal_ and bool(self.images)))
         return y
 
-    def lanczos_bounds(self, veff, tol=1e-2, max_iter=1000, seed=1, nonlocal_=True):
-        """Lanczos.hpp:104-198 (returns eig_min-0.1, eig_max*1.01, iterations)."""
+    def lanczos_bounds(self, veff, tol=1e-2, max_iter=1000, seed=1, nonlocal_=True, complex_=None):
+        """Lanczos.hpp:104-198 (returns eig_min-0.1, eig_max*1.01, iterations).
+        complex_: the complex (k-point) form, also at k = 0; default: a
+        non-zero k-point."""
         veff = np.ascontiguousarray(veff, np.float64)
         mn = ctypes.c_double(); mx = ctypes.c_double(); it = ctypes.c_int()
-        fn = lib().orc_lanczos_cplx if self.is_complex else lib().orc_lanczos_real
+        cplx = self.is_complex if complex_ is None else complex_
+        fn = lib().orc_lanczos_cplx if cplx else lib().orc_lanczos_real
         fn(ctypes.byref(self.o), _p(veff), ctypes.c_double(tol), max_iter, ctypes.c_uint(seed),
            int(nonlocal_ and bool(self.images)), ctypes.byref(mn), ctypes.byref(mx), ctypes.byref(it))
         return mn.value, mx.value, it.value
@@ -777,26 +780,72 @@ def setup_system(grid: Grid, psps, types, frac, order=12, kpt=(0.0, 0.0, 0.0)):
 # SCF eigen stage (SCF.cpp:310-467, 883-935; Occupation.cpp:11-176)
 
 def smearing(x, beta, kind=0):
-    """Occupation.cpp:11-35: Fermi-Dirac (0) or 0.5 erfc (1, Gaussian)."""
-    from scipy.special import erfc
+    """Occupation.cpp:11-35: Fermi-Dirac (0) or 0.5 erfc (1, Gaussian).
+
+    erfc is the C library's (Python's math.erfc calls it): in a gapped
+    spectrum the Fermi-level root is flat and the last bits of the tails
+    decide where the root finder stops, so the oracle uses the same erfc
+    as the compiled code it is checked against."""
+    import math
     x = np.asarray(x, float)
     if kind == 1:
-        return 0.5 * erfc(beta * x)
+        return 0.5 * np.vectorize(math.erfc, otypes=[float])(beta * x)
     a = x * beta
     return np.where(a > 50.0, 0.0, np.where(a < -50.0, 1.0, 1.0 / (1.0 + np.exp(np.clip(a, -50, 50)))))
 
 
+def brent(f, a, b, tol=1e-14, max_iter=200):
+    """Brent's root finder as NumericalMethods.cpp:14-78 takes its steps."""
+    fa, fb = f(a), f(b)
+    c, fc, d = a, fa, b - a
+    e = d
+    eps = 2.220446049250313e-16
+    for _ in range(max_iter):
+        if (fb > 0 and fc > 0) or (fb < 0 and fc < 0):
+            c, fc = a, fa
+            d = e = b - a
+        if abs(fc) < abs(fb):
+            a, b, c = b, c, b
+            fa, fb, fc = fb, fc, fb
+        tol1 = 2.0 * eps * abs(b) + 0.5 * tol
+        xm = 0.5 * (c - b)
+        if abs(xm) <= tol1 or fb == 0.0:
+            return b
+        if abs(e) >= tol1 and abs(fa) > abs(fb):
+            s = fb / fa
+            if a == c:
+                p, q = 2.0 * xm * s, 1.0 - s
+            else:
+                qq, r = fa / fc, fb / fc
+                p = s * (2.0 * xm * qq * (qq - r) - (b - a) * (r - 1.0))
+                q = (qq - 1.0) * (r - 1.0) * (s - 1.0)
+            if p > 0:
+                q = -q
+            p = abs(p)
+            if 2.0 * p < min(3.0 * xm * q - abs(tol1 * q), abs(e * q)):
+                e, d = d, p / q
+            else:
+                d = xm; e = d
+        else:
+            d = xm; e = d
+        a, fa = b, fb
+        b += d if abs(d) > tol1 else (tol1 if xm >= 0 else -tol1)
+        fb = f(b)
+    return b
+
+
 def fermi_level(eigs, weights, nelectron, beta, kind=0):
-    """Occupation.cpp:53-87: root of sum w f(e - Ef) = Ne on the bracket
-    [emin - 10/beta, emax + 10/beta] (scipy's brentq, xtol 1e-14)."""
-    from scipy.optimize import brentq
+    """Occupation.cpp:53-84: root of sum w f(e - Ef) = Ne on the bracket
+    [emin - 10/beta, emax + 10/beta] (else 100/beta), Brent to 1e-14."""
     e = np.asarray(eigs, float)
     w = np.asarray(weights, float)
-    g = lambda ef: float((w * smearing(e - ef, beta, kind)).sum() - nelectron)  # noqa: E731
+    # summed in order, as Occupation.cpp:36-46 does: near a flat root the
+    # rounding of the sum is part of where the root finder stops
+    g = lambda ef: float(np.cumsum(w * smearing(e - ef, beta, kind))[-1] - nelectron)  # noqa: E731
     a, b = e.min() - 10.0 / beta, e.max() + 10.0 / beta
     if g(a) * g(b) > 0:
         a, b = e.min() - 100.0 / beta, e.max() + 100.0 / beta
-    return brentq(g, a, b, xtol=1e-14, rtol=4 * np.finfo(float).eps, maxiter=400)
+    return brent(g, a, b)
 
 
 class ScfEigStage:
@@ -828,7 +877,8 @@ class ScfEigStage:
             for s in range(self.nspin):
                 if self.kpts[0] is not None:
                     op.set_kpoint(self.kpts[0])
-                self.eig_min[s], self.eig_max[s], _ = op.lanczos_bounds(veff[s])
+                self.eig_min[s], self.eig_max[s], _ = op.lanczos_bounds(veff[s],
+                                                                        complex_=self.kpts[0] is not None)
             self.lambda_c = 0.5 * (self.eig_min[0] + self.eig_max[0])
             self.bounds = True
         # SCF.cpp:323: rho_trigger passes in the first iteration

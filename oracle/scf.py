@@ -188,3 +188,453 @@ class Electrostatics:
         self.rho_at = rho_at * (self.nelectron / tot) if tot > 1e-10 else rho_at
         self.rho_core = rho_core.ravel()
         self.has_nlcc = any(p.fchrg > 1e-10 for p in psps)
+
+
+# ---------------------------------------------------------------------------
+# exchange-correlation (energy densities rho * eps; derivatives by the
+# complex step, an independent differentiation of the same energies)
+
+_PI = math.pi
+
+
+def _cbrt(x):
+    return x ** (1.0 / 3.0) if np.iscomplexobj(x) else np.cbrt(x)
+
+
+def fx_slater(rho):
+    """Slater exchange, unpolarized: rho eps_x = -(3/4)(3/pi)^(1/3) rho^(4/3)."""
+    return -0.75 * (3.0 / _PI) ** (1.0 / 3.0) * rho * _cbrt(rho)
+
+
+def ec_pz(rs, pol):
+    """Perdew & Zunger, PRB 23, 5048 (1981), Table XII."""
+    g, b1, b2 = (-0.0843, 1.3981, 0.2611) if pol else (-0.1423, 1.0529, 0.3334)
+    A, B, C, D = (0.01555, -0.0269, 0.0007, -0.0048) if pol else (0.0311, -0.048, 0.0020, -0.0116)
+    hi = g / (1.0 + b1 * np.sqrt(rs) + b2 * rs)
+    lr = np.log(rs)
+    lo = A * lr + B + C * rs * lr + D * rs
+    return np.where(np.real(rs) >= 1.0, hi, lo)
+
+
+PW92 = {False: ((0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294),
+                (0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517),
+                (0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671), 1.709921),
+        True: ((0.0310907, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294),
+               (0.01554535, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517),
+               (0.0168869, 0.11125, 10.357, 3.6231, 0.88026, 0.49671), 1.709920934161365617563962776245)}
+
+
+def pw_g(rs, par):
+    """Perdew & Wang, PRB 45, 13244 (1992), eq. 10 with p = 1."""
+    A, a1, b1, b2, b3, b4 = par
+    s = np.sqrt(rs)
+    return -2.0 * A * (1.0 + a1 * rs) * np.log(1.0 + 1.0 / (2.0 * A * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs)))
+
+
+def fzeta(z):
+    return ((1 + z) * _cbrt(1 + z) + (1 - z) * _cbrt(1 - z) - 2.0) / (2.0 * 2.0 ** (1.0 / 3.0) - 2.0)
+
+
+def ec_unif(rs, zeta, kind, mod=False):
+    """Correlation per particle: PZ81 (kind 0) or PW92 (kind 1; mod: the
+    more precise A values PBE uses); zeta None for unpolarized."""
+    if kind == 0:
+        eu = ec_pz(rs, False)
+        return eu if zeta is None else eu + fzeta(zeta) * (ec_pz(rs, True) - eu)
+    p0, p1, p2, fz20 = PW92[mod]
+    e0 = pw_g(rs, p0)
+    if zeta is None:
+        return e0
+    e1 = pw_g(rs, p1)
+    ac = -pw_g(rs, p2)
+    f = fzeta(zeta)
+    z4 = zeta ** 4
+    return e0 + ac * f * (1 - z4) / fz20 + (e1 - e0) * f * z4
+
+
+KAPPA, BETA_PBE = 0.8040, 0.06672455060314922
+MU_PBE = BETA_PBE * _PI * _PI / 3.0
+GAMMA_PBE = (1.0 - math.log(2.0)) / (_PI * _PI)
+
+
+def fx_pbe(rho, sigma):
+    """PBE exchange of an unpolarized density, PRL 77, 3865 (1996) eqs. 13-14."""
+    kf = _cbrt(3.0 * _PI * _PI * rho)
+    s2 = sigma / (4.0 * kf * kf * rho * rho)
+    return fx_slater(rho) * (1.0 + KAPPA - KAPPA / (1.0 + MU_PBE * s2 / KAPPA))
+
+
+def fc_pbe(rho, sigma, zeta=None):
+    """PBE correlation, eqs. 3, 7, 8, with PW92 (precise A values) for the
+    uniform gas."""
+    rs = _cbrt(3.0 / (4.0 * _PI * rho))
+    ec = ec_unif(rs, zeta, 1, mod=True)
+    phi = 1.0 if zeta is None else 0.5 * (_cbrt(1 + zeta) ** 2 + _cbrt(1 - zeta) ** 2)
+    kf = _cbrt(3.0 * _PI * _PI * rho)
+    ks2 = 4.0 * kf / _PI
+    t2 = sigma / (4.0 * phi * phi * ks2 * rho * rho)
+    A = (BETA_PBE / GAMMA_PBE) / (np.exp(-ec / (GAMMA_PBE * phi ** 3)) - 1.0)
+    At2 = A * t2
+    H = GAMMA_PBE * phi ** 3 * np.log(1.0 + (BETA_PBE / GAMMA_PBE) * t2 * (1 + At2) / (1 + At2 + At2 * At2))
+    return rho * (ec + H)
+
+
+def _f_unpol(kind, rho, sigma):
+    if kind == 2:
+        return fx_pbe(rho, sigma) + fc_pbe(rho, sigma)
+    return fx_slater(rho) + rho * ec_unif(_cbrt(3.0 / (4.0 * _PI * rho)), None, kind)
+
+
+def _f_pol(kind, ru, rd, suu, sud, sdd):
+    r = ru + rd
+    z = (ru - rd) / r
+    if kind == 2:
+        fx = 0.5 * fx_pbe(2 * ru, 4 * suu) + 0.5 * fx_pbe(2 * rd, 4 * sdd)
+        return fx + fc_pbe(r, suu + 2 * sud + sdd, z)
+    return 0.5 * fx_slater(2 * ru) + 0.5 * fx_slater(2 * rd) + r * ec_unif(_cbrt(3.0 / (4.0 * _PI * r)), z, kind)
+
+
+def _cstep(f, args, k):
+    """d f / d args[k] by the complex step."""
+    h = 1e-20
+    a = [np.asarray(x, complex) for x in args]
+    scale = np.maximum(1.0, np.abs(a[k].real))
+    a[k] = a[k] + 1j * h * scale
+    return f(*a).imag / (h * scale)
+
+
+RHO_MIN = 1e-15
+
+
+def xc_unpol(kind, rho, sigma=None):
+    """exc (per particle), vrho, v2 = 2 d(rho eps)/d sigma (0 for LDA)."""
+    rho = np.asarray(rho, float)
+    sig = np.zeros_like(rho) if sigma is None else np.maximum(np.asarray(sigma, float), 1e-40)
+    ok = rho > RHO_MIN
+    r, s = rho[ok], sig[ok]
+    f = lambda a, b: _f_unpol(kind, a, b)  # noqa: E731
+    exc = np.zeros_like(rho); vr = np.zeros_like(rho); v2 = np.zeros_like(rho)
+    exc[ok] = f(r, s).real / r
+    vr[ok] = _cstep(f, [r, s], 0)
+    if kind == 2:
+        v2[ok] = 2.0 * _cstep(f, [r, s], 1)
+    return exc, vr, v2
+
+
+def xc_pol(kind, ru, rd, suu=None, sud=None, sdd=None):
+    """exc (per particle of the total), v_up, v_dn and (PBE) vsigma_uu,
+    vsigma_ud, vsigma_dd."""
+    ru = np.asarray(ru, float); rd = np.asarray(rd, float)
+    z = np.zeros_like(ru)
+    suu = z if suu is None else np.maximum(np.asarray(suu, float), 1e-40)
+    sdd = z if sdd is None else np.maximum(np.asarray(sdd, float), 1e-40)
+    sud = z if sud is None else np.asarray(sud, float)
+    ok = ru + rd > RHO_MIN
+    args = [x[ok] for x in (ru, rd, suu, sud, sdd)]
+    f = lambda *a: _f_pol(kind, *a)  # noqa: E731
+    out = [np.zeros_like(ru) for _ in range(6)]
+    out[0][ok] = f(*args).real / (args[0] + args[1])
+    for k in range(5 if kind == 2 else 2):
+        out[1 + k][ok] = _cstep(f, args, k)
+    return tuple(out)
+
+
+# ---------------------------------------------------------------------------
+# linear solves, potential, mixing, density, energy, SCF loop
+
+AAR_M, AAR_P, AAR_OMEGA, AAR_BETA = 7, 6, 0.6, 0.6
+
+
+def gauss_solve(A, b):
+    """NumericalMethods.cpp:84-120 (partial pivoting, vanishing pivots skipped)."""
+    A = np.array(A, float); b = np.array(b, float)
+    n = len(b)
+    for k in range(n):
+        p = k + int(np.argmax(np.abs(A[k:, k])))
+        if p != k:
+            A[[k, p]] = A[[p, k]]; b[[k, p]] = b[[p, k]]
+        d = A[k, k]
+        if abs(d) < 1e-14:
+            continue
+        for i in range(k + 1, n):
+            f = A[i, k] / d
+            A[i, k + 1:] -= f * A[k, k + 1:]
+            b[i] -= f * b[k]
+    for k in range(n - 1, -1, -1):
+        if abs(A[k, k]) < 1e-14:
+            continue
+        b[k] -= A[k, k + 1:] @ b[k + 1:]
+        b[k] /= A[k, k]
+    return b
+
+
+def aar(apply_A, b, x, tol, m_inv, max_iter):
+    """LinearSolver.cpp:23-130 / PoissonSolver.cpp:43-130: Richardson steps
+    with weight omega, an Anderson step every p-th iteration over the last
+    m differences, the residual norm checked only after Anderson steps."""
+    x = np.array(x, float)
+    n = len(b)
+    X = np.zeros((AAR_M, n)); F = np.zeros((AAR_M, n))
+    xo = x.copy(); fo = np.zeros(n)
+    abs_tol = tol * math.sqrt(float(b @ b))
+    r = b - apply_A(x)
+    rnorm = abs_tol + 1.0
+    it = 0
+    while rnorm > abs_tol and it < max_iter:
+        f = m_inv * r
+        if it > 0:
+            h = (it - 1) % AAR_M
+            X[h] = x - xo; F[h] = f - fo
+        xo = x.copy(); fo = f.copy()
+        if (it + 1) % AAR_P == 0 and it > 0:
+            cols = min(it, AAR_M)
+            FtF = F[:cols] @ F[:cols].T
+            g = gauss_solve(FtF, F[:cols] @ f)
+            x = xo + AAR_BETA * f - g @ (X[:cols] + AAR_BETA * F[:cols])
+            r = b - apply_A(x)
+            rnorm = math.sqrt(float(r @ r))
+        else:
+            x = xo + AAR_OMEGA * f
+            r = b - apply_A(x)
+        it += 1
+    return x, it
+
+
+class ScfOracle:
+    """SCF::run (SCF.cpp:110-291) in numpy around the oracle eigen stage:
+    density mixing with Pulay + Kerker, collinear spin, LDA (PZ, PW) or PBE."""
+
+    KTF, IDIEMAC = 1.0, 0.1
+
+    def __init__(self, grid, psps, types, frac, nband, beta, xc=2, nspin=1, kpts=None, kpt_weights=None,
+                 smearing=0, degree=20, rho_trigger=4, nchefsi=1, tol=1e-6, max_iter=100, min_iter=2,
+                 mixing_history=7, mixing_param=0.3, kerker=True, order=12):
+        from . import ScfEigStage, setup_system
+        self.grid = grid
+        self.st = stencil_coeffs(order, grid)
+        self.op = setup_system(grid, psps, list(types), frac, order)
+        self.lap_op = Operator(grid, order)
+        self.E = Electrostatics(grid, psps, types, frac, order)
+        self.nband, self.beta, self.xc, self.nspin = nband, beta, xc, nspin
+        self.kpts = [None] if kpts is None else [tuple(k) for k in kpts]
+        nk = len(self.kpts)
+        self.kw = np.full(nk, 1.0 / nk) if kpt_weights is None else np.asarray(kpt_weights, float)
+        self.smearing, self.tol, self.max_iter, self.min_iter = smearing, tol, max_iter, min_iter
+        self.m, self.amix, self.kerker = mixing_history, mixing_param, kerker
+        self.stage = ScfEigStage(self.op, nband, self.E.nelectron, beta, None if kpts is None else self.kpts,
+                                 self.kw, degree, rho_trigger, nchefsi, smearing, nspin)
+        h = grid.h
+        heff = h[0] if (abs(h[0] - h[1]) < 1e-12 and abs(h[1] - h[2]) < 1e-12) else \
+            math.sqrt(3.0 / sum(1.0 / (x * x) for x in h))
+        self.poisson_tol = tol * 0.01
+        self.precond_tol = heff * heff * 1e-3
+        self.phi = np.zeros(grid.nd)
+        self.diag = float(self.st.D2[0][0] + self.st.D2[1][0] + self.st.D2[2][0])
+        lat = grid.lattice
+        self.M = np.eye(3) if lat.orthogonal else lat.lapc_T
+
+    # -- operators ---------------------------------------------------------
+    def _A(self, x, k2):
+        """-lap x + k2 x through the oracle's stencil (Veff 0, no projectors)."""
+        z = np.zeros(self.grid.nd)
+        return 2.0 * self.lap_op.apply(x.reshape(-1, 1), z, c=0.5 * k2, nonlocal_=False).ravel()
+
+    def _grad(self, x, a):
+        g = self.grid
+        X = x.reshape(g.nz, g.ny, g.nx)
+        ax = {0: 2, 1: 1, 2: 0}[a]
+        d1 = self.st.D1[a]
+        out = np.zeros_like(X)
+        for p in range(1, self.st.fdn + 1):
+            out += d1[p] * (np.roll(X, -p, axis=ax) - np.roll(X, p, axis=ax))
+        return out.ravel()
+
+    def poisson(self, rho_total):
+        """EffectivePotential.cpp:44-72."""
+        rhs = 4.0 * math.pi * (rho_total + self.E.b)
+        rhs = rhs - rhs.sum() / len(rhs)
+        m_inv = 1.0 if abs(self.diag) < 1e-14 else -1.0 / self.diag
+        self.phi, self.poisson_iters = aar(lambda x: self._A(x, 0.0), rhs, self.phi, self.poisson_tol, m_inv, 3000)
+        self.phi = self.phi - self.phi.sum() / len(self.phi)
+        return self.phi
+
+    def potential(self, rho):
+        """EffectivePotential::compute; rho is (nspin == 1: total) or
+        (total, up, down)."""
+        nd = self.grid.nd
+        core = self.E.rho_core if self.E.has_nlcc else None
+        gga = self.xc == 2
+        M = self.M
+        if self.nspin == 1:
+            r = rho[0] if core is None else np.maximum(rho[0] + core, 1e-14)
+            if gga:
+                g = [self._grad(r, a) for a in range(3)]
+                Mg = [sum(M[i][j] * g[j] for j in range(3)) for i in range(3)]
+                sigma = sum(Mg[i] * g[i] for i in range(3))
+                exc, v, v2 = xc_unpol(2, r, sigma)
+                for a in range(3):
+                    v = v - self._grad(v2 * Mg[a], a)
+            else:
+                exc, v, _ = xc_unpol(self.xc, r)
+            vxc = [v]
+        else:
+            ru, rd = rho[1].copy(), rho[2].copy()
+            if core is not None:
+                ru = ru + 0.5 * core; rd = rd + 0.5 * core
+            ru = np.maximum(ru, 0.5e-14); rd = np.maximum(rd, 0.5e-14)
+            if gga:
+                gu = [self._grad(ru, a) for a in range(3)]
+                gd = [self._grad(rd, a) for a in range(3)]
+                Mgu = [sum(M[i][j] * gu[j] for j in range(3)) for i in range(3)]
+                Mgd = [sum(M[i][j] * gd[j] for j in range(3)) for i in range(3)]
+                suu = sum(Mgu[i] * gu[i] for i in range(3))
+                sud = sum(Mgu[i] * gd[i] for i in range(3))
+                sdd = sum(Mgd[i] * gd[i] for i in range(3))
+                exc, vu, vd, vsuu, vsud, vsdd = xc_pol(2, ru, rd, suu, sud, sdd)
+                for a in range(3):
+                    vu = vu - self._grad(2 * vsuu * Mgu[a] + vsud * Mgd[a], a)
+                    vd = vd - self._grad(vsud * Mgu[a] + 2 * vsdd * Mgd[a], a)
+            else:
+                exc, vu, vd = xc_pol(self.xc, ru, rd)[:3]
+            vxc = [vu, vd]
+        phi = self.poisson(rho[0])
+        self.exc, self.vxc = exc, vxc
+        self.veff = np.concatenate([v + phi for v in vxc])
+        return self.veff
+
+    def kerker_apply(self, r):
+        """KerkerPreconditioner.cpp:37-95."""
+        k2 = self.KTF ** 2
+        Lf = -self._A(r, self.IDIEMAC * k2)         # (lap - idiemac kTF^2) r
+        md = self.diag - k2
+        m_inv = 1.0 if abs(md) < 1e-14 else -1.0 / md
+        z, _ = aar(lambda x: self._A(x, k2), Lf, np.zeros_like(r), self.precond_tol, m_inv, 1000)
+        return -self.amix * z
+
+    def mix(self, x, g, ncol):
+        """Mixer.cpp:116-266 (density mixing) with the density constraint
+        (Mixer.cpp:70-114)."""
+        if not hasattr(self, "_R"):
+            self._it = 0
+            N = len(x)
+            self._fk = np.zeros(N); self._xkm1 = np.zeros(N)
+            self._R = np.zeros((self.m, N)); self._F = np.zeros((self.m, N))
+        fkm1 = self._fk.copy()
+        fk = g - x
+        if self._it > 0:
+            h = (self._it - 1) % self.m
+            self._R[h] = x - self._xkm1
+            self._F[h] = fk - fkm1
+        self._fk = fk
+        if self._it > 0:
+            cols = min(self._it, self.m)
+            F = self._F[:cols]
+            gm = gauss_solve(F @ F.T, F @ fk)
+            xw = x - gm @ self._R[:cols]
+            fw = fk - gm @ F
+        else:
+            xw, fw = x.copy(), fk.copy()
+        nd = self.grid.nd
+        if self.kerker:
+            Pf = np.concatenate([self.kerker_apply(fw[:nd])] + [self.amix * fw[c * nd:(c + 1) * nd]
+                                                                  for c in range(1, ncol)])
+        else:
+            Pf = self.amix * fw
+        self._xkm1 = x.copy()
+        x = xw + Pf
+        if ncol == 2:
+            up = np.maximum(0.5 * (x[:nd] + x[nd:]), 0.0)
+            dn = np.maximum(0.5 * (x[:nd] - x[nd:]), 0.0)
+            x = np.concatenate([up + dn, up - dn])
+        else:
+            x = np.maximum(x, 0.0)
+        ne = x[:nd].sum() * self.grid.dV
+        if ne > 1e-10:
+            x = x * (self.E.nelectron / ne)
+        self._it += 1
+        return x
+
+    def density(self, psis, occ):
+        """ElectronDensity.cpp:55-140."""
+        nd, nk, N = self.grid.nd, len(self.kpts), self.nband
+        spin_fac = 2.0 if self.nspin == 1 else 1.0
+        rs = []
+        for s in range(self.nspin):
+            r = np.zeros(nd)
+            for k in range(nk):
+                f = occ[(s * nk + k) * N:(s * nk + k + 1) * N]
+                w = np.where(f < 1e-16, 0.0, spin_fac * self.kw[k] * f)
+                r += (np.abs(psis[s * nk + k]) ** 2) @ w
+            rs.append(r)
+        return [rs[0]] if self.nspin == 1 else [rs[0] + rs[1], rs[0], rs[1]]
+
+    def initial_psis(self):
+        """Wavefunction.cpp:53-76 with seed spin * 1000 + 1 (SCF.cpp:863-881)."""
+        from . import rand_uniform
+        nd, N = self.grid.nd, self.nband
+        out = []
+        for s in range(self.nspin):
+            for k in self.kpts:
+                if k is None:
+                    v = rand_uniform(s * 1000 + 1, -0.5, 0.5, nd * N)
+                    out.append(np.asfortranarray(v.reshape(N, nd).T))
+                else:
+                    v = rand_uniform(s * 1000 + 1, -0.5, 0.5, 2 * nd * N)
+                    c = v[0::2] + 1j * v[1::2]
+                    out.append(np.asfortranarray(c.reshape(N, nd).T))
+        return out
+
+    def run(self, callback=None):
+        nd = self.grid.nd
+        dV = self.grid.dV
+        E = self.E
+        rho = [E.rho_at.copy()] if self.nspin == 1 else [E.rho_at.copy(), 0.5 * E.rho_at, 0.5 * E.rho_at]
+        veff = self.potential(rho)
+        psis = self.initial_psis()
+        err = 1.0
+        spin_fac = 2.0 if self.nspin == 1 else 1.0
+        self.history = []
+        for it in range(self.max_iter):
+            psis, eig = self.stage.step(it, veff, psis)
+            occ = self.stage.occ
+            new = self.density(psis, occ)
+            # Energy.cpp:140-200 with the input density's fields
+            w = self.stage.weights
+            eband = float((w * occ * eig).sum())
+            if self.smearing == 0:
+                f = occ
+                m = (f > 1e-14) & (f < 1 - 1e-14)
+                S = float((w[m] * -(f[m] * np.log(f[m]) + (1 - f[m]) * np.log(1 - f[m]))).sum())
+            else:
+                x = self.beta * (eig - self.stage.fermi)
+                S = float((w * (0.5 / math.sqrt(math.pi)) * np.exp(-x * x)).sum())
+            entropy = -S / self.beta
+            rt = rho[0]
+            core = E.rho_core if E.has_nlcc else 0.0
+            exc = float(((rt + core) * self.exc).sum() * dV)
+            if self.nspin == 1:
+                e2 = float((rt * self.vxc[0]).sum() * dV)
+            else:
+                e2 = float((rho[1] * self.vxc[0] + rho[2] * self.vxc[1]).sum() * dV)
+            e3 = float((rt * self.phi).sum() * dV)
+            ehart = 0.5 * float(((rt + E.b) * self.phi).sum() * dV)
+            etot = eband - e2 + exc - e3 + ehart + E.eself + E.ec + entropy
+            err = float(np.linalg.norm(new[0] - rt) / np.linalg.norm(new[0]))
+            rec = dict(etotal=etot, eband=eband, exc=exc, ehart=ehart, entropy=entropy, error=err,
+                       fermi=self.stage.fermi, eself=E.eself, ec=E.ec)
+            self.history.append(rec)
+            if callback:
+                callback(it, rec)
+            if it >= self.min_iter and err < self.tol:
+                rec["converged"] = True
+                return rec
+            if self.nspin == 1:
+                rho = [self.mix(rho[0], new[0], 1)]
+            else:
+                x = self.mix(np.concatenate([rho[0], rho[1] - rho[2]]), np.concatenate([new[0], new[1] - new[2]]), 2)
+                t, mg = x[:nd], x[nd:]
+                rho = [t, 0.5 * (t + mg), 0.5 * (t - mg)]
+            veff = self.potential(rho)
+        rec = dict(self.history[-1])
+        rec["converged"] = False
+        return rec

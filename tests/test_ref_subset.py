@@ -188,15 +188,27 @@ def test_randomize_matches_reference(ref, cplx):
     assert np.array_equal(stream, out)
 
 
+@pytest.mark.parametrize("gapped", [False, True])
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("nspin,nkpt", [(1, 1), (2, 3)])
-def test_occupation_matches_reference(ref, kind, nspin, nkpt):
+def test_occupation_matches_reference(ref, kind, nspin, nkpt, gapped):
     # Occupation::compute (Occupation.cpp:86-176) against the oracle's Fermi
-    # level and the library's host lynx_fermi_level
+    # level and the library's host lynx_fermi_level.  In a gapped spectrum
+    # (15 electrons filling the lowest levels exactly, a 0.3 Ha gap above)
+    # the root is flat across the gap and the point returned is fixed only
+    # by the root finder's own steps.
     from lynx_amd import _lib as L
     nb = 13
     rng = np.random.default_rng(11 + kind + nspin)
-    eig = np.sort(rng.uniform(-0.4, 0.3, (nspin, nkpt, nb)), axis=2).ravel()
+    eig = np.sort(rng.uniform(-0.4, 0.3, (nspin, nkpt, nb)), axis=2)
+    if gapped:
+        eig = np.sort(rng.uniform(-0.4, -0.1, (nspin, nkpt, nb)), axis=2)
+        if nspin == 1:
+            eig[:, :, 7:] += 0.3
+        else:
+            eig[0, :, 8:] += 0.3
+            eig[1, :, 7:] += 0.3
+    eig = eig.ravel()
     kw = rng.uniform(0.5, 1.5, nkpt)
     kw /= kw.sum()
     ne, beta = 15.0, 1.0 / (3.166811563e-6 * 3000.0)
