@@ -14,6 +14,13 @@ One step = one degree-20 filter call over the whole block, inputs resident in
 HBM.  value = B_filter / t_step with B_filter = s*P*(3m-1) + 8*Nd*m, the
 algorithmic bytes of the fused recurrence (P = Nd*ncol point-columns,
 s = bytes per element, m = degree).
+
+Beside the headline line (under "variants"): the same filter in FP32; the
+SCF wall-time of the full FP64 SCF on Si8 (30^3, PBE, 24 states) with the
+oracle's CPU restatement timed beside it; Si216 (90^3, 480 states) eigen
+stage and whole SCF in FP64, FP32-until-1e-4 and BF16-storage-until-1e-4
+with their energy and eigenvalue errors against the FP64 run; the 112
+k-point complex64 batch and the 64-frame heterogeneous batch.
 """
 from __future__ import annotations
 
@@ -351,6 +358,113 @@ def run_mixed_si216():
     return res
 
 
+KB_HA = 3.1668115634556e-6   # Boltzmann constant, Ha/K
+
+
+def _scf_history_summary(S):
+    h = S.history()
+    prec = [int(v) for v in h["precision"]]
+    return {"iterations_per_precision": {str(p): prec.count(p) for p in sorted(set(prec))},
+            "s_per_iteration": [round(float(t), 4) for t in h["seconds"]]}
+
+
+def run_scf_si8(with_cpu):
+    """Full FP64 SCF on the GPU: Si8 diamond cell (a = 10.26 Bohr, C.pot for
+    Si), 30^3 grid, FD order 12, GGA-PBE, Fermi-Dirac 1000 K, 24 states,
+    degree 20, orbitals from the glibc rand() stream, to a 1e-6 density
+    residual.  One untimed run tunes the launch shapes; the timed run starts
+    from the atomic density again.  With with_cpu the oracle's restatement of
+    the same SCF (its filter in C, the rest numpy; one core) is timed beside
+    it and its converged energy is the check of the GPU run."""
+    from systems import si8
+    from lynx_amd.scf import SCF
+    from lynx_amd.hipmem import synchronize
+    s = si8(30)
+    H = s.hamiltonian(0)
+    beta = 1.0 / (KB_HA * 1000.0)
+    kw = dict(xc="GGA_PBE", tol=1e-6)
+    warm = SCF(H, 24, beta, **kw)
+    warm.run()
+    warm.close()
+    S = SCF(H, 24, beta, **kw)
+    synchronize()
+    t0 = time.perf_counter()
+    r = S.run()
+    synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": "Si8 a=10.26 Bohr, 30^3, FD12, GGA-PBE, Fermi-Dirac 1000 K, 24 states, degree 20, "
+                       "SCF to 1e-6 density residual; C.pot stands in for Si",
+           "scf_wall_s": round(wall, 4), "iterations": r["iterations"], "converged": bool(r["converged"]),
+           "etotal_Ha": r["etotal"], "etotal_per_atom_Ha": r["etotal"] / 8}
+    out.update(_scf_history_summary(S))
+    S.close()
+    if with_cpu:
+        import oracle
+        import oracle.scf as osc
+        og = oracle.Grid(30, 30, 30, oracle.Lattice(np.diag([s.a] * 3), True))
+        O = osc.ScfOracle(og, [oracle.load_psp8(f) for f in s.psp_files], [0] * 8, s.frac, 24, beta, xc=2,
+                          tol=1e-6)
+        t0 = time.perf_counter()
+        ref = O.run()
+        dt = time.perf_counter() - t0
+        out["cpu_restatement"] = {"scf_wall_s": round(dt, 3), "iterations": len(O.history), "cores": 1,
+                                  "kind": "port", "cpu_model": _cpu_model(),
+                                  "sample": "oracle SCF (CPU restatement, not the reference) on the same "
+                                            "input, one core"}
+        out["etotal_err_per_atom_vs_oracle_Ha"] = abs(r["etotal"] - ref["etotal"]) / 8
+        out["speedup_vs_cpu_restatement"] = round(dt / wall, 2)
+    return out
+
+
+def run_scf_si216():
+    """SCF wall-time on Si216 (the 3x3x3 supercell, a = 30.78 Bohr,
+    sigma = 0.03 Bohr seed 3, 90^3 grid, 480 states, GGA-PBE at Gamma,
+    Fermi-Dirac 1000 K, degree 20, to a 1e-6 density residual) in three
+    modes: FP64 throughout; FP32 passes (filter, Gram products, rotation)
+    until the residual falls below 1e-4, FP64 after; the same with the
+    filter's blocks stored in bfloat16.  Each mode first runs one untimed
+    iteration so the launch shapes it uses are tuned.  Errors are against
+    the FP64 run: total energy per atom and occupied eigenvalues."""
+    from systems import si216_mixed
+    from lynx_amd.scf import SCF
+    from lynx_amd.hipmem import synchronize
+    s = si216_mixed()
+    H = s.hamiltonian(0)
+    beta = 1.0 / (KB_HA * 1000.0)
+    modes = (("fp64", 0.0, "fp32"), ("fp32_until_1e-4", 1e-4, "fp32"), ("bf16_until_1e-4", 1e-4, "bf16"))
+    out, res = {}, {}
+    for name, thr, low in modes:
+        kw = dict(xc="GGA_PBE", tol=1e-6, fp64_switch_residual=thr, low_precision=low)
+        warm = SCF(H, 480, beta, max_iter=1, **kw)
+        warm.run()
+        warm.close()
+        S = SCF(H, 480, beta, **kw)
+        synchronize()
+        t0 = time.perf_counter()
+        r = S.run()
+        synchronize()
+        wall = time.perf_counter() - t0
+        eig, occ, _, _ = S.state()
+        res[name] = (r, eig, occ)
+        out[name] = {"scf_wall_s": round(wall, 3), "iterations": r["iterations"], "converged": bool(r["converged"]),
+                     "etotal_per_atom_Ha": r["etotal"] / 216}
+        out[name].update(_scf_history_summary(S))
+        S.close()
+        print(f"[bench] Si216 SCF {name}: {wall:.2f} s, {r['iterations']} iterations, "
+              f"E/atom {r['etotal'] / 216:.10f}", file=sys.stderr, flush=True)
+    r64, e64, o64 = res["fp64"]
+    occ_mask = o64 > 1e-6
+    summary = {"workload": "Si216 3x3x3 supercell a=30.78 Bohr, sigma=0.03 (seed 3), 90^3, GGA-PBE, Gamma, "
+                           "480 states, Fermi-Dirac 1000 K, degree 20, SCF to 1e-6 density residual; C.pot "
+                           "stands in for Si", "modes": out}
+    for name in ("fp32_until_1e-4", "bf16_until_1e-4"):
+        r, e, _ = res[name]
+        summary[name] = {"speedup_scf_wall": round(out["fp64"]["scf_wall_s"] / out[name]["scf_wall_s"], 3),
+                         "etotal_err_per_atom_Ha": abs(r["etotal"] - r64["etotal"]) / 216,
+                         "max_abs_occupied_eig_err_Ha": float(np.abs(e - e64)[occ_mask].max())}
+    return summary
+
+
 def stream_ceiling(sysm):
     """Practical HBM ceiling of the stencil step's traffic on this device
     (lynx_stream_ceiling: x, Y_(k-2) and y streams of the same 60^3 x 160
@@ -405,7 +519,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batches", action="store_true", help="skip the k-point and heterogeneous batch legs")
-    ap.add_argument("--no-si216", action="store_true", help="skip the Si216 mixed-precision eigen-stage leg")
+    ap.add_argument("--no-si216", action="store_true", help="skip the Si216 mixed-precision legs")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time legs (Si8 and Si216)")
     ap.add_argument("--stencil-shape", default=None, metavar="VEC,THREADS,DIV,ZCHUNKS",
                     help="force the FP64 stencil launch shape instead of the autotuned one (e.g. 1,512,2,1)")
     args = ap.parse_args()
@@ -436,9 +551,15 @@ def main():
                            "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
                            "launch_shape": r32["stencil_config"]}
 
+    if not args.no_scf:
+        variants["scf_si8"] = run_scf_si8(not args.no_cpu_baseline)
+        print(f"[bench] Si8 SCF: {variants['scf_si8']}", file=sys.stderr, flush=True)
     if not args.no_si216:
         variants["si216_mixed_precision"] = run_mixed_si216()
         print(f"[bench] Si216 mixed: {variants['si216_mixed_precision']}", file=sys.stderr)
+        if not args.no_scf:
+            variants["scf_si216"] = run_scf_si216()
+            print(f"[bench] Si216 SCF: {variants['scf_si216']}", file=sys.stderr, flush=True)
 
     if not args.no_batches:
         variants["c64_kpoint_batch"] = run_kpoint_batch(args.steps, args.warmup)

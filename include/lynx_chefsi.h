@@ -393,6 +393,17 @@ lynx_status lynx_scf_eig_state(const lynx_scf_eig *driver, double *lambda_cutoff
  * densities.  Density mixing only; collinear spin; LDA (PZ81 or PW92) or
  * GGA PBE, written from the published forms (the reference calls libxc). */
 enum lynx_xc { LYNX_XC_LDA_PZ = 0, LYNX_XC_LDA_PW = 1, LYNX_XC_GGA_PBE = 2 };
+/* The two constant-coefficient periodic solves of the SCF tail, Poisson
+ * (-lap phi = 4 pi (rho + b)) and Kerker ((-lap + kTF^2) z = f), by
+ *   LYNX_SOLVE_SPECTRAL: the FD operator is diagonal in the discrete Fourier
+ *     basis of the periodic grid, so each solve is one real-to-complex FFT,
+ *     a division by the stencil's own symbol and one inverse FFT (rocFFT):
+ *     exact to rounding, for orthogonal and non-orthogonal cells;
+ *   LYNX_SOLVE_AAR: the reference's alternating Anderson-Richardson
+ *     iteration with a Jacobi preconditioner (PoissonSolver.cpp:43-166,
+ *     KerkerPreconditioner.cpp:37-95) to poisson_tol / precond_tol.
+ * Both reach the same discrete solution; AAR stops at its tolerance. */
+enum lynx_linear_solver { LYNX_SOLVE_SPECTRAL = 0, LYNX_SOLVE_AAR = 1 };
 typedef struct lynx_scf lynx_scf;
 typedef struct {
     int nspin;                    /* 1 or 2 */
@@ -414,6 +425,7 @@ typedef struct {
     int kerker;                   /* 1: Kerker preconditioner (the reference's default), 0: none */
     double fp64_switch_residual;  /* as lynx_scf_eig_params: reduced-precision passes above it */
     int low_precision;            /* LYNX_FILTER_FP32 or LYNX_FILTER_BF16 */
+    int linear_solver;            /* LYNX_SOLVE_SPECTRAL (0) or LYNX_SOLVE_AAR (1) */
 } lynx_scf_params;
 typedef struct {
     double etotal, eband, exc, ehart, eself, ec, entropy;   /* Ha (Energy::compute_all) */

@@ -17,6 +17,7 @@ OK, E_ARG, E_HIP, E_CHOLESKY, E_EIG, E_NONFINITE, E_IO, E_STATE, E_UNSUPPORTED =
 F64, C128, F32, C64, BF16 = 0, 1, 2, 3, 4
 FILTER_FP64, FILTER_FP32, FILTER_BF16 = 0, 1, 2
 XC_LDA_PZ, XC_LDA_PW, XC_GGA_PBE = 0, 1, 2
+SOLVE_SPECTRAL, SOLVE_AAR = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 CELL_ORTHOGONAL, CELL_NONORTHOGONAL = 0, 1
 

@@ -390,6 +390,19 @@ __global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, do
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         y[i] = a * x[i] + b * y[i];
 }
+__global__ void k_spectral_scale(double2 *__restrict__ spec, const double *__restrict__ sym, long n, int kind,
+                                 double amix, double c1, double c2, double norm)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double L = sym[i];
+        const double g = kind == 0 ? (i == 0 ? 0.0 : norm / L) : norm * amix * (L + c1) / (L + c2);
+        double2 v = spec[i];
+        v.x *= g;
+        v.y *= g;
+        spec[i] = v;
+    }
+}
+
 __global__ void k_fill(double *__restrict__ y, double v, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = v;
@@ -624,6 +637,14 @@ void scf_axpby(double *y, const double *x, double a, double b, long n, hipStream
 {
     k_axpby<<<blocks_for(n), kNT, 0, st>>>(y, x, a, b, n);
     check_launch("k_axpby");
+}
+
+void scf_spectral_scale(double *spec, const double *sym, long n, int kind, double amix, double c1, double c2,
+                        double norm, hipStream_t st)
+{
+    k_spectral_scale<<<blocks_for(n), kNT, 0, st>>>(reinterpret_cast<double2 *>(spec), sym, n, kind, amix, c1, c2,
+                                                    norm);
+    check_launch("k_spectral_scale");
 }
 
 void scf_fill(double *y, double v, long n, hipStream_t st)

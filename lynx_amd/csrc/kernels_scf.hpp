@@ -41,6 +41,11 @@ int scf_clamp_density(double *x, int ncol, long nd, double *part, hipStream_t st
 void scf_gga_flux(const double M[9], const double *c1, const double *const g[3], const double *c2,
                   const double *const h[3], double *const f[3], long n, hipStream_t st);
 void scf_sigma(const double M[9], const double *const a[3], const double *const b[3], double *s, long n, hipStream_t st);
+// multiply the n interleaved complex Fourier coefficients spec by a factor
+// of the symbol L of -lap at each: kind 0 (Poisson), norm / L with the
+// q = 0 coefficient set to zero; kind 1 (Kerker), norm amix (L + c1) / (L + c2)
+void scf_spectral_scale(double *spec, const double *sym, long n, int kind, double amix, double c1, double c2,
+                        double norm, hipStream_t st);
 
 }  // namespace gpu
 }  // namespace lynx

@@ -6,14 +6,25 @@
 // (LinearSolver.cpp:23-130), KerkerPreconditioner::apply
 // (KerkerPreconditioner.cpp:37-95), Mixer::mix (Mixer.cpp:116-266) and
 // Energy::compute_all (Energy.cpp:140-200).  Every Nd-sized step runs in the
-// kernels of kernels_scf.hip or the fused stencil; the host solves the
-// Anderson / Pulay normal equations (at most 7 x 7) and sums partials.
+// kernels of kernels_scf.hip, the fused stencil or rocFFT; the host solves
+// the Anderson / Pulay normal equations (at most 7 x 7) and sums partials.
+//
+// The Poisson and Kerker equations have constant coefficients on a periodic
+// grid, so besides the reference's AAR iteration they are solved exactly in
+// the Fourier basis: the FD operator's symbol at wavevector theta is
+//   L(theta) = -sum_a (D2_a[0] + 2 sum_p D2_a[p] cos(p theta_a))
+//              + 4 sum_{(a,b)} (sum_p D2m_ab[p] sin(p theta_a)) (sum_q D1_b[q] sin(q theta_b))
+// (mixed pairs xy, xz, yz of a non-orthogonal cell), the same operator the
+// stencil kernels apply, evaluated from the same coefficients.
 #include "ctx.hpp"
 #include "kernels_scf.hpp"
+
+#include <rocfft/rocfft.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +65,113 @@ void gauss_solve(std::vector<double> A, std::vector<double> &b, int n)
     }
 }
 
+// symbol of -lap on the half-spectrum (nx/2+1, ny, nz) layout of a
+// real-to-complex transform, x fastest
+std::vector<double> fd_symbol(const Stencil &st)
+{
+    const int nx = st.nx, ny = st.ny, nz = st.nz, nxh = nx / 2 + 1, R = st.fdn;
+    auto cosine = [&](const std::vector<double> &c, int n, int m) {
+        double v = c[0];
+        for (int p = 1; p <= R; ++p) v += 2.0 * c[p] * std::cos(2.0 * kPi * p * m / n);
+        return v;
+    };
+    auto sine = [&](const std::vector<double> &c, int n, int m) {
+        double v = 0.0;
+        for (int p = 1; p <= R; ++p) v += c[p] * std::sin(2.0 * kPi * p * m / n);
+        return v;
+    };
+    std::vector<double> cx(nxh), sxy(nxh), sxz(nxh), cy(ny), dy(ny), syz(ny), cz(nz), dz(nz);
+    for (int i = 0; i < nxh; ++i) {
+        cx[i] = cosine(st.D2[0], nx, i);
+        sxy[i] = sine(st.D2m[0], nx, i);
+        sxz[i] = sine(st.D2m[1], nx, i);
+    }
+    for (int j = 0; j < ny; ++j) {
+        cy[j] = cosine(st.D2[1], ny, j);
+        dy[j] = sine(st.D1[1], ny, j);
+        syz[j] = sine(st.D2m[2], ny, j);
+    }
+    for (int k = 0; k < nz; ++k) {
+        cz[k] = cosine(st.D2[2], nz, k);
+        dz[k] = sine(st.D1[2], nz, k);
+    }
+    std::vector<double> L((size_t)nxh * ny * nz);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nxh; ++i)
+                L[i + (size_t)nxh * (j + (size_t)ny * k)] =
+                    -(cx[i] + cy[j] + cz[k]) + 4.0 * (sxy[i] * dy[j] + sxz[i] * dz[k] + syz[j] * dz[k]);
+    return L;
+}
+
+void fft_ok(rocfft_status s, const char *what)
+{
+    if (s != rocfft_status_success) fail(LYNX_E_HIP, std::string(what) + " failed (rocFFT status " +
+                                                         std::to_string((int)s) + ")");
+}
+
+// forward and inverse real 3-D transforms of one grid, their work buffer,
+// the half spectrum and the symbol
+struct Spectral {
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info info = nullptr;
+    DevBuf work, spec, sym, in;
+    long ncplx = 0;
+
+    void init(const Stencil &st, hipStream_t hs)
+    {
+        static std::once_flag once;
+        rocfft_status setup = rocfft_status_success;
+        std::call_once(once, [&] { setup = rocfft_setup(); });
+        fft_ok(setup, "rocfft_setup");
+        const size_t len[3] = {(size_t)st.nx, (size_t)st.ny, (size_t)st.nz};
+        fft_ok(rocfft_plan_create(&fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                  rocfft_precision_double, 3, len, 1, nullptr),
+               "rocfft_plan_create (forward)");
+        fft_ok(rocfft_plan_create(&inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                  rocfft_precision_double, 3, len, 1, nullptr),
+               "rocfft_plan_create (inverse)");
+        size_t w1 = 0, w2 = 0;
+        fft_ok(rocfft_plan_get_work_buffer_size(fwd, &w1), "rocfft_plan_get_work_buffer_size");
+        fft_ok(rocfft_plan_get_work_buffer_size(inv, &w2), "rocfft_plan_get_work_buffer_size");
+        fft_ok(rocfft_execution_info_create(&info), "rocfft_execution_info_create");
+        const size_t wb = std::max(w1, w2);
+        if (wb > 0) {
+            work.get(wb);
+            fft_ok(rocfft_execution_info_set_work_buffer(info, work.p, wb), "rocfft_execution_info_set_work_buffer");
+        }
+        fft_ok(rocfft_execution_info_set_stream(info, hs), "rocfft_execution_info_set_stream");
+        const std::vector<double> L = fd_symbol(st);
+        ncplx = (long)L.size();
+        spec.get((size_t)ncplx * 2 * sizeof(double));
+        in.get((size_t)st.nx * st.ny * st.nz * sizeof(double));
+        upload(sym, L, hs);
+    }
+
+    // x = F^-1 g(L) F b; b is left unchanged (a transform may overwrite its
+    // input, so it reads a copy)
+    void solve(const double *b, double *x, long nd, int kind, double amix, double c1, double c2, hipStream_t hs)
+    {
+        HIP_OK(hipMemcpyAsync(in.p, b, nd * sizeof(double), hipMemcpyDeviceToDevice, hs));
+        void *src[1] = {in.p}, *dst[1] = {spec.p};
+        fft_ok(rocfft_execute(fwd, src, dst, info), "rocfft_execute (forward)");
+        gpu::scf_spectral_scale(spec.as<double>(), sym.as<double>(), ncplx, kind, amix, c1, c2, 1.0 / (double)nd,
+                                hs);
+        void *src2[1] = {spec.p}, *dst2[1] = {x};
+        fft_ok(rocfft_execute(inv, src2, dst2, info), "rocfft_execute (inverse)");
+    }
+
+    void release()
+    {
+        if (fwd) rocfft_plan_destroy(fwd);
+        if (inv) rocfft_plan_destroy(inv);
+        if (info) rocfft_execution_info_destroy(info);
+        fwd = inv = nullptr;
+        info = nullptr;
+        for (DevBuf *b : {&work, &spec, &sym, &in}) b->release();
+    }
+};
+
 }  // namespace
 
 struct lynx_scf {
@@ -80,6 +198,7 @@ struct lynx_scf {
     DevBuf rhs, ar, af, afo, axo, aX, aF;        // AAR work
     DevBuf mfk, mxkm1, mR, mF, mxw, mfw, mpf, mx, mg;   // mixer state
     DevBuf partials;
+    Spectral fft;                                // LYNX_SOLVE_SPECTRAL
     int mix_iter = 0;
     std::vector<double> hpart;
 
@@ -96,6 +215,7 @@ struct lynx_scf {
                           &aux, &rhs, &ar, &af, &afo, &axo, &aX, &aF, &mfk, &mxkm1, &mR, &mF, &mxw, &mfw,
                           &mpf, &mx, &mg, &partials};
         if (ctx && ctx->device >= 0) (void)hipSetDevice(ctx->device);
+        fft.release();
         for (DevBuf *x : bufs) x->release();
     }
 };
@@ -205,6 +325,11 @@ void solve_poisson(lynx_scf *S, const double *rho_total)
     dcopy(S, rhs, rho_total, n);
     gpu::scf_axpby(rhs, dptr(S->b), 1.0, 1.0, n, S->ctx->stream);
     gpu::scf_scale(rhs, 4.0 * kPi, n, S->ctx->stream);
+    if (S->p.linear_solver == LYNX_SOLVE_SPECTRAL) {
+        // zero-mean right side and solution: the q = 0 coefficient is dropped
+        S->fft.solve(rhs, dptr(S->phi), n, 0, 0.0, 0.0, 0.0, S->ctx->stream);
+        return;
+    }
     const double ntot = (double)n;
     const double mean = dot1(S, rhs, dptr(S->ones), n) / ntot;
     gpu::scf_axpby(rhs, dptr(S->ones), -mean, 1.0, n, S->ctx->stream);
@@ -357,6 +482,11 @@ void kerker(lynx_scf *S, const double *r, double *z, double amix)
 {
     const long n = S->nd;
     const double k2 = kKerkerKTF * kKerkerKTF;
+    if (S->p.linear_solver == LYNX_SOLVE_SPECTRAL) {
+        // z = -amix (-L - idiemac k2) / (L + k2) r
+        S->fft.solve(r, z, n, 1, amix, kKerkerIdiemac * k2, k2, S->ctx->stream);
+        return;
+    }
     double *Lf = dptr(S->rhs);
     lynx_ctx *c = S->ctx;
     const double k0[3] = {0, 0, 0};
@@ -441,7 +571,8 @@ lynx_status lynx_scf_create(lynx_scf **out, lynx_ctx *ctx, const lynx_scf_params
         need_gpu_stencil(ctx);
         if (p->nspin < 1 || p->nspin > 2 || p->nkpt < 1 || p->nband < 1 || p->cheb_degree < 1 ||
             p->rho_trigger < 1 || p->nchefsi < 1 || !(p->beta > 0) || p->xc < 0 || p->xc > 2 || !(p->tol > 0) ||
-            p->max_iter < 1 || p->mixing_history < 1 || p->mixing_history > 8 || !(p->mixing_param > 0))
+            p->max_iter < 1 || p->mixing_history < 1 || p->mixing_history > 8 || !(p->mixing_param > 0) ||
+            (p->linear_solver != LYNX_SOLVE_SPECTRAL && p->linear_solver != LYNX_SOLVE_AAR))
             fail(LYNX_E_ARG, "bad SCF parameters");
         if (!p->complex_kpt && p->nkpt != 1) fail(LYNX_E_ARG, "real orbitals are for one (Gamma) k-point");
         const ElecHost &E = electrostatics(ctx);
@@ -504,6 +635,7 @@ lynx_status lynx_scf_create(lynx_scf **out, lynx_ctx *ctx, const lynx_scf_params
         S->mR.get((size_t)p->mixing_history * N * sizeof(double));
         S->mF.get((size_t)p->mixing_history * N * sizeof(double));
         S->partials.get((size_t)gpu::kScfPartialBlocks * 40 * sizeof(double));
+        if (p->linear_solver == LYNX_SOLVE_SPECTRAL) S->fft.init(ctx->st, hs);
         lynx_scf_eig_params ep{};
         ep.nspin = ns;
         ep.nkpt = p->nkpt;

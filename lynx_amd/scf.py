@@ -33,7 +33,8 @@ class ScfParams(ctypes.Structure):
                 ("xc", ctypes.c_int), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("min_iter", ctypes.c_int), ("poisson_tol", ctypes.c_double), ("precond_tol", ctypes.c_double),
                 ("mixing_history", ctypes.c_int), ("mixing_param", ctypes.c_double), ("kerker", ctypes.c_int),
-                ("fp64_switch_residual", ctypes.c_double), ("low_precision", ctypes.c_int)]
+                ("fp64_switch_residual", ctypes.c_double), ("low_precision", ctypes.c_int),
+                ("linear_solver", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -50,12 +51,15 @@ class ScfResult(ctypes.Structure):
 class SCF:
     """FP64 SCF loop on the GPU for the Hamiltonian ``H`` (set up with atoms
     from psp8 files): nband orbitals per (spin, k-point), Fermi-Dirac
-    (smearing 0) or Gaussian (1) occupations at inverse temperature beta."""
+    (smearing 0) or Gaussian (1) occupations at inverse temperature beta.
+    ``solver`` picks how the Poisson and Kerker equations are solved:
+    "spectral" (FFT, exact to rounding) or "aar" (the reference's iteration
+    to poisson_tol / precond_tol)."""
 
     def __init__(self, H, nband, beta, xc="GGA_PBE", nspin=1, kpts=None, kpt_weights=None, smearing=0,
                  cheb_degree=20, rho_trigger=4, nchefsi=1, tol=1e-6, max_iter=100, min_iter=2, poisson_tol=0.0,
                  precond_tol=0.0, mixing_history=7, mixing_param=0.3, kerker=True, fp64_switch_residual=0.0,
-                 low_precision="fp32"):
+                 low_precision="fp32", solver="spectral"):
         self.H = H
         self.kpts = None if kpts is None else np.ascontiguousarray(kpts, float).reshape(-1, 3)
         nk = 1 if self.kpts is None else len(self.kpts)
@@ -63,7 +67,8 @@ class SCF:
         p = ScfParams(nspin, nk, int(self.kpts is not None), nband, cheb_degree, rho_trigger, nchefsi, float(beta),
                       smearing, XC[xc] if isinstance(xc, str) else int(xc), float(tol), max_iter, min_iter,
                       float(poisson_tol), float(precond_tol), mixing_history, float(mixing_param), int(kerker),
-                      float(fp64_switch_residual), {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision])
+                      float(fp64_switch_residual), {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision],
+                      {"spectral": L.SOLVE_SPECTRAL, "aar": L.SOLVE_AAR}[solver])
         kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
         self._s = ctypes.c_void_p()
         L.check(L.lib().lynx_scf_create(ctypes.byref(self._s), H.ctx, ctypes.byref(p),

@@ -11,7 +11,9 @@ checker; the product library never loads it.  numpy restatements of:
     3865 (1996)), written from the published formulas (the reference calls
     libxc, XCFunctional.cpp:44-340; libxc is absent here);
   * the AAR Poisson solve (PoissonSolver.cpp:43-166) and the Kerker
-    preconditioner (KerkerPreconditioner.cpp:37-95, LinearSolver.cpp:23-130);
+    preconditioner (KerkerPreconditioner.cpp:37-95, LinearSolver.cpp:23-130),
+    and the same two equations solved exactly in the Fourier basis of the
+    periodic grid (fd_symbol; the library's LYNX_SOLVE_SPECTRAL);
   * Pulay mixing with the density constraint (Mixer.cpp:116-266);
   * density, energies and the SCF loop (ElectronDensity.cpp:55-140,
     Energy.cpp:140-200, SCF.cpp:110-291, 470-790).
@@ -400,15 +402,43 @@ def aar(apply_A, b, x, tol, m_inv, max_iter):
     return x, it
 
 
+def fd_symbol(grid, st):
+    """Fourier symbol of the periodic FD operator -lap (the stencil of
+    Hamiltonian.cpp:126-267 with its mixed terms) on numpy's rfftn layout
+    (nz, ny, nx // 2 + 1): a plane wave with phases theta_a per grid step
+    picks up 2 cos(p theta) from each symmetric pair and (2i sin)(2i sin)
+    from each mixed four-point group."""
+    nx, ny, nz, R = grid.nx, grid.ny, grid.nz, st.fdn
+    tx = 2 * np.pi * np.arange(nx // 2 + 1) / nx
+    ty = 2 * np.pi * np.arange(ny) / ny
+    tz = 2 * np.pi * np.arange(nz) / nz
+    p = np.arange(1, R + 1)
+
+    def cos_sum(c, t):
+        return c[0] + 2.0 * (np.cos(np.outer(t, p)) * np.asarray(c)[1:]).sum(axis=1)
+
+    def sin_sum(c, t):
+        return (np.sin(np.outer(t, p)) * np.asarray(c)[1:]).sum(axis=1)
+
+    lap = (cos_sum(st.D2[0], tx)[None, None, :] + cos_sum(st.D2[1], ty)[None, :, None]
+           + cos_sum(st.D2[2], tz)[:, None, None])
+    lap = lap - 4.0 * (sin_sum(st.D2m[0], tx)[None, None, :] * sin_sum(st.D1[1], ty)[None, :, None]
+                       + sin_sum(st.D2m[1], tx)[None, None, :] * sin_sum(st.D1[2], tz)[:, None, None]
+                       + sin_sum(st.D2m[2], ty)[None, :, None] * sin_sum(st.D1[2], tz)[:, None, None])
+    return -lap
+
+
 class ScfOracle:
     """SCF::run (SCF.cpp:110-291) in numpy around the oracle eigen stage:
-    density mixing with Pulay + Kerker, collinear spin, LDA (PZ, PW) or PBE."""
+    density mixing with Pulay + Kerker, collinear spin, LDA (PZ, PW) or PBE.
+    solver "aar" solves the Poisson and Kerker equations as the reference
+    does; "spectral" solves them exactly with fd_symbol."""
 
     KTF, IDIEMAC = 1.0, 0.1
 
     def __init__(self, grid, psps, types, frac, nband, beta, xc=2, nspin=1, kpts=None, kpt_weights=None,
                  smearing=0, degree=20, rho_trigger=4, nchefsi=1, tol=1e-6, max_iter=100, min_iter=2,
-                 mixing_history=7, mixing_param=0.3, kerker=True, order=12):
+                 mixing_history=7, mixing_param=0.3, kerker=True, order=12, solver="aar"):
         from . import ScfEigStage, setup_system
         self.grid = grid
         self.st = stencil_coeffs(order, grid)
@@ -432,6 +462,13 @@ class ScfOracle:
         self.diag = float(self.st.D2[0][0] + self.st.D2[1][0] + self.st.D2[2][0])
         lat = grid.lattice
         self.M = np.eye(3) if lat.orthogonal else lat.lapc_T
+        self.solver = solver
+        self.sym = fd_symbol(grid, self.st) if solver == "spectral" else None
+
+    def _spectral(self, b, factor):
+        g = self.grid
+        B = np.fft.rfftn(b.reshape(g.nz, g.ny, g.nx))
+        return np.fft.irfftn(B * factor, s=(g.nz, g.ny, g.nx), axes=(0, 1, 2)).ravel()
 
     # -- operators ---------------------------------------------------------
     def _A(self, x, k2):
@@ -452,6 +489,13 @@ class ScfOracle:
     def poisson(self, rho_total):
         """EffectivePotential.cpp:44-72."""
         rhs = 4.0 * math.pi * (rho_total + self.E.b)
+        if self.solver == "spectral":
+            inv = np.zeros_like(self.sym)
+            nz0 = np.ones(self.sym.shape, bool)
+            nz0[0, 0, 0] = False
+            inv[nz0] = 1.0 / self.sym[nz0]
+            self.phi = self._spectral(rhs, inv)
+            return self.phi
         rhs = rhs - rhs.sum() / len(rhs)
         m_inv = 1.0 if abs(self.diag) < 1e-14 else -1.0 / self.diag
         self.phi, self.poisson_iters = aar(lambda x: self._A(x, 0.0), rhs, self.phi, self.poisson_tol, m_inv, 3000)
@@ -505,6 +549,8 @@ class ScfOracle:
     def kerker_apply(self, r):
         """KerkerPreconditioner.cpp:37-95."""
         k2 = self.KTF ** 2
+        if self.solver == "spectral":
+            return self._spectral(r, self.amix * (self.sym + self.IDIEMAC * k2) / (self.sym + k2))
         Lf = -self._A(r, self.IDIEMAC * k2)         # (lap - idiemac kTF^2) r
         md = self.diag - k2
         m_inv = 1.0 if abs(md) < 1e-14 else -1.0 / md

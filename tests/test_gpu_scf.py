@@ -3,14 +3,16 @@
 reference's published total energy:
 
 - one potential build (XC, gradients, Poisson, Veff) from a given density,
-  LDA PZ81 / PW92 and PBE, one and two spins: exc and Vxc to 1e-11
-  relative, phi and Veff to 1e-9 (the Poisson solve's own tolerance is
-  1e-8 relative);
+  LDA PZ81 / PW92 and PBE, one and two spins, the Poisson equation solved by
+  AAR or spectrally: exc and Vxc to 1e-11 relative, phi and Veff to 1e-9
+  (the AAR solve's own tolerance is 1e-8 relative); the spectral solve also
+  on an uneven orthogonal grid and the FCC cell to 1e-11;
 - whole SCF runs: the per-iteration total energy and density residual of
   the GPU run follow the oracle's to 1e-7 Ha and 1e-6 relative, the same
   iteration count, the converged energy to 1e-8 Ha/atom: diamond Si8
-  (C.pot stand-in) with PBE at Gamma, and BeCu with collinear spin, LDA
-  PZ81, Gaussian smearing and two k-points;
+  (C.pot stand-in) with PBE at Gamma on a 16^3 grid and on the full 30^3
+  grid, and BeCu with collinear spin, LDA PZ81, Gaussian smearing and two
+  k-points;
 - BeCu with the reference's own input (tests/golden/becu/input.json:
   21^3 grid, 6x6x6 k-points reduced to 112, 18 states, spin, PZ81) against
   tests/golden/becu/reference.json.
@@ -63,13 +65,14 @@ def rel(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
 
 
+@pytest.mark.parametrize("solver", ["aar", "spectral"])
 @pytest.mark.parametrize("xc", [0, 1, 2])
 @pytest.mark.parametrize("nspin", [1, 2])
-def test_potential_build_matches_oracle(xc, nspin):
+def test_potential_build_matches_oracle(xc, nspin, solver):
     from lynx_amd.scf import SCF
     H, og, ps, types, frac = si8(14)
-    S = SCF(H, 20, 1 / (KB * 1000), xc=xc, nspin=nspin, tol=1e-6)
-    O = osc.ScfOracle(og, ps, types, frac, 20, 1 / (KB * 1000), xc=xc, nspin=nspin, tol=1e-6)
+    S = SCF(H, 20, 1 / (KB * 1000), xc=xc, nspin=nspin, tol=1e-6, solver=solver)
+    O = osc.ScfOracle(og, ps, types, frac, 20, 1 / (KB * 1000), xc=xc, nspin=nspin, tol=1e-6, solver=solver)
     rng = np.random.default_rng(10 + xc)
     rho = O.E.rho_at * (1.0 + 0.2 * rng.uniform(-1, 1, og.nd))
     if nspin == 1:
@@ -89,6 +92,35 @@ def test_potential_build_matches_oracle(xc, nspin):
     S.close()
 
 
+@pytest.mark.parametrize("cell", ["orthogonal_uneven", "fcc"])
+def test_spectral_potential_on_uneven_and_nonorthogonal_grids(cell):
+    # the FFT axis order and the mixed-term symbol: an orthogonal cell with
+    # three different grid lengths, and the FCC primitive cell
+    from lynx_amd import core
+    from lynx_amd.scf import SCF
+    from systems import FCC_PRIMITIVE, SI2
+    if cell == "fcc":
+        lv, n, frac, orth = A_SI * FCC_PRIMITIVE, (16, 16, 16), SI2, False
+    else:
+        lv, n, frac, orth = np.diag([A_SI, 1.1 * A_SI, 1.25 * A_SI]), (14, 16, 18), DIAMOND, True
+    lat = core.Lattice(lv, core.CellType.Orthogonal if orth else core.CellType.NonOrthogonal)
+    grid = core.FDGrid(*n, lat)
+    st = core.FDStencil(12, grid, lat)
+    dom = core.Domain(grid)
+    vnl = core.NonlocalProjector().setup_atoms([psp("C.pot")], [0] * len(frac), frac, True)
+    H = core.Hamiltonian(0).setup(st, dom, grid, core.HaloExchange(dom, 6), vnl)
+    og = oracle.Grid(*n, oracle.Lattice(lv, orth))
+    ps = [oracle.load_psp8(psp("C.pot"))]
+    S = SCF(H, 24, 1 / (KB * 1000), xc=2, tol=1e-6, solver="spectral")
+    O = osc.ScfOracle(og, ps, [0] * len(frac), frac, 24, 1 / (KB * 1000), xc=2, tol=1e-6, solver="spectral")
+    rho = O.E.rho_at * (1.0 + 0.2 * np.random.default_rng(3).uniform(-1, 1, og.nd))
+    veff = O.potential([rho])
+    got = S.build_potential(rho)
+    assert rel(got["phi"], O.phi) < 1e-11
+    assert rel(got["veff"], veff) < 1e-11
+    S.close()
+
+
 def _compare_runs(S, O, natom, e_tol=1e-7):
     got = S.run()
     ref = O.run()
@@ -104,26 +136,42 @@ def _compare_runs(S, O, natom, e_tol=1e-7):
     return got, ref
 
 
-def test_scf_si8_pbe_gamma_matches_oracle():
+@pytest.mark.parametrize("solver", ["aar", "spectral"])
+def test_scf_si8_pbe_gamma_matches_oracle(solver):
     from lynx_amd.scf import SCF
     H, og, ps, types, frac = si8(16)
-    S = SCF(H, 24, 1 / (KB * 1000), xc="GGA_PBE", tol=1e-6)
-    O = osc.ScfOracle(og, ps, types, frac, 24, 1 / (KB * 1000), xc=2, tol=1e-6)
+    S = SCF(H, 24, 1 / (KB * 1000), xc="GGA_PBE", tol=1e-6, solver=solver)
+    O = osc.ScfOracle(og, ps, types, frac, 24, 1 / (KB * 1000), xc=2, tol=1e-6, solver=solver)
     got, ref = _compare_runs(S, O, 8)
-    print(f"[scf] Si8 16^3 PBE: {got['iterations']} iterations, E = {got['etotal']:.10f} Ha "
+    print(f"[scf] Si8 16^3 PBE ({solver}): {got['iterations']} iterations, E = {got['etotal']:.10f} Ha "
           f"(oracle {ref['etotal']:.10f}), {got['seconds']:.2f} s", flush=True)
     S.close()
 
 
-def test_scf_becu_spin_kpoints_matches_oracle():
+def test_scf_si8_30cubed_pbe_matches_oracle():
+    # the Si8 SCF workload at its full size: 30^3 grid, 24 states, PBE,
+    # Fermi-Dirac 1000 K, degree 20, to a 1e-6 density residual
+    from lynx_amd.scf import SCF
+    H, og, ps, types, frac = si8(30)
+    S = SCF(H, 24, 1 / (KB * 1000), xc="GGA_PBE", tol=1e-6, solver="spectral")
+    O = osc.ScfOracle(og, ps, types, frac, 24, 1 / (KB * 1000), xc=2, tol=1e-6, solver="spectral")
+    got, ref = _compare_runs(S, O, 8)
+    print(f"[scf] Si8 30^3 PBE: {got['iterations']} iterations, E = {got['etotal']:.10f} Ha "
+          f"(oracle {ref['etotal']:.10f}), {got['seconds']:.2f} s", flush=True)
+    S.close()
+
+
+@pytest.mark.parametrize("solver", ["aar", "spectral"])
+def test_scf_becu_spin_kpoints_matches_oracle(solver):
     from lynx_amd.scf import SCF
     H, og, ps, types, frac, ks = becu(2)
     beta = 1 / (KB * 1160.45)
-    S = SCF(H, 18, beta, xc="LDA_PZ", nspin=2, kpts=ks, kpt_weights=[0.5, 0.5], smearing=1, tol=1.58e-4)
+    S = SCF(H, 18, beta, xc="LDA_PZ", nspin=2, kpts=ks, kpt_weights=[0.5, 0.5], smearing=1, tol=1.58e-4,
+            solver=solver)
     O = osc.ScfOracle(og, ps, types, frac, 18, beta, xc=0, nspin=2, kpts=ks, kpt_weights=[0.5, 0.5], smearing=1,
-                      tol=1.58e-4)
+                      tol=1.58e-4, solver=solver)
     got, ref = _compare_runs(S, O, 2)
-    print(f"[scf] BeCu 2 k-points, spin, PZ81: {got['iterations']} iterations, E = {got['etotal']:.10f} Ha "
+    print(f"[scf] BeCu 2 k-points, spin, PZ81 ({solver}): {got['iterations']} iterations, E = {got['etotal']:.10f} Ha "
           f"(oracle {ref['etotal']:.10f}), {got['seconds']:.2f} s", flush=True)
     S.close()
 

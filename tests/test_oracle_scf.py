@@ -152,3 +152,41 @@ def test_oracle_scf_converges(si8_scf):
     # last iterations agree to the density residual squared scale
     e = [h["etotal"] for h in S.history]
     assert abs(e[-1] - e[-2]) < 1e-6
+
+
+@pytest.mark.parametrize("cell", ["orthogonal_uneven", "fcc"])
+def test_fd_symbol_is_the_stencil_operator(cell):
+    # the FFT of the stencil's own -lap x equals the symbol times the FFT of
+    # x for a random x (every wavevector at once), on an orthogonal cell with
+    # three different grid lengths and spacings and on the non-orthogonal FCC
+    # primitive cell with its three mixed terms
+    from systems import FCC_PRIMITIVE
+    if cell == "fcc":
+        g = oracle.Grid(12, 12, 12, oracle.Lattice(A_SI * FCC_PRIMITIVE, False))
+    else:
+        g = oracle.Grid(10, 12, 14, oracle.Lattice(np.diag([9.0, 10.5, 13.0]), True))
+    st = oracle.stencil_coeffs(12, g)
+    op = oracle.Operator(g, 12)
+    x = np.random.default_rng(5).normal(size=g.nd)
+    y = 2.0 * op.apply(x.reshape(-1, 1), np.zeros(g.nd), nonlocal_=False).ravel()
+    L = osc.fd_symbol(g, st)
+    X = np.fft.rfftn(x.reshape(g.nz, g.ny, g.nx))
+    Y = np.fft.rfftn(y.reshape(g.nz, g.ny, g.nx))
+    assert np.abs(Y - L * X).max() < 1e-11 * np.abs(Y).max()
+    assert abs(L[0, 0, 0]) < 1e-9 and L.ravel()[1:].min() > 0
+
+
+def test_spectral_and_aar_scf_reach_the_same_energy():
+    # the two ways of solving the Poisson and Kerker equations reach one
+    # discrete solution (AAR to its tolerance): the converged SCF energies
+    # agree to 1e-8 Ha/atom
+    g = _grid(12)
+    ps = [oracle.load_psp8(psp("C.pot"))]
+    beta = 1 / (3.1668115634556e-6 * 1000)
+    out = {}
+    for solver in ("aar", "spectral"):
+        S = osc.ScfOracle(g, ps, [0] * 8, DIAMOND, 20, beta, xc=2, tol=1e-7, solver=solver)
+        r = S.run()
+        assert r["converged"]
+        out[solver] = r["etotal"]
+    assert abs(out["aar"] - out["spectral"]) / 8 < 1e-8, out
