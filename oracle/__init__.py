@@ -47,8 +47,9 @@ PINNING = {
     "lanczos": "the reference's own lanczos_bounds<T> (Lanczos.hpp, LAPACK dsterf_) run on this oracle's H.v: "
                "same step count and bounds; bounds bracket the dense spectrum; glibc rand() start vector",
     "rand_uniform": "the reference's Wavefunction::randomize[_kpt], compiled from source, bitwise",
-    "fermi_level": "the reference's Occupation::compute, compiled from source; closed forms (symmetric levels, "
-                   "one half-filled level, sum w f = Ne)",
+    "fermi_level": "the reference's Occupation::compute, compiled from source, on metallic and gapped spectra "
+                   "(in a gap the root is flat and the Brent steps fix the point); closed forms (symmetric "
+                   "levels, one half-filled level, sum w f = Ne)",
     "ScfEigStage": "composed of the pinned solve, Lanczos and Fermi level; its bookkeeping by properties a "
                    "plausible mistake fails: a constant Veff shift of one spin shifts that spin's Lanczos bounds "
                    "exactly and makes it set lambda_c; eig_min kept at the Lanczos value in iteration 0 and "

@@ -32,6 +32,31 @@ from . import Grid, Lattice, Operator, positions_noncart, stencil_coeffs  # noqa
 
 RC_REF = 0.5   # cutoff of the smooth reference potential (Electrostatics.cpp:176)
 
+# What fixes each function here independently of its own formula
+# (tests/test_oracle_scf.py, tests/test_ref_subset.py, tests/test_gpu_scf.py).
+PINNING = {
+    "Electrostatics (v_ref, v_local, atom_boxes, pseudocharge, b_ref, Vc, Eself, Ec, rho_at, rho_core)":
+        "the reference's own Crystal + Electrostatics compiled from source, on an orthogonal Si8 cell, the FCC "
+        "Si2 cell and two-species BeCu, to 1e-12; integral of b = -Ne",
+    "fx_slater": "the constants C2, C3 of tests/unit/test_XCFunctional.cpp:46-70; its derivative by finite "
+                 "differences",
+    "ec_pz": "the two PZ81 branches join at rs = 1 (they were fitted to); within a few mHa of PW92",
+    "ec_unif (PW92)": "the published PW92 correlation energies at rs = 1, 2, 5, 10",
+    "fx_pbe, fc_pbe": "zero gradient gives the LDA with PW92 (test_XCFunctional.cpp:128-150); the published "
+                      "small-gradient coefficients mu = 0.21951 and beta = 0.066725 (phi^3 scaling at full "
+                      "polarisation); the large-gradient limits 1 + kappa and 0",
+    "xc_unpol, xc_pol": "complex-step derivatives against finite differences; zeta = 0 equals the unpolarized "
+                        "forms; spin scaling of exchange",
+    "aar": "the closed-form FD Poisson solution of a plane wave",
+    "fd_symbol": "the FFT of the stencil's own -lap x equals the symbol times the FFT of x (random x, uneven "
+                 "orthogonal grid and FCC cell)",
+    "ScfOracle (spectral vs aar)": "both converge to the same SCF energy (1e-8 Ha/atom)",
+    "ScfOracle.kerker_apply": "z solves (-lap + kTF^2) z = -amix (lap - idiemac kTF^2) r",
+    "ScfOracle.mix": "the density constraint: non-negative, integral Ne",
+    "ScfOracle.run": "converges; the library following it iteration by iteration reproduces the reference's "
+                     "published BeCu energy (test_suite/systems/BeCu_bulk/reference.json) to 1e-7 Ha/atom",
+}
+
 
 def v_ref(r, rc, Z):
     """Electrostatics.cpp:16-33: -Z/r outside rc, a polynomial inside."""

@@ -190,3 +190,24 @@ def test_spectral_and_aar_scf_reach_the_same_energy():
         assert r["converged"]
         out[solver] = r["etotal"]
     assert abs(out["aar"] - out["spectral"]) / 8 < 1e-8, out
+
+
+def test_pbe_gradient_expansions():
+    # PBE's published small-gradient limits (PRL 77, 3865): the exchange
+    # enhancement F_x = 1 + mu s^2 with mu = 0.21951, and the correlation
+    # gradient term H = beta phi^3 t^2 with beta = 0.066725 (phi = 1
+    # unpolarized, phi = 2^(-1/3) fully polarized); the numbers are typed
+    # from the paper, not taken from the oracle's constants
+    rho = np.array([0.02, 0.2, 2.0])
+    kf = np.cbrt(3 * math.pi ** 2 * rho)
+    s2 = 1e-6
+    sigma = s2 * (2 * kf * rho) ** 2
+    fx = osc.fx_pbe(rho, sigma) / osc.fx_slater(rho)
+    assert np.allclose((fx - 1) / s2, 0.21951, rtol=1e-4)
+    ks = np.sqrt(4 * kf / math.pi)
+    rs = np.cbrt(3 / (4 * math.pi * rho))
+    for zeta, phi in ((None, 1.0), (1 - 1e-12, 2 ** (-1 / 3))):
+        t2 = 1e-7
+        sig = t2 * (2 * phi * ks * rho) ** 2
+        H = osc.fc_pbe(rho, sig, zeta) / rho - osc.ec_unif(rs, zeta, 1, mod=True)
+        assert np.allclose(H / t2, 0.066725 * phi ** 3, rtol=2e-4)
