@@ -425,7 +425,8 @@ def run_scf_si216():
     until the residual falls below 1e-4, FP64 after; the same with the
     filter's blocks stored in bfloat16.  Each mode first runs one untimed
     iteration so the launch shapes it uses are tuned.  Errors are against
-    the FP64 run: total energy per atom and occupied eigenvalues."""
+    the FP64 run: total energy per atom, occupied eigenvalues and atomic
+    forces (lynx_scf_forces after each run, timed)."""
     from systems import si216_mixed
     from lynx_amd.scf import SCF
     from lynx_amd.hipmem import synchronize
@@ -446,23 +447,28 @@ def run_scf_si216():
         synchronize()
         wall = time.perf_counter() - t0
         eig, occ, _, _ = S.state()
-        res[name] = (r, eig, occ)
+        t0 = time.perf_counter()
+        F = S.forces()
+        tf = time.perf_counter() - t0
+        res[name] = (r, eig, occ, F)
         out[name] = {"scf_wall_s": round(wall, 3), "iterations": r["iterations"], "converged": bool(r["converged"]),
-                     "etotal_per_atom_Ha": r["etotal"] / 216}
+                     "etotal_per_atom_Ha": r["etotal"] / 216, "forces_s": round(tf, 3),
+                     "max_abs_force_Ha_per_Bohr": float(np.abs(F).max())}
         out[name].update(_scf_history_summary(S))
         S.close()
         print(f"[bench] Si216 SCF {name}: {wall:.2f} s, {r['iterations']} iterations, "
               f"E/atom {r['etotal'] / 216:.10f}", file=sys.stderr, flush=True)
-    r64, e64, o64 = res["fp64"]
+    r64, e64, o64, F64 = res["fp64"]
     occ_mask = o64 > 1e-6
     summary = {"workload": "Si216 3x3x3 supercell a=30.78 Bohr, sigma=0.03 (seed 3), 90^3, GGA-PBE, Gamma, "
                            "480 states, Fermi-Dirac 1000 K, degree 20, SCF to 1e-6 density residual; C.pot "
                            "stands in for Si", "modes": out}
     for name in ("fp32_until_1e-4", "bf16_until_1e-4"):
-        r, e, _ = res[name]
+        r, e, _, F = res[name]
         summary[name] = {"speedup_scf_wall": round(out["fp64"]["scf_wall_s"] / out[name]["scf_wall_s"], 3),
                          "etotal_err_per_atom_Ha": abs(r["etotal"] - r64["etotal"]) / 216,
-                         "max_abs_occupied_eig_err_Ha": float(np.abs(e - e64)[occ_mask].max())}
+                         "max_abs_occupied_eig_err_Ha": float(np.abs(e - e64)[occ_mask].max()),
+                         "max_force_err_Ha_per_Bohr": float(np.abs(F - F64).max())}
     return summary
 
 

@@ -454,6 +454,17 @@ lynx_status lynx_scf_get(const lynx_scf *scf, double *eig, double *occ, double *
 /* The electrostatic potential phi and the XC energy density of the last
  * potential build (Nd each); any pointer may be NULL. */
 lynx_status lynx_scf_get_fields(const lynx_scf *scf, double *phi, double *exc, double *vxc);
+/* Atomic forces (Ha/Bohr, Cartesian, 3 per atom in the order of
+ * lynx_set_atoms) of the state the last lynx_scf_run left (its final phi,
+ * Vxc, orbitals and occupations): local (pseudocharge) term, nonlocal
+ * (projector) term with <chi|psi> and <chi|grad psi> on the GPU, and the
+ * core-correction XC term; components along the lattice directions mapped
+ * to Cartesian (non-orthogonal cells), then the mean force removed.
+ * Replaces Forces::compute (Forces.cpp:25-184, 186-420, 509-800,
+ * 1108-1288).  parts (optional, 9 per atom): the local, nonlocal and
+ * core-correction terms (Cartesian, before the mean is removed), each
+ * 3 x natom.  LYNX_E_STATE before a completed run. */
+lynx_status lynx_scf_forces(lynx_scf *scf, double *forces, double *parts);
 /* One potential build from a given density (nspin x Nd as lynx_scf_get),
  * without the eigen stage: XC, Poisson (warm start from the previous phi)
  * and Veff; for testing the SCF tail alone. */

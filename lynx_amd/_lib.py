@@ -81,6 +81,7 @@ SIGNATURES = {
     "lynx_scf_get": (_I, [_P, _DP, _DP, _DP, _DP]),
     "lynx_scf_get_fields": (_I, [_P, _DP, _DP, _DP]),
     "lynx_scf_build_potential": (_I, [_P, _DP]),
+    "lynx_scf_forces": (_I, [_P, _DP, _DP]),
     "lynx_batch_create": (_I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), _DP]),
     "lynx_batch_destroy": (_I, [_P]),
     "lynx_batch_last_error": (ctypes.c_char_p, [_P]),

@@ -123,6 +123,67 @@ std::vector<Box> atom_boxes(const Lattice &lat, const Stencil &st, const std::ve
     return out;
 }
 
+// Crystal.cpp:327-336: rc_max over every species' projector radii and
+// radial table, plus 8 grid spacings
+double influence_radius(const std::vector<Psp8> &psps, const Stencil &st)
+{
+    double rc_max = 0.0;
+    for (const Psp8 &ps : psps) {
+        for (double r : ps.rc) rc_max = std::max(rc_max, r);
+        if (!ps.r.empty()) rc_max = std::max(rc_max, ps.r.back());
+    }
+    return rc_max + 8.0 * std::max({st.h[0], st.h[1], st.h[2]});
+}
+
+// -1/(4 pi) times the FD Laplacian (mixed terms unfactored, as
+// Electrostatics.cpp:48-127) of a box-extended field V at index idx
+struct BoxLaplacian {
+    const Lattice &lat;
+    const Stencil &st;
+    long sy, sz;
+    double lap(const std::vector<double> &V, long idx) const
+    {
+        const int R = st.fdn;
+        const double coef = -0.25 / M_PI;
+        const double *cx = st.D2[0].data(), *cy = st.D2[1].data(), *cz = st.D2[2].data();
+        const double *cxy = st.D2m[0].data(), *cxz = st.D2m[1].data(), *cyz = st.D2m[2].data();
+        const double *d1y = st.D1[1].data(), *d1z = st.D1[2].data();
+        double val = (cx[0] + cy[0] + cz[0]) * coef * V[idx];
+        for (int p = 1; p <= R; ++p) {
+            val += coef * cx[p] * (V[idx + p] + V[idx - p]);
+            val += coef * cy[p] * (V[idx + p * sy] + V[idx - p * sy]);
+            val += coef * cz[p] * (V[idx + p * sz] + V[idx - p * sz]);
+        }
+        if (!lat.orth) {
+            if (std::fabs(cxy[1]) > 1e-30)
+                for (int p = 1; p <= R; ++p)
+                    for (int q = 1; q <= R; ++q)
+                        val += coef * cxy[p] * d1y[q] *
+                               (V[idx + p + q * sy] - V[idx + p - q * sy] - V[idx - p + q * sy] + V[idx - p - q * sy]);
+            if (std::fabs(cxz[1]) > 1e-30)
+                for (int p = 1; p <= R; ++p)
+                    for (int q = 1; q <= R; ++q)
+                        val += coef * cxz[p] * d1z[q] *
+                               (V[idx + p + q * sz] - V[idx + p - q * sz] - V[idx - p + q * sz] + V[idx - p - q * sz]);
+            if (std::fabs(cyz[1]) > 1e-30)
+                for (int p = 1; p <= R; ++p)
+                    for (int q = 1; q <= R; ++q)
+                        val += coef * cyz[p] * d1z[q] *
+                               (V[idx + p * sy + q * sz] - V[idx + p * sy - q * sz] - V[idx - p * sy + q * sz] +
+                                V[idx - p * sy - q * sz]);
+        }
+        return val;
+    }
+    // FD first derivative along lattice axis a
+    double d1(const std::vector<double> &V, long idx, int a) const
+    {
+        const long s = a == 0 ? 1 : (a == 1 ? sy : sz);
+        double v = 0.0;
+        for (int p = 1; p <= st.fdn; ++p) v += (V[idx + p * s] - V[idx - p * s]) * st.D1[a][p];
+        return v;
+    }
+};
+
 }  // namespace
 
 void build_electrostatics(const Lattice &lat, const Stencil &st, const std::vector<Psp8> &psps,
@@ -138,14 +199,7 @@ void build_electrostatics(const Lattice &lat, const Stencil &st, const std::vect
     out.vc.assign(nd, 0.0);
     out.rho_at.assign(nd, 0.0);
     out.rho_core.assign(nd, 0.0);
-    // Crystal.cpp:327-336: rc_max over every species' projector radii and
-    // radial table, plus 8 grid spacings
-    double rc_max = 0.0;
-    for (const Psp8 &ps : psps) {
-        for (double r : ps.rc) rc_max = std::max(rc_max, r);
-        if (!ps.r.empty()) rc_max = std::max(rc_max, ps.r.back());
-    }
-    rc_max += 8.0 * std::max({st.h[0], st.h[1], st.h[2]});
+    const double rc_max = influence_radius(psps, st);
     out.nelectron = 0;
     for (int t : types) out.nelectron += (int)psps[t].zion;
     out.natom = (int)types.size();
@@ -260,6 +314,81 @@ void build_electrostatics(const Lattice &lat, const Stencil &st, const std::vect
     if (int_rho > 1e-10) {
         const double scale = out.nelectron / int_rho;
         for (double &v : out.rho_at) v *= scale;
+    }
+}
+
+void local_forces(const Lattice &lat, const Stencil &st, const std::vector<Psp8> &psps,
+                  const std::vector<int> &types, const std::vector<double> &pos, const ElecHost &E,
+                  const double *const dphi[3], const double *const dvc[3], const double *vxc,
+                  std::vector<double> &f_loc, std::vector<double> &f_xc)
+{
+    const int natom = (int)types.size();
+    f_loc.assign(3 * (size_t)natom, 0.0);
+    f_xc.assign(3 * (size_t)natom, 0.0);
+    const int R = st.fdn;
+    const double dV = st.dV;
+    const double rc_ref = 0.5;
+    const std::vector<Box> boxes = atom_boxes(lat, st, types, pos, influence_radius(psps, st));
+    const long nxg = st.nx, nyg = st.ny;
+    for (const Box &B : boxes) {
+        const Psp8 &ps = psps[B.type];
+        const double Z = ps.zion;
+        const bool core = vxc && ps.fchrg > 1e-10 && !ps.rho_c.empty();
+        const int nl[3] = {B.e[0] - B.s[0] + 1, B.e[1] - B.s[1] + 1, B.e[2] - B.s[2] + 1};
+        const int np[3] = {nl[0] + 2 * R, nl[1] + 2 * R, nl[2] + 2 * R};
+        const long npl = (long)np[0] * np[1];
+        const size_t nbox = (size_t)npl * np[2];
+        std::vector<double> VJ(nbox), VR(nbox), VC(nbox), RC(core ? nbox : 0);
+        parallel_for(np[2], [&](int kk) {
+            const double rz = (B.s[2] - R + kk) * st.h[2] - B.c[2];
+            for (int jj = 0; jj < np[1]; ++jj) {
+                const double ry = (B.s[1] - R + jj) * st.h[1] - B.c[1];
+                for (int ii = 0; ii < np[0]; ++ii) {
+                    const double rx = (B.s[0] - R + ii) * st.h[0] - B.c[0];
+                    const double r = distance(lat, rx, ry, rz);
+                    const long idx = ii + jj * (long)np[0] + kk * npl;
+                    double vj = v_local(ps, r), vr = v_ref(r, rc_ref, Z);
+                    if (std::isnan(vj)) vj = ps.vloc0;
+                    if (std::isnan(vr)) vr = v_ref(0.0, rc_ref, Z);
+                    VJ[idx] = vj;
+                    VR[idx] = vr;
+                    VC[idx] = vr - vj;
+                    if (core) RC[idx] = r <= ps.r.back() ? spline_eval1(ps.r, ps.rho_c, ps.rho_c_d, r) : 0.0;
+                }
+            }
+        });
+        const BoxLaplacian L{lat, st, (long)np[0], npl};
+        // per-plane sums, added over planes in order
+        std::vector<double> part((size_t)nl[2] * 6, 0.0);
+        parallel_for(nl[2], [&](int kk) {
+            double f[3] = {0, 0, 0}, corr[3] = {0, 0, 0}, fx[3] = {0, 0, 0};
+            for (int jj = 0; jj < nl[1]; ++jj)
+                for (int ii = 0; ii < nl[0]; ++ii) {
+                    const long idx = (ii + R) + (jj + R) * (long)np[0] + (kk + R) * npl;
+                    const long g = (B.s[0] + ii) + (B.s[1] + jj) * nxg + (B.s[2] + kk) * nxg * nyg;
+                    const double bj = L.lap(VJ, idx), br = L.lap(VR, idx);
+                    const double bb = E.b[g] + E.b_ref[g];
+                    for (int a = 0; a < 3; ++a) {
+                        f[a] -= bj * dphi[a][g];
+                        corr[a] += L.d1(VC, idx, a) * bb - dvc[a][g] * (bj + br);
+                        if (core) fx[a] += vxc[g] * L.d1(RC, idx, a);
+                    }
+                }
+            for (int a = 0; a < 3; ++a) {
+                part[(size_t)kk * 6 + a] = f[a] + 0.5 * corr[a];
+                part[(size_t)kk * 6 + 3 + a] = fx[a];
+            }
+        });
+        double fl[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+        for (int kk = 0; kk < nl[2]; ++kk)
+            for (int a = 0; a < 3; ++a) {
+                fl[a] += part[(size_t)kk * 6 + a];
+                fc[a] += part[(size_t)kk * 6 + 3 + a];
+            }
+        for (int a = 0; a < 3; ++a) {
+            f_loc[3 * (size_t)B.atom + a] += fl[a] * dV;
+            f_xc[3 * (size_t)B.atom + a] += fc[a] * dV;
+        }
     }
 }
 

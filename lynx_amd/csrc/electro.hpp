@@ -20,4 +20,13 @@ struct ElecHost {
 void build_electrostatics(const Lattice &lat, const Stencil &st, const std::vector<Psp8> &psps,
                           const std::vector<int> &types, const std::vector<double> &pos_nc, ElecHost &out);
 
+// Local and core-correction forces per atom, lattice-aligned components
+// (3 per atom): dphi and dvc are the FD gradients of phi and Vc along the
+// three lattice directions on the grid; vxc (Nd, or null) weights the
+// gradient of each atom's core density.
+void local_forces(const Lattice &lat, const Stencil &st, const std::vector<Psp8> &psps,
+                  const std::vector<int> &types, const std::vector<double> &pos_nc, const ElecHost &E,
+                  const double *const dphi[3], const double *const dvc[3], const double *vxc,
+                  std::vector<double> &f_loc, std::vector<double> &f_xc);
+
 }  // namespace lynx

@@ -390,6 +390,50 @@ __global__ void k_axpby(double *__restrict__ y, const double *__restrict__ x, do
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         y[i] = a * x[i] + b * y[i];
 }
+struct GradCoef {
+    double d1[kMaxFdn + 1];
+};
+
+template <bool CPLX>
+__global__ void __launch_bounds__(kNT) k_grad_block(const double *__restrict__ x, long ldx, double *__restrict__ y,
+                                                     long ldy, int ncol, int nx, int ny, int nz, int dir,
+                                                     GradCoef c, int R, double lo_re, double lo_im, double hi_re,
+                                                     double hi_im)
+{
+    const long nd = (long)nx * ny * nz;
+    const long total = nd * ncol;
+    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    const long stride = dir == 0 ? 1 : (dir == 1 ? (long)nx : (long)nx * ny);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const long col = t / nd, g = t - col * nd;
+        const int i = (int)(dir == 0 ? g % nx : (dir == 1 ? (g / nx) % ny : g / ((long)nx * ny)));
+        const long base = g - (long)i * stride;
+        if (CPLX) {
+            const double2 *xc = reinterpret_cast<const double2 *>(x) + col * ldx;
+            double re = 0.0, im = 0.0;
+            for (int p = 1; p <= R; ++p) {
+                const int ip = i + p, im_ = i - p;
+                double2 u = xc[base + (long)(ip >= n ? ip - n : ip) * stride];
+                double2 v = xc[base + (long)(im_ < 0 ? im_ + n : im_) * stride];
+                if (ip >= n) u = make_double2(u.x * hi_re - u.y * hi_im, u.x * hi_im + u.y * hi_re);
+                if (im_ < 0) v = make_double2(v.x * lo_re - v.y * lo_im, v.x * lo_im + v.y * lo_re);
+                re += c.d1[p] * (u.x - v.x);
+                im += c.d1[p] * (u.y - v.y);
+            }
+            reinterpret_cast<double2 *>(y)[col * ldy + g] = make_double2(re, im);
+        } else {
+            const double *xr = x + col * ldx;
+            double acc = 0.0;
+            for (int p = 1; p <= R; ++p) {
+                const int ip = i + p, im_ = i - p;
+                acc += c.d1[p] * (xr[base + (long)(ip >= n ? ip - n : ip) * stride] -
+                                  xr[base + (long)(im_ < 0 ? im_ + n : im_) * stride]);
+            }
+            y[col * ldy + g] = acc;
+        }
+    }
+}
+
 __global__ void k_spectral_scale(double2 *__restrict__ spec, const double *__restrict__ sym, long n, int kind,
                                  double amix, double c1, double c2, double norm)
 {
@@ -637,6 +681,24 @@ void scf_axpby(double *y, const double *x, double a, double b, long n, hipStream
 {
     k_axpby<<<blocks_for(n), kNT, 0, st>>>(y, x, a, b, n);
     check_launch("k_axpby");
+}
+
+void scf_grad_block(bool cplx, const double *x, long ldx, double *y, long ldy, int ncol, int nx, int ny, int nz,
+                    int dir, const double *d1, int R, const double ph_lo[2], const double ph_hi[2], hipStream_t st)
+{
+    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    if (R < 1 || R > kMaxFdn || R > n || dir < 0 || dir > 2)
+        fail(8, "scf_grad_block: stencil radius outside 1..min(12, grid length)");   // LYNX_E_UNSUPPORTED
+    GradCoef c{};
+    for (int p = 0; p <= R; ++p) c.d1[p] = d1[p];
+    const long total = (long)nx * ny * nz * ncol;
+    if (cplx)
+        k_grad_block<true><<<blocks_for(total), kNT, 0, st>>>(x, ldx, y, ldy, ncol, nx, ny, nz, dir, c, R, ph_lo[0],
+                                                              ph_lo[1], ph_hi[0], ph_hi[1]);
+    else
+        k_grad_block<false><<<blocks_for(total), kNT, 0, st>>>(x, ldx, y, ldy, ncol, nx, ny, nz, dir, c, R, 0, 0, 0,
+                                                               0);
+    check_launch("k_grad_block");
 }
 
 void scf_spectral_scale(double *spec, const double *sym, long n, int kind, double amix, double c1, double c2,

@@ -44,6 +44,12 @@ void scf_sigma(const double M[9], const double *const a[3], const double *const 
 // multiply the n interleaved complex Fourier coefficients spec by a factor
 // of the symbol L of -lap at each: kind 0 (Poisson), norm / L with the
 // q = 0 coefficient set to zero; kind 1 (Kerker), norm amix (L + c1) / (L + c2)
+// y[:, n] = D_dir x[:, n] for ncol columns of a real (cplx false) or
+// interleaved complex block: periodic FD first derivative along lattice
+// axis dir with coefficients d1[1..R] (R <= n_dir); complex values that wrap
+// across the upper (lower) face are multiplied by ph_hi (ph_lo)
+void scf_grad_block(bool cplx, const double *x, long ldx, double *y, long ldy, int ncol, int nx, int ny, int nz,
+                    int dir, const double *d1, int R, const double ph_lo[2], const double ph_hi[2], hipStream_t st);
 void scf_spectral_scale(double *spec, const double *sym, long n, int kind, double amix, double c1, double c2,
                         double norm, hipStream_t st);
 

@@ -204,6 +204,7 @@ struct lynx_scf {
 
     std::vector<double> eig, occ;
     double fermi = 0;
+    bool ran = false;                            // a lynx_scf_run completed
     lynx_scf_result res{};
     std::vector<double> h_e, h_err, h_t;
     std::vector<int> h_prec;
@@ -845,8 +846,135 @@ lynx_status lynx_scf_run(lynx_scf *S, lynx_scf_result *out)
         S->err = S->ctx->err;
         return st;
     }
+    S->ran = true;
     if (out) *out = S->res;
     return LYNX_OK;
+}
+
+lynx_status lynx_scf_forces(lynx_scf *S, double *forces, double *parts)
+{
+    if (!S || !forces) return LYNX_E_ARG;
+    if (!S->ran) return LYNX_E_STATE;
+    const lynx_status st = guarded(S->ctx, [&] {
+        lynx_ctx *c = S->ctx;
+        hipStream_t hs = c->stream;
+        const long n = S->nd;
+        const int ns = S->p.nspin, nk = S->p.nkpt, N = S->p.nband;
+        const ElecHost &E = electrostatics(c);
+        const Stencil &stn = c->st;
+        const int natom = S->natom;
+        // local and core-correction terms: gradients of phi and Vc on the
+        // device, the per-atom box sums on the host (Forces.cpp:186-420, 1108-1275)
+        double *g = dptr(S->grad);   // 9 Nd
+        double *vc = dptr(S->aux);
+        HIP_OK(hipMemcpyAsync(vc, E.vc.data(), n * sizeof(double), hipMemcpyHostToDevice, hs));
+        for (int a = 0; a < 3; ++a) {
+            gpu::scf_grad(dptr(S->phi), g + a * n, a, stn.nx, stn.ny, stn.nz, stn.D1[a].data(), stn.fdn, 1.0, false,
+                          hs);
+            gpu::scf_grad(vc, g + (3 + a) * n, a, stn.nx, stn.ny, stn.nz, stn.D1[a].data(), stn.fdn, 1.0, false, hs);
+        }
+        std::vector<double> hg(6 * n), vxc;
+        HIP_OK(hipMemcpyAsync(hg.data(), g, 6 * n * sizeof(double), hipMemcpyDeviceToHost, hs));
+        if (E.has_nlcc) {
+            // the first spin's Vxc, as the reference passes SCF::Vxc()
+            vxc.resize(n);
+            HIP_OK(hipMemcpyAsync(vxc.data(), S->vxc.p, n * sizeof(double), hipMemcpyDeviceToHost, hs));
+        }
+        HIP_OK(hipStreamSynchronize(hs));
+        const double *dphi[3] = {hg.data(), hg.data() + n, hg.data() + 2 * n};
+        const double *dvc[3] = {hg.data() + 3 * n, hg.data() + 4 * n, hg.data() + 5 * n};
+        std::vector<double> f_loc, f_xc;
+        local_forces(c->lat, stn, c->psps, c->atom_types, c->atom_pos, E, dphi, dvc,
+                     E.has_nlcc ? vxc.data() : nullptr, f_loc, f_xc);
+        // nonlocal term (Forces.cpp:509-800): alpha = dV Gamma <chi|psi> and
+        // beta_d = dV <chi|D_d psi> per (atom, projector, band), both with
+        // the image Bloch factors, through the apply path's gather
+        std::vector<double> f_nl(3 * (size_t)natom, 0.0);
+        if (c->has_nl && c->nl64.n_atom > 0) {
+            const gpu::NlDev &nlg = c->nl64;
+            const int npm = nlg.npm;
+            gpu::NlDev nlb = nlg;
+            DevBuf dvg;
+            upload(dvg, std::vector<double>((size_t)nlg.n_atom * npm, S->dV), hs);
+            nlb.gamma = dvg.p;
+            const int e = S->cplx ? gpu::C128 : gpu::F64;
+            const size_t es = S->cplx ? 16 : 8;
+            const size_t per_col = (size_t)nlg.n_atom * npm;
+            DevBuf dpsi, ga, gb;
+            dpsi.get((size_t)S->ldpsi * N * es);
+            ga.get(per_col * N * es);
+            gb.get(per_col * N * es);
+            const int vals = S->cplx ? 2 : 1;
+            std::vector<double> ha(per_col * N * vals), hb(per_col * N * vals);
+            const double spn_fac = (ns == 1 ? 2.0 : 1.0) * 2.0;
+            const double k0[3] = {0, 0, 0};
+            for (int s = 0; s < ns; ++s)
+                for (int k = 0; k < nk; ++k) {
+                    const int blk = s * nk + k;
+                    const double *kp = S->cplx ? &S->kpts[3 * k] : k0;
+                    if (S->cplx) {
+                        const lynx_status ks = lynx_set_kpoint(c, kp);
+                        if (ks != LYNX_OK) fail(ks, std::string("lynx_set_kpoint: ") + lynx_last_error(c));
+                    }
+                    const void *psi = static_cast<const char *>(S->psi.p) + (size_t)blk * S->ldpsi * N * es;
+                    gpu::nl_gather(e, nlg, psi, S->ldpsi, ga.p, N, hs);
+                    HIP_OK(hipMemcpyAsync(ha.data(), ga.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, hs));
+                    const gpu::StencilGeom geo = geom_at(c, kp);
+                    for (int a = 0; a < 3; ++a) {
+                        gpu::scf_grad_block(S->cplx, static_cast<const double *>(psi), S->ldpsi, dpsi.as<double>(),
+                                            S->ldpsi, N, stn.nx, stn.ny, stn.nz, a, stn.D1[a].data(), stn.fdn,
+                                            geo.phase[a][0], geo.phase[a][1], hs);
+                        gpu::nl_gather(e, nlb, dpsi.p, S->ldpsi, gb.p, N, hs);
+                        HIP_OK(hipMemcpyAsync(hb.data(), gb.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                              hs));
+                        HIP_OK(hipStreamSynchronize(hs));
+                        for (int j = 0; j < N; ++j) {
+                            const double gn = S->occ[(size_t)blk * N + j];
+                            if (std::fabs(gn) < 1e-15) continue;
+                            const double w = spn_fac * S->kw[k] * gn;
+                            for (int at = 0; at < natom; ++at) {
+                                double fj = 0.0;
+                                for (int q = 0; q < npm; ++q) {
+                                    const size_t i = ((size_t)j * per_col + (size_t)at * npm + q) * vals;
+                                    fj += vals == 2 ? ha[i] * hb[i] + ha[i + 1] * hb[i + 1] : ha[i] * hb[i];
+                                }
+                                f_nl[3 * (size_t)at + a] -= w * fj;
+                            }
+                        }
+                    }
+                }
+        }
+        // lattice-aligned components to Cartesian, F = U^-1 F_nc
+        // (Forces.cpp:132-170), then the mean removed (Forces.cpp:1277-1288)
+        auto to_cart = [&](std::vector<double> &f) {
+            if (c->lat.orth) return;
+            const double *U = c->lat.uvec_inv;
+            for (int at = 0; at < natom; ++at) {
+                const double x = f[3 * at], y = f[3 * at + 1], z = f[3 * at + 2];
+                for (int i = 0; i < 3; ++i) f[3 * at + i] = U[3 * i] * x + U[3 * i + 1] * y + U[3 * i + 2] * z;
+            }
+        };
+        std::vector<double> tot(3 * (size_t)natom);
+        for (size_t i = 0; i < tot.size(); ++i) tot[i] = f_loc[i] + f_nl[i] + f_xc[i];
+        to_cart(tot);
+        to_cart(f_loc);
+        to_cart(f_nl);
+        to_cart(f_xc);
+        for (int a = 0; a < 3; ++a) {
+            double avg = 0.0;
+            for (int at = 0; at < natom; ++at) avg += tot[3 * at + a];
+            avg /= natom;
+            for (int at = 0; at < natom; ++at) tot[3 * at + a] -= avg;
+        }
+        std::copy(tot.begin(), tot.end(), forces);
+        if (parts) {
+            std::copy(f_loc.begin(), f_loc.end(), parts);
+            std::copy(f_nl.begin(), f_nl.end(), parts + 3 * natom);
+            std::copy(f_xc.begin(), f_xc.end(), parts + 6 * natom);
+        }
+    });
+    if (st != LYNX_OK) S->err = S->ctx->err;
+    return st;
 }
 
 lynx_status lynx_scf_history(const lynx_scf *S, int cap, double *etotal, double *error, double *seconds,

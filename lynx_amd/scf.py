@@ -6,7 +6,8 @@ eigen stage).  ``Calculator`` mirrors pylynx's Calculator
 reference's JSON input keys (src/io/InputParser.cpp:68-242), ``setup()``
 builds the grid, atoms and k-points, ``run()`` runs the SCF and returns the
 total energy, with the accessors ``energy()``, ``fermi_energy()``,
-``n_iterations()``, ``is_converged()``.  Defaults follow
+``n_iterations()``, ``is_converged()``, and ``compute_forces()`` after a
+run.  Defaults follow
 ParameterDefaults.cpp:10-80.  All numerical work goes through the C ABI.
 """
 from __future__ import annotations
@@ -82,6 +83,7 @@ class SCF:
     def run(self):
         r = ScfResult()
         self._check(L.lib().lynx_scf_run(self._s, ctypes.byref(r)))
+        self.natom = r.natom
         return r.as_dict()
 
     def history(self):
@@ -101,6 +103,20 @@ class SCF:
         rho, veff = np.zeros(self.nspin * nd), np.zeros(self.nspin * nd)
         self._check(L.lib().lynx_scf_get(self._s, L.dptr(eig), L.dptr(occ), L.dptr(rho), L.dptr(veff)))
         return eig, occ, rho, veff
+
+    def forces(self, with_parts=False):
+        """Atomic forces (natom x 3, Ha/Bohr, Cartesian) of the last run's
+        state, mean removed (Forces::compute); with_parts also returns the
+        local, nonlocal and core-correction terms before the mean is
+        removed."""
+        n = getattr(self, "natom", 0)   # known after run(); before it the library reports LYNX_E_STATE
+        f = np.zeros(3 * max(n, 1))
+        parts = np.zeros(9 * max(n, 1))
+        self._check(L.lib().lynx_scf_forces(self._s, L.dptr(f), L.dptr(parts)))
+        f = f.reshape(n, 3)
+        if with_parts:
+            return f, [parts[i * 3 * n:(i + 1) * 3 * n].reshape(n, 3) for i in range(3)]
+        return f
 
     def build_potential(self, rho):
         """XC + Poisson + Veff of a given density (nspin x Nd: total, or up and down)."""
@@ -254,6 +270,13 @@ class Calculator:
                        kpt_weights=self.kw, smearing=self.smearing, cheb_degree=self.cheb_degree, **kw)
         self._result = self.scf.run()
         return self._result["etotal"]
+
+    def compute_forces(self):
+        """Atomic forces after run(), (Natom, 3) in Ha/Bohr (pylynx's
+        Calculator.compute_forces, python/src/bind_calculator.cpp:104-112)."""
+        if self._result is None:
+            raise RuntimeError("compute_forces needs a completed run()")
+        return self.scf.forces()
 
     # accessors of pylynx's Calculator
     def is_setup(self): return self._setup

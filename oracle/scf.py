@@ -16,7 +16,10 @@ checker; the product library never loads it.  numpy restatements of:
     periodic grid (fd_symbol; the library's LYNX_SOLVE_SPECTRAL);
   * Pulay mixing with the density constraint (Mixer.cpp:116-266);
   * density, energies and the SCF loop (ElectronDensity.cpp:55-140,
-    Energy.cpp:140-200, SCF.cpp:110-291, 470-790).
+    Energy.cpp:140-200, SCF.cpp:110-291, 470-790);
+  * atomic forces: local (pseudocharge), nonlocal (Kleinman-Bylander
+    projectors) and the core-correction XC term (Forces.cpp:82-420,
+    509-800, 1108-1288).
 """
 from __future__ import annotations
 
@@ -158,6 +161,47 @@ def _lap_box(V, st, R, orth, coef):
     return val
 
 
+def _d1_box(V, st, R, a):
+    """Forces.cpp:371-376: FD first derivative along lattice axis a on the
+    inner region of an extended box indexed [k, j, i]."""
+    nzp, nyp, nxp = V.shape
+    K, J, I = slice(R, nzp - R), slice(R, nyp - R), slice(R, nxp - R)
+    out = np.zeros((nzp - 2 * R, nyp - 2 * R, nxp - 2 * R))
+    d1 = st.D1[a]
+    for p in range(1, R + 1):
+        if a == 0:
+            out += (V[K, J, R + p:nxp - R + p] - V[K, J, R - p:nxp - R - p]) * d1[p]
+        elif a == 1:
+            out += (V[K, R + p:nyp - R + p, I] - V[K, R - p:nyp - R - p, I]) * d1[p]
+        else:
+            out += (V[R + p:nzp - R + p, J, I] - V[R - p:nzp - R - p, J, I]) * d1[p]
+    return out
+
+
+def grad_periodic(x, grid, st, a, kpt=None):
+    """FD first derivative along lattice axis a of a periodic field (complex:
+    Bloch field, wrapped values times e^{+-i k_a L_a}), HaloExchange.cpp
+    34-189 then Gradient::apply."""
+    X = np.asarray(x).reshape(grid.nz, grid.ny, grid.nx)
+    ax = {0: 2, 1: 1, 2: 0}[a]
+    n = X.shape[ax]
+    ph = 1.0
+    if kpt is not None:
+        ph = np.exp(1j * kpt[a] * grid.lattice.lengths[a])
+    out = np.zeros_like(X)
+    idx = np.arange(n)
+    for p in range(1, st.fdn + 1):
+        up = np.take(X, (idx + p) % n, axis=ax)
+        dn = np.take(X, (idx - p) % n, axis=ax)
+        if kpt is not None:
+            shp = [1, 1, 1]
+            shp[ax] = n
+            up = up * np.where(idx + p >= n, ph, 1.0).reshape(shp)
+            dn = dn * np.where(idx - p < 0, np.conj(ph), 1.0).reshape(shp)
+        out = out + st.D1[a][p] * (up - dn)
+    return out.ravel()
+
+
 class Electrostatics:
     """Pseudocharge, reference charge, Vc, Eself, Ec, atomic and core
     densities of the atoms at fractional coordinates (type-major order)."""
@@ -215,6 +259,50 @@ class Electrostatics:
         self.rho_at = rho_at * (self.nelectron / tot) if tot > 1e-10 else rho_at
         self.rho_core = rho_core.ravel()
         self.has_nlcc = any(p.fchrg > 1e-10 for p in psps)
+        self.grid, self.st, self.psps, self.types, self.pos = grid, st, psps, types, pos
+
+    def local_forces(self, phi, vxc0=None):
+        """Forces.cpp:186-420 and 1108-1275, per atom (lattice-aligned
+        components): -int b_J grad(phi) + (1/2) int [grad(Vc_J)(b + b_ref)
+        - grad(Vc)(b_J + b_J,ref)], and with a core density int Vxc
+        grad(rho_c,J) (the first spin's Vxc for two spins, as the
+        reference passes SCF::Vxc())."""
+        g, st = self.grid, self.st
+        lat = g.lattice
+        R = st.fdn
+        shape = (g.nz, g.ny, g.nx)
+        dphi = [grad_periodic(phi, g, st, a).reshape(shape) for a in range(3)]
+        dvc = [grad_periodic(self.vc, g, st, a).reshape(shape) for a in range(3)]
+        bb = (self.b + self.b_ref).reshape(shape)
+        coef = -0.25 / math.pi
+        natom = len(self.types)
+        f_loc = np.zeros((natom, 3))
+        f_xc = np.zeros((natom, 3))
+        for it, ia, c, s, e in self.boxes:
+            ps = self.psps[it]
+            gi = np.arange(s[0] - R, e[0] + R + 1)
+            gj = np.arange(s[1] - R, e[1] + R + 1)
+            gk = np.arange(s[2] - R, e[2] + R + 1)
+            rz, ry, rx = np.meshgrid(gk * g.h[2] - c[2], gj * g.h[1] - c[1], gi * g.h[0] - c[0], indexing="ij")
+            r = _dist(lat, rx, ry, rz)
+            VJ = v_local(ps, r.ravel()).reshape(r.shape)
+            VR = v_ref(r, RC_REF, ps.zion)
+            bj = _lap_box(VJ, st, R, lat.orthogonal, coef)
+            br = _lap_box(VR, st, R, lat.orthogonal, coef)
+            inner = (slice(s[2], e[2] + 1), slice(s[1], e[1] + 1), slice(s[0], e[0] + 1))
+            for a in range(3):
+                dvcj = _d1_box(VR - VJ, st, R, a)
+                f = -float((bj * dphi[a][inner]).sum())
+                corr = float((dvcj * bb[inner]).sum() - (dvc[a][inner] * (bj + br)).sum())
+                f_loc[ia, a] += (f + 0.5 * corr) * g.dV
+            if vxc0 is not None and ps.fchrg > 1e-10 and ps.rho_c is not None:
+                rc = np.zeros(r.shape)
+                m = r <= ps.r[-1]
+                rc[m] = _spline1(ps.r, ps.rho_c, ps.rho_c_d, r[m])
+                v = np.asarray(vxc0).reshape(shape)[inner]
+                for a in range(3):
+                    f_xc[ia, a] += float((v * _d1_box(rc, st, R, a)).sum()) * g.dV
+        return f_loc, f_xc
 
 
 # ---------------------------------------------------------------------------
@@ -625,6 +713,57 @@ class ScfOracle:
         self._it += 1
         return x
 
+    def forces(self):
+        """Forces.cpp:82-184 after run(): local + nonlocal (+ core
+        correction) in lattice-aligned components, mapped to Cartesian for
+        non-orthogonal cells (F = U^-1 F_nc), then the mean force removed
+        (Forces.cpp:1277-1288).  Returns (total, parts) with parts the
+        unsymmetrized local, nonlocal and XC terms in the same frame."""
+        g, st = self.grid, self.st
+        lat = g.lattice
+        natom = len(self.E.types)
+        vxc0 = self.vxc[0] if self.E.has_nlcc else None
+        f_loc, f_xc = self.E.local_forces(self.phi, vxc0)
+        f_nl = np.zeros((natom, 3))
+        op = self.op
+        ip = np.asarray(op.arrs["ip_displ"])
+        gam = np.asarray(op.arrs["gamma"])
+        tot = int(ip[-1])
+        occfac = 2.0 if self.nspin == 1 else 1.0
+        spn_fac = occfac * 2.0
+        nk, N = len(self.kpts), self.nband
+        for s in range(self.nspin):
+            for k, kp in enumerate(self.kpts):
+                psi = self.psis[s * nk + k]
+                occ = self.occ[(s * nk + k) * N:(s * nk + k + 1) * N]
+                cplx = kp is not None
+                kv = np.zeros(3) if kp is None else np.asarray(kp, float)
+                alpha = np.zeros((tot, N), complex if cplx else float)
+                for im in op.images:
+                    ph = np.exp(-1j * float(kv @ im.shift)) if cplx else 1.0
+                    o = ip[im.atom]
+                    alpha[o:o + im.chi.shape[1]] += ph * g.dV * (im.chi.T @ psi[im.gpos])
+                for a in range(3):
+                    D = np.stack([grad_periodic(psi[:, n], g, st, a, kv if cplx else None) for n in range(N)],
+                                 axis=1)
+                    beta = np.zeros_like(alpha)
+                    for im in op.images:
+                        ph = np.exp(-1j * float(kv @ im.shift)) if cplx else 1.0
+                        o = ip[im.atom]
+                        beta[o:o + im.chi.shape[1]] += ph * g.dV * (im.chi.T @ D[im.gpos])
+                    prod = np.real(np.conj(alpha) * beta) * gam[:, None]
+                    for at in range(natom):
+                        fj = prod[ip[at]:ip[at + 1]].sum(axis=0)
+                        f_nl[at, a] -= spn_fac * self.kw[k] * float((occ * fj).sum())
+        total = f_loc + f_nl + f_xc
+        parts = [f_loc.copy(), f_nl.copy(), f_xc.copy()]
+        if not lat.orthogonal:
+            Ui = np.linalg.inv(lat.uvec)
+            total = total @ Ui.T
+            parts = [p @ Ui.T for p in parts]
+        total = total - total.mean(axis=0)
+        return total, parts
+
     def density(self, psis, occ):
         """ElectronDensity.cpp:55-140."""
         nd, nk, N = self.grid.nd, len(self.kpts), self.nband
@@ -696,6 +835,7 @@ class ScfOracle:
             self.history.append(rec)
             if callback:
                 callback(it, rec)
+            self.psis, self.occ = psis, occ
             if it >= self.min_iter and err < self.tol:
                 rec["converged"] = True
                 return rec
