@@ -308,11 +308,12 @@ def run_mixed_si216():
     """Mixed precision on Si216: the 3x3x3 diamond supercell (30.78 Bohr),
     sigma = 0.03 Bohr (seed 3), 90^3 grid, 480 orbitals (seed 7), degree 20.
     The SCF eigen stage (lynx_scf_eig) runs iteration 0 (4 CheFSI passes from
-    the random start, Lanczos bounds) and iteration 1 (one pass) three times
+    the random start, Lanczos bounds) and iteration 1 (one pass) four times
     from the same start: FP64 throughout; the FP32 pass (filter, Gram
     products and rotations in FP32, Cholesky and eigensolve in FP64) while
-    the SCF residual exceeds 1e-4 (residual 1e-3 given for iteration 1); and
-    the same with the filter's blocks stored in bfloat16.  The band-energy
+    the SCF residual exceeds 1e-4 (residual 1e-3 given for iteration 1); the
+    same with the Gram products as split-BF16x3 matrix-core products; and
+    that with the filter's blocks stored in bfloat16.  The band-energy
     difference per atom uses the occupations of the FP64 run (Fermi-Dirac,
     1000 K).  The first stage also pays the launch-shape tuning."""
     from systems import si216_mixed
@@ -324,10 +325,11 @@ def run_mixed_si216():
     X0 = rng.uniform(-0.5, 0.5, (nb, s.nd))
     out = {}
     eigs = {}
-    modes = (("fp64", 0.0, "fp32"), ("fp32_until_1e-4", 1e-4, "fp32"), ("bf16_until_1e-4", 1e-4, "bf16"))
-    for name, thr, low in modes:
+    modes = (("fp64", 0.0, "fp32", "fp32"), ("fp32_until_1e-4", 1e-4, "fp32", "fp32"),
+             ("fp32_bf16x3_gram_until_1e-4", 1e-4, "fp32", "bf16x3"), ("bf16_until_1e-4", 1e-4, "bf16", "bf16x3"))
+    for name, thr, low, gram in modes:
         H = s.hamiltonian(0)
-        drv = core.ScfEigenStage(H, nb, ne, beta, fp64_switch_residual=thr, low_precision=low)
+        drv = core.ScfEigenStage(H, nb, ne, beta, fp64_switch_residual=thr, low_precision=low, gram=gram)
         d = DeviceArray.from_numpy(X0)
         synchronize()
         t0 = time.perf_counter()
@@ -348,8 +350,10 @@ def run_mixed_si216():
     occ_n = int(ne // 2)
     res = {"workload": "Si216 3x3x3 supercell a=30.78 Bohr, sigma=0.03 (seed 3), 90^3 grid, 480 orbitals, degree 20; "
                        "SCF iteration 0 (4 passes + Lanczos) and 1 (1 pass) of the eigen stage",
-           "fp64": out["fp64"], "mixed": out["fp32_until_1e-4"], "mixed_bf16": out["bf16_until_1e-4"]}
-    for key, name in (("", "fp32_until_1e-4"), ("_bf16", "bf16_until_1e-4")):
+           "fp64": out["fp64"], "mixed": out["fp32_until_1e-4"], "mixed_bf16x3_gram": out["fp32_bf16x3_gram_until_1e-4"],
+           "mixed_bf16": out["bf16_until_1e-4"]}
+    for key, name in (("", "fp32_until_1e-4"), ("_bf16x3_gram", "fp32_bf16x3_gram_until_1e-4"),
+                      ("_bf16", "bf16_until_1e-4")):
         emx, _ = eigs[name]
         res["speedup_iter1" + key] = round(out["fp64"]["iter1_s"] / out[name]["iter1_s"], 3)
         res["speedup_iter0" + key] = round(out["fp64"]["iter0_s"] / out[name]["iter0_s"], 3)
@@ -420,10 +424,11 @@ def run_scf_si8(with_cpu):
 def run_scf_si216():
     """SCF wall-time on Si216 (the 3x3x3 supercell, a = 30.78 Bohr,
     sigma = 0.03 Bohr seed 3, 90^3 grid, 480 states, GGA-PBE at Gamma,
-    Fermi-Dirac 1000 K, degree 20, to a 1e-6 density residual) in three
+    Fermi-Dirac 1000 K, degree 20, to a 1e-6 density residual) in four
     modes: FP64 throughout; FP32 passes (filter, Gram products, rotation)
-    until the residual falls below 1e-4, FP64 after; the same with the
-    filter's blocks stored in bfloat16.  Each mode first runs one untimed
+    until the residual falls below 1e-4, FP64 after; the same with
+    split-BF16x3 Gram products; and that with the filter's blocks stored in
+    bfloat16.  Each mode first runs one untimed
     iteration so the launch shapes it uses are tuned.  Errors are against
     the FP64 run: total energy per atom, occupied eigenvalues and atomic
     forces (lynx_scf_forces after each run, timed)."""
@@ -433,10 +438,11 @@ def run_scf_si216():
     s = si216_mixed()
     H = s.hamiltonian(0)
     beta = 1.0 / (KB_HA * 1000.0)
-    modes = (("fp64", 0.0, "fp32"), ("fp32_until_1e-4", 1e-4, "fp32"), ("bf16_until_1e-4", 1e-4, "bf16"))
+    modes = (("fp64", 0.0, "fp32", "fp32"), ("fp32_until_1e-4", 1e-4, "fp32", "fp32"),
+             ("fp32_bf16x3_gram_until_1e-4", 1e-4, "fp32", "bf16x3"), ("bf16_until_1e-4", 1e-4, "bf16", "bf16x3"))
     out, res = {}, {}
-    for name, thr, low in modes:
-        kw = dict(xc="GGA_PBE", tol=1e-6, fp64_switch_residual=thr, low_precision=low)
+    for name, thr, low, gram in modes:
+        kw = dict(xc="GGA_PBE", tol=1e-6, fp64_switch_residual=thr, low_precision=low, gram=gram)
         warm = SCF(H, 480, beta, max_iter=1, **kw)
         warm.run()
         warm.close()
@@ -463,13 +469,66 @@ def run_scf_si216():
     summary = {"workload": "Si216 3x3x3 supercell a=30.78 Bohr, sigma=0.03 (seed 3), 90^3, GGA-PBE, Gamma, "
                            "480 states, Fermi-Dirac 1000 K, degree 20, SCF to 1e-6 density residual; C.pot "
                            "stands in for Si", "modes": out}
-    for name in ("fp32_until_1e-4", "bf16_until_1e-4"):
+    for name in ("fp32_until_1e-4", "fp32_bf16x3_gram_until_1e-4", "bf16_until_1e-4"):
         r, e, _, F = res[name]
         summary[name] = {"speedup_scf_wall": round(out["fp64"]["scf_wall_s"] / out[name]["scf_wall_s"], 3),
                          "etotal_err_per_atom_Ha": abs(r["etotal"] - r64["etotal"]) / 216,
                          "max_abs_occupied_eig_err_Ha": float(np.abs(e - e64)[occ_mask].max()),
                          "max_force_err_Ha_per_Bohr": float(np.abs(F - F64).max())}
     return summary
+
+
+BF16_MFMA_PEAK_TFLOPS, FP32_PEAK_TFLOPS, FP64_PEAK_TFLOPS = 2500.0, 157.3, 78.6
+
+
+def run_gram_si216(reps=5):
+    """The overlap product dV X^T X of the Si216 pass (729,000 grid points x
+    480 orbitals, uniform random blocks resident in HBM) through lynx_gram:
+    FP64 (rocBLAS split-K dgemm), FP32 (rocBLAS split-K sgemm) and
+    split-BF16x3 (the hand-written matrix-core kernel).  Best of reps calls
+    after one untimed call; each call includes the 1.8 MB result copy to
+    the host.  Effective rate 2 K N^2 / t; for split-BF16x3 also the rate of
+    its three bfloat16 products against the dense bf16 peak."""
+    from lynx_amd import _lib as L
+    from lynx_amd.hipmem import DeviceArray
+    from systems import si216_mixed
+    s = si216_mixed()
+    H = s.hamiltonian(0)
+    nd, n = s.nd, 480
+    ld = ((nd + 63) // 64) * 64
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-0.5, 0.5, (n, ld))
+    d64 = DeviceArray.from_numpy(X)
+    d32 = DeviceArray.from_numpy(X.astype(np.float32))
+    C = np.zeros((n, n))
+    out = {}
+    flops = 2.0 * nd * n * n
+    ref = None
+    for name, dt, mode, buf in (("fp64", L.F64, L.GRAM_FP32, d64), ("fp32_gemm", L.F32, L.GRAM_FP32, d32),
+                                ("split_bf16x3", L.F32, L.GRAM_BF16X3, d32)):
+        L.check(L.lib().lynx_set_gram_precision(H.ctx, mode), H.ctx)
+        best = 1e9
+        for i in range(reps + 1):
+            t0 = time.perf_counter()
+            L.check(L.lib().lynx_gram(H.ctx, dt, buf.value, ld, buf.value, ld, n, L.dptr(C), L.MEM_DEVICE), H.ctx)
+            dt_s = time.perf_counter() - t0
+            if i > 0:
+                best = min(best, dt_s)
+        r = {"ms": round(best * 1e3, 3), "effective_tflops": round(flops / best / 1e12, 1)}
+        if name == "fp64":
+            ref = C.copy()
+            r["frac_of_fp64_peak"] = round(flops / best / 1e12 / FP64_PEAK_TFLOPS, 3)
+        else:
+            r["max_rel_err_vs_fp64"] = float(np.abs(C - ref).max() / np.abs(ref).max())
+        if name == "fp32_gemm":
+            r["frac_of_fp32_peak"] = round(flops / best / 1e12 / FP32_PEAK_TFLOPS, 3)
+        if name == "split_bf16x3":
+            r["bf16_mfma_tflops"] = round(3 * flops / best / 1e12, 1)
+            r["frac_of_bf16_mfma_peak"] = round(3 * flops / best / 1e12 / BF16_MFMA_PEAK_TFLOPS, 3)
+        out[name] = r
+    d64.free(); d32.free()
+    out["workload"] = "dV X^T X, X 729,000 x 480 (Si216 90^3 grid, 480 orbitals), device-resident"
+    return out
 
 
 def stream_ceiling(sysm):
@@ -562,6 +621,8 @@ def main():
         variants["scf_si8"] = run_scf_si8(not args.no_cpu_baseline)
         print(f"[bench] Si8 SCF: {variants['scf_si8']}", file=sys.stderr, flush=True)
     if not args.no_si216:
+        variants["gram_si216"] = run_gram_si216()
+        print(f"[bench] Si216 Gram: {variants['gram_si216']}", file=sys.stderr, flush=True)
         variants["si216_mixed_precision"] = run_mixed_si216()
         print(f"[bench] Si216 mixed: {variants['si216_mixed_precision']}", file=sys.stderr)
         if not args.no_scf:

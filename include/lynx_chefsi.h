@@ -275,6 +275,28 @@ lynx_status lynx_stream_ceiling(lynx_ctx *ctx, long nd, int ncol, double *gbps);
 enum lynx_filter_precision { LYNX_FILTER_FP64 = 0, LYNX_FILTER_FP32 = 1, LYNX_FILTER_BF16 = 2 };
 lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision);
 lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision);
+/* Single-precision Gram products of the reduced-precision pass (the overlap
+ * dV X^T X and the projection dV X^T H X of real blocks).  LYNX_GRAM_FP32
+ * runs them as FP32 GEMMs (rocBLAS) with FP64 chunk sums.  LYNX_GRAM_BF16X3
+ * runs them as split-BF16x3 products on the matrix cores: each operand split
+ * into a bfloat16 head and tail, hi.hi + hi.lo + lo.hi with FP32
+ * accumulation per K chunk, chunks summed in FP64 (about 4e-6 relative
+ * against FP64; FP32 GEMMs about 2e-7).  Complex blocks, and real blocks
+ * whose columns are not 16-byte aligned, use FP32 GEMMs either way.  No
+ * reference counterpart: the reference forms these products in FP64
+ * (EigenSolver.cpp:621-660). */
+enum lynx_gram_precision { LYNX_GRAM_FP32 = 0, LYNX_GRAM_BF16X3 = 1 };
+lynx_status lynx_set_gram_precision(lynx_ctx *ctx, int mode);
+lynx_status lynx_get_gram_precision(const lynx_ctx *ctx, int *mode);
+/* c = dV a^H b (n x n, FP64 for real dtypes, complex128 for complex ones,
+ * written to host memory) for two blocks of n columns (leading dimensions
+ * >= Nd, in host or device memory per mem) on the context's grid, through
+ * the product the solve uses for dtype: FP64 / complex128 GEMMs, or for
+ * single precision the FP32 GEMMs or split-BF16x3 products selected by
+ * lynx_set_gram_precision.  The overlap product of EigenSolver::orthogonalize
+ * and project_hamiltonian (EigenSolver.cpp:621-660). */
+lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const void *b, int ldb, int n, void *c,
+                      int mem);
 /* Number of lynx_set_veff calls made on the context so far (accepted or
  * refused).  A caller that skips re-uploading an unchanged potential keeps
  * the generation of its own last upload and uploads again when another
@@ -346,6 +368,8 @@ typedef struct {
     int smearing;                 /* 0 Fermi-Dirac, 1 Gaussian */
     double fp64_switch_residual;  /* 0: FP64 filter throughout */
     int low_precision;            /* before the switch: LYNX_FILTER_FP32 (or 0) or LYNX_FILTER_BF16 */
+    int gram_precision;           /* Gram products of the reduced-precision passes: LYNX_GRAM_FP32 (0) or
+                                     LYNX_GRAM_BF16X3 (split-BF16x3 on the matrix cores) */
 } lynx_scf_eig_params;
 lynx_status lynx_scf_eig_create(lynx_scf_eig **driver, lynx_ctx *ctx, const lynx_scf_eig_params *params,
                                 const double *kpts, const double *kpt_weights);
@@ -426,6 +450,7 @@ typedef struct {
     double fp64_switch_residual;  /* as lynx_scf_eig_params: reduced-precision passes above it */
     int low_precision;            /* LYNX_FILTER_FP32 or LYNX_FILTER_BF16 */
     int linear_solver;            /* LYNX_SOLVE_SPECTRAL (0) or LYNX_SOLVE_AAR (1) */
+    int gram_precision;           /* as lynx_scf_eig_params */
 } lynx_scf_params;
 typedef struct {
     double etotal, eband, exc, ehart, eself, ec, entropy;   /* Ha (Energy::compute_all) */

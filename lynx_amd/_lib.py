@@ -18,6 +18,7 @@ F64, C128, F32, C64, BF16 = 0, 1, 2, 3, 4
 FILTER_FP64, FILTER_FP32, FILTER_BF16 = 0, 1, 2
 XC_LDA_PZ, XC_LDA_PW, XC_GGA_PBE = 0, 1, 2
 SOLVE_SPECTRAL, SOLVE_AAR = 0, 1
+GRAM_FP32, GRAM_BF16X3 = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 CELL_ORTHOGONAL, CELL_NONORTHOGONAL = 0, 1
 
@@ -63,6 +64,9 @@ SIGNATURES = {
     "lynx_mixed_fallbacks": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L)]),
     "lynx_set_filter_precision": (_I, [_P, _I]),
     "lynx_get_filter_precision": (_I, [_P, _IP]),
+    "lynx_set_gram_precision": (_I, [_P, _I]),
+    "lynx_get_gram_precision": (_I, [_P, _IP]),
+    "lynx_gram": (_I, [_P, _I, _P, _I, _P, _I, _I, _P, _I]),
     "lynx_get_veff_generation": (_I, [_P, ctypes.POINTER(_L)]),
     "lynx_solve_count": (_I, [_P, ctypes.POINTER(_L)]),
     "lynx_fermi_level": (_I, [_I, _I, _I, _DP, _DP, _D, _D, _I, _DP, _DP]),

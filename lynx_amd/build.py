@@ -27,6 +27,7 @@ SOURCES = [
     ("capi.cpp", "hip"),
     ("kernels.hip", "hip"),
     ("kernels_scf.hip", "hip"),
+    ("kernels_mfma.hip", "hip"),
     ("scf.cpp", "hip"),
 ]
 

@@ -490,7 +490,7 @@ class _ScfEigParams(ctypes.Structure):
                 ("nband", ctypes.c_int), ("cheb_degree", ctypes.c_int), ("rho_trigger", ctypes.c_int),
                 ("nchefsi", ctypes.c_int), ("nelectron", ctypes.c_double), ("beta", ctypes.c_double),
                 ("smearing", ctypes.c_int), ("fp64_switch_residual", ctypes.c_double),
-                ("low_precision", ctypes.c_int)]
+                ("low_precision", ctypes.c_int), ("gram_precision", ctypes.c_int)]
 
 
 class ScfEigenStage:
@@ -500,10 +500,12 @@ class ScfEigenStage:
     block and returns eigenvalues, occupations and the Fermi level.  With
     ``fp64_switch_residual`` > 0 the pass runs in reduced precision
     (``low_precision``: "fp32", or "bf16" for bfloat16 filter storage) until
-    the SCF residual drops below it (lynx_scf_eig_*)."""
+    the SCF residual drops below it, with the Gram products of those passes
+    as FP32 GEMMs or split-BF16x3 MFMA products (``gram``: "fp32" or
+    "bf16x3") (lynx_scf_eig_*)."""
 
     def __init__(self, H, nband, nelectron, beta, kpts=None, kpt_weights=None, nspin=1, cheb_degree=20,
-                 rho_trigger=4, nchefsi=1, smearing=0, fp64_switch_residual=0.0, low_precision="fp32"):
+                 rho_trigger=4, nchefsi=1, smearing=0, fp64_switch_residual=0.0, low_precision="fp32", gram="fp32"):
         self.H = H
         self.kpts = None if kpts is None else np.ascontiguousarray(kpts, float).reshape(-1, 3)
         nk = 1 if self.kpts is None else len(self.kpts)
@@ -511,7 +513,8 @@ class ScfEigenStage:
         self.complex_ = self.kpts is not None
         p = _ScfEigParams(nspin, nk, int(self.complex_), nband, cheb_degree, rho_trigger, nchefsi,
                           float(nelectron), float(beta), smearing, float(fp64_switch_residual),
-                          {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision])
+                          {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision],
+                          {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3}[gram])
         kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
         self._d = ctypes.c_void_p()
         L.check(L.lib().lynx_scf_eig_create(ctypes.byref(self._d), H.ctx, ctypes.byref(p),

@@ -570,6 +570,15 @@ constexpr double kCholQrFp64Ratio = 100.0;
 void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, int n, double alpha, void *C)
 {
     const long nd = c->nd;
+    if (e == gpu::F32 && c->gram_precision == LYNX_GRAM_BF16X3 && lda % 4 == 0 && ldb % 4 == 0 &&
+        reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) {
+        const int ns = gpu::gram_bf16x3_splits(nd, n);
+        void *W = c->gwork.get((size_t)ns * n * n * sizeof(float));
+        const int used = gpu::gram_bf16x3(static_cast<const float *>(A), lda, static_cast<const float *>(B), ldb, nd,
+                                          n, static_cast<float *>(W), ns, c->stream);
+        gpu::gram_reduce(e, W, used, n, alpha, C, c->stream);
+        return;
+    }
     const bool single = e == gpu::F32 || e == gpu::C64;
     const int ks = (int)std::max<long>(1, std::min<long>(single ? 64 : 16, nd / 8192));
     const long kch = nd / ks;
@@ -1666,6 +1675,63 @@ lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision)
     }
     ctx->filter_precision = precision;
     return LYNX_OK;
+}
+
+lynx_status lynx_set_gram_precision(lynx_ctx *ctx, int mode)
+{
+    if (!ctx) return LYNX_E_ARG;
+    if (mode != LYNX_GRAM_FP32 && mode != LYNX_GRAM_BF16X3) {
+        ctx->err = "Gram precision must be LYNX_GRAM_FP32 or LYNX_GRAM_BF16X3";
+        return LYNX_E_ARG;
+    }
+    ctx->gram_precision = mode;
+    return LYNX_OK;
+}
+
+lynx_status lynx_get_gram_precision(const lynx_ctx *ctx, int *mode)
+{
+    if (!ctx || !mode) return LYNX_E_ARG;
+    *mode = ctx->gram_precision;
+    return LYNX_OK;
+}
+
+lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const void *b, int ldb, int n, void *c,
+                      int mem)
+{
+    if (!ctx || !a || !b || !c || n < 1) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_grid(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128 && dtype != LYNX_F32 && dtype != LYNX_C64)
+            fail(LYNX_E_ARG, "lynx_gram: dtype must be f64, c128, f32 or c64");
+        if (lda < ctx->nd || ldb < ctx->nd) fail(LYNX_E_ARG, "lynx_gram: leading dimension below Nd");
+        const int e = dtype;
+        const size_t es = gpu::elem_size(e);
+        const void *da = a, *db = b;
+        if (mem == LYNX_MEM_HOST) {
+            // padded device copies (16-byte aligned columns)
+            const long ld = padded_ld(ctx->nd);
+            void *A = ctx->wA.get((size_t)ld * n * es);
+            HIP_OK(hipMemcpy2DAsync(A, ld * es, a, (size_t)lda * es, ctx->nd * es, n, hipMemcpyHostToDevice,
+                                    ctx->stream));
+            void *B = A;
+            if (b != a) {
+                B = ctx->wB.get((size_t)ld * n * es);
+                HIP_OK(hipMemcpy2DAsync(B, ld * es, b, (size_t)ldb * es, ctx->nd * es, n, hipMemcpyHostToDevice,
+                                        ctx->stream));
+            }
+            da = A;
+            db = B;
+            lda = ldb = (int)ld;
+        } else if (mem != LYNX_MEM_DEVICE) {
+            fail(LYNX_E_ARG, "lynx_gram: mem must be LYNX_MEM_HOST or LYNX_MEM_DEVICE");
+        }
+        const int ew = wide_of(e);
+        const size_t ews = gpu::elem_size(ew);
+        void *C = ctx->s2.get((size_t)n * n * ews);
+        gram(ctx, e, da, lda, db, ldb, n, ctx->st.dV, C);
+        HIP_OK(hipMemcpyAsync(c, C, (size_t)n * n * ews, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
 }
 
 lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision)

@@ -122,6 +122,7 @@ struct lynx_ctx {
 
     // precision of the filter inside lynx_chefsi_solve (LYNX_FILTER_*)
     int filter_precision = 0;
+    int gram_precision = 0;            // LYNX_GRAM_* for single-precision real Gram products
     long solves = 0;                   // completed lynx_chefsi_solve calls
     long fp64_overlap_fallbacks = 0;   // FP32 passes redone in FP64
     long cholqr2_passes = 0;           // FP32 passes with a second Cholesky-QR

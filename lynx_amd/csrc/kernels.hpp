@@ -109,6 +109,15 @@ int batch_launches_per_step(const BatchPlan *p);
 // (fixed order).  C is FP64 (F64 or C128) whatever the precision e of the
 // partials: single-precision partials are summed in double precision.
 void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void *C, hipStream_t st);
+// Split-BF16x3 Gram partials on the matrix cores (kernels_mfma.hip): for
+// real single-precision column-major A, B (K x N, leading dimensions and
+// base addresses 16-byte aligned), part[s] (N x N, FP32) = A_s^T B_s over
+// consecutive K chunks, each operand split into bfloat16 hi + lo and the
+// product taken as hi.hi + hi.lo + lo.hi with FP32 accumulation.  At most
+// nsplit chunks; returns the number written (sum them with gram_reduce).
+int gram_bf16x3(const float *A, long lda, const float *B, long ldb, long K, int N, float *part, int nsplit,
+                hipStream_t st);
+int gram_bf16x3_splits(long K, int N);
 // Small dense helpers on n x n column-major matrices.
 void hermitian_part(int e, void *H, int n, hipStream_t st);
 void zero_lower(int e, void *H, int n, hipStream_t st);

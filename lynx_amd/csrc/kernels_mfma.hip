@@ -1,0 +1,244 @@
+// Split-BF16x3 Gram products on the CDNA4 matrix cores.
+//
+// C = A^T B for real single-precision blocks A, B (K x N, column-major,
+// K = Nd grid points, N orbitals).  Each operand is split into a bfloat16
+// head and a bfloat16 tail, x = hi + lo (hi = bf16(x), lo = bf16(x - hi)),
+// and the product is hi.hi + hi.lo + lo.hi, three v_mfma_f32_32x32x16_bf16
+// per 16-deep step with FP32 accumulation (the lo.lo term, 2^-16 below, is
+// dropped).  K is split into chunks; each chunk's FP32 partial product is
+// written out and the chunks are summed in FP64 in a fixed order by
+// gram_reduce, so no FP32 sum runs over more than one chunk.
+//
+// Work decomposition: a workgroup of five waves owns a 160 x 160 tile of C
+// and one K chunk (for A^T A only the tiles on and above the diagonal,
+// written to both halves).  Per 32-deep block it stages the tile's 160
+// columns of A and of B as (hi, lo) bfloat16 pairs in LDS, column-major with
+// k contiguous, the split done once per element by the loading thread, and
+// the next block's loads are in flight while the current block's products
+// run.  Wave w computes rows 32w..32w+31 of the tile against all five
+// 32-column blocks: its A fragment is read once per step and reused by five
+// products.
+//
+// Operand layout of v_mfma_f32_32x32x16_bf16 (checked on gfx950): lane l
+// supplies A[l % 32][8 (l / 32) + i] and B[8 (l / 32) + i][l % 32],
+// i = 0..7, and holds D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32], j = 0..15.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace lynx {
+namespace gpu {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTile = 160;            // C tile edge
+constexpr int kWaves = kTile / 32;    // one 32-row strip per wave
+constexpr int kThreads = 64 * kWaves;
+constexpr int kKB = 32;               // K rows per LDS block
+constexpr int kPitch = kKB + 8;       // bf16 per LDS column (padded against bank conflicts)
+
+// round to nearest even
+__device__ inline unsigned short bf16_bits(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_value(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// This thread's share of a 32-deep block of a 160-column tile: 16
+// consecutive K rows (from k0 + koff, masked by kend and the column count)
+// of column c = t / 2.
+struct Slab {
+    float v[16];
+};
+
+__device__ inline void load_slab(const float *__restrict__ X, long ldx, long k0, long kend, int col0, int N, int t,
+                                 Slab &s)
+{
+    const int c = t >> 1;
+    const int koff = (t & 1) * 16;
+    const int gc = col0 + c;
+    const long r0 = k0 + koff;
+    if (gc < N && r0 + 16 <= kend) {
+        const float4 *p = reinterpret_cast<const float4 *>(X + (long)gc * ldx + r0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 f = p[q];
+            s.v[4 * q] = f.x; s.v[4 * q + 1] = f.y; s.v[4 * q + 2] = f.z; s.v[4 * q + 3] = f.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s.v[i] = (gc < N && r0 + i < kend) ? X[(long)gc * ldx + r0 + i] : 0.0f;
+    }
+}
+
+// Splits the slab into bfloat16 hi and lo and stores both into LDS.
+__device__ inline void store_slab(const Slab &s, unsigned short *hi, unsigned short *lo, int t)
+{
+    const int c = t >> 1;
+    const int koff = (t & 1) * 16;
+    unsigned short h[16], l[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        h[i] = bf16_bits(s.v[i]);
+        l[i] = bf16_bits(s.v[i] - bf16_value(h[i]));
+    }
+    uint4 *dh = reinterpret_cast<uint4 *>(hi + c * kPitch + koff);
+    uint4 *dl = reinterpret_cast<uint4 *>(lo + c * kPitch + koff);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint4 a, b;
+        a.x = h[8 * q] | ((unsigned)h[8 * q + 1] << 16);
+        a.y = h[8 * q + 2] | ((unsigned)h[8 * q + 3] << 16);
+        a.z = h[8 * q + 4] | ((unsigned)h[8 * q + 5] << 16);
+        a.w = h[8 * q + 6] | ((unsigned)h[8 * q + 7] << 16);
+        b.x = l[8 * q] | ((unsigned)l[8 * q + 1] << 16);
+        b.y = l[8 * q + 2] | ((unsigned)l[8 * q + 3] << 16);
+        b.z = l[8 * q + 4] | ((unsigned)l[8 * q + 5] << 16);
+        b.w = l[8 * q + 6] | ((unsigned)l[8 * q + 7] << 16);
+        dh[q] = a;
+        dl[q] = b;
+    }
+}
+
+__device__ inline bf16x8 frag(const unsigned short *s, int col, int k)
+{
+    return *reinterpret_cast<const bf16x8 *>(s + col * kPitch + k);
+}
+
+// Tile pair of workgroup b: all tiles_n^2 pairs, or for A^T A only the
+// pairs tm <= tn (row-major over the upper triangle); the lower tiles of
+// A^T A are written as the transposes of the upper ones.
+__device__ inline void tile_of(int b, int tiles_n, int same, int &tm, int &tn)
+{
+    if (!same) {
+        tm = b / tiles_n;
+        tn = b % tiles_n;
+        return;
+    }
+    tm = 0;
+    int rem = b;
+    while (rem >= tiles_n - tm) {
+        rem -= tiles_n - tm;
+        ++tm;
+    }
+    tn = tm + rem;
+}
+
+// One-dimensional grid of 8 * per_xcd workgroups over items = pairs x
+// chunks (item = chunk * pairs + pair; chunk s covers rows
+// [s * kch, min(K, (s + 1) * kch))).  Workgroup b runs on XCD b mod 8 and
+// takes item (b mod 8) * per_xcd + b / 8, so consecutive items, the tiles
+// of one chunk, share an XCD's L2; workgroups past the last item return.
+// The next block's global loads are issued before the current block's
+// products, so they overlap.
+__global__ void __launch_bounds__(kThreads) k_gram_bf16x3(const float *__restrict__ A, long lda,
+                                                         const float *__restrict__ B, long ldb, long K, int N,
+                                                         long kch, int tiles_n, int same, int pairs, int items,
+                                                         int per_xcd, float *__restrict__ part)
+{
+    const int item = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (item >= items) return;
+    const int chunk = item / pairs, pair = item % pairs;
+    __shared__ __attribute__((aligned(16))) unsigned short sAh[kTile * kPitch], sAl[kTile * kPitch];
+    __shared__ __attribute__((aligned(16))) unsigned short sBh[kTile * kPitch], sBl[kTile * kPitch];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    int tm, tn;
+    tile_of(pair, tiles_n, same, tm, tn);
+    const int m0 = tm * kTile, n0 = tn * kTile;
+    const long kbeg = (long)chunk * kch, kend = std::min(K, kbeg + kch);
+    const bool one = same && tm == tn;          // B's tile is A's
+    const unsigned short *bh = one ? sAh : sBh;
+    const unsigned short *bl = one ? sAl : sBl;
+    const float *Bsrc = same ? A : B;
+    const long ldbs = same ? lda : ldb;
+    floatx16 acc[kWaves];
+#pragma unroll
+    for (int j = 0; j < kWaves; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
+    const int row = w * 32 + (l & 31), kl = 8 * (l >> 5);
+    Slab sa, sb;
+    if (kbeg < kend) {
+        load_slab(A, lda, kbeg, kend, m0, N, t, sa);
+        if (!one) load_slab(Bsrc, ldbs, kbeg, kend, n0, N, t, sb);
+    }
+    for (long k0 = kbeg; k0 < kend; k0 += kKB) {
+        store_slab(sa, sAh, sAl, t);
+        if (!one) store_slab(sb, sBh, sBl, t);
+        __syncthreads();
+        if (k0 + kKB < kend) {
+            load_slab(A, lda, k0 + kKB, kend, m0, N, t, sa);
+            if (!one) load_slab(Bsrc, ldbs, k0 + kKB, kend, n0, N, t, sb);
+        }
+#pragma unroll
+        for (int ks = 0; ks < kKB; ks += 16) {
+            const bf16x8 ah = frag(sAh, row, ks + kl), al = frag(sAl, row, ks + kl);
+#pragma unroll
+            for (int j = 0; j < kWaves; ++j) {
+                const bf16x8 b_h = frag(bh, j * 32 + (l & 31), ks + kl);
+                const bf16x8 b_l = frag(bl, j * 32 + (l & 31), ks + kl);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b_h, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b_l, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b_h, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float *P = part + (size_t)chunk * N * N;
+    const bool mirror = same && tm != tn;
+#pragma unroll
+    for (int j = 0; j < kWaves; ++j) {
+        const int n = n0 + j * 32 + (l & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = m0 + w * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+            if (m < N && n < N) {
+                P[m + (size_t)n * N] = acc[j][r];
+                if (mirror) P[n + (size_t)m * N] = acc[j][r];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int gram_bf16x3_splits(long K, int N)
+{
+    const int tiles = (N + kTile - 1) / kTile;
+    const int pairs = tiles * (tiles + 1) / 2;   // the symmetric product's tile count
+    // about three resident workgroups per CU on 256 CUs, chunks of at
+    // least one LDS block
+    const long want = std::max(1L, (768L + pairs - 1) / pairs);
+    return (int)std::max(1L, std::min(want, (K + kKB - 1) / kKB));
+}
+
+int gram_bf16x3(const float *A, long lda, const float *B, long ldb, long K, int N, float *part, int nsplit,
+                hipStream_t st)
+{
+    if (N < 1 || K < 1 || nsplit < 1) throw std::runtime_error("gram_bf16x3: empty product");
+    if ((lda % 4) || (ldb % 4) || (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
+        throw std::runtime_error("gram_bf16x3: columns must be 16-byte aligned");
+    const int tiles = (N + kTile - 1) / kTile;
+    const int same = A == B && lda == ldb ? 1 : 0;
+    const int pairs = same ? tiles * (tiles + 1) / 2 : tiles * tiles;
+    // chunk length: a multiple of the LDS block, so 16-byte loads stay aligned
+    long kch = (K + nsplit - 1) / nsplit;
+    kch = (kch + kKB - 1) / kKB * kKB;
+    const int used = (int)((K + kch - 1) / kch);
+    const int items = pairs * used, per_xcd = (items + 7) / 8;
+    k_gram_bf16x3<<<8 * per_xcd, kThreads, 0, st>>>(A, lda, B, ldb, K, N, kch, tiles, same, pairs, items, per_xcd,
+                                                    part);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("k_gram_bf16x3: ") + hipGetErrorString(e));
+    return used;
+}
+
+}  // namespace gpu
+}  // namespace lynx

@@ -650,6 +650,7 @@ lynx_status lynx_scf_create(lynx_scf **out, lynx_ctx *ctx, const lynx_scf_params
         ep.smearing = p->smearing;
         ep.fp64_switch_residual = p->fp64_switch_residual;
         ep.low_precision = p->low_precision;
+        ep.gram_precision = p->gram_precision;
         const lynx_status es = lynx_scf_eig_create(&S->stage, ctx, &ep, S->kpts.data(), S->kw.data());
         if (es != LYNX_OK) fail(es, "lynx_scf_eig_create failed");
         HIP_OK(hipStreamSynchronize(hs));

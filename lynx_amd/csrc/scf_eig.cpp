@@ -174,15 +174,20 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
                                    double *fermi, int *precision_used)
 {
     if (!d || !veff || !psi || !eig) return LYNX_E_ARG;
-    // the context's filter precision is the caller's setting: the step sets
-    // its own and restores the caller's on every return
-    int caller_precision = LYNX_FILTER_FP64;
+    // the context's filter and Gram precisions are the caller's settings:
+    // the step sets its own and restores the caller's on every return
+    int caller_precision = LYNX_FILTER_FP64, caller_gram = LYNX_GRAM_FP32;
     if (lynx_get_filter_precision(d->ctx, &caller_precision) != LYNX_OK) return LYNX_E_ARG;
+    if (lynx_get_gram_precision(d->ctx, &caller_gram) != LYNX_OK) return LYNX_E_ARG;
     struct Restore {
         lynx_ctx *c;
-        int p;
-        ~Restore() { (void)lynx_set_filter_precision(c, p); }
-    } restore{d->ctx, caller_precision};
+        int p, g;
+        ~Restore()
+        {
+            (void)lynx_set_filter_precision(c, p);
+            (void)lynx_set_gram_precision(c, g);
+        }
+    } restore{d->ctx, caller_precision, caller_gram};
     try {
         const lynx_scf_eig_params &p = d->p;
         const int dtype = p.complex_kpt ? LYNX_C128 : LYNX_F64;
@@ -203,6 +208,7 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
         d->last_precision = !single ? LYNX_FILTER_FP64
                                     : (p.low_precision == LYNX_FILTER_BF16 ? LYNX_FILTER_BF16 : LYNX_FILTER_FP32);
         ok(lynx_set_filter_precision(d->ctx, d->last_precision), d, "lynx_set_filter_precision");
+        ok(lynx_set_gram_precision(d->ctx, p.gram_precision), d, "lynx_set_gram_precision");
         const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
         for (int pass = 0; pass < passes; ++pass) {
             for (int s = 0; s < p.nspin; ++s) {

@@ -35,7 +35,7 @@ class ScfParams(ctypes.Structure):
                 ("min_iter", ctypes.c_int), ("poisson_tol", ctypes.c_double), ("precond_tol", ctypes.c_double),
                 ("mixing_history", ctypes.c_int), ("mixing_param", ctypes.c_double), ("kerker", ctypes.c_int),
                 ("fp64_switch_residual", ctypes.c_double), ("low_precision", ctypes.c_int),
-                ("linear_solver", ctypes.c_int)]
+                ("linear_solver", ctypes.c_int), ("gram_precision", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -55,12 +55,13 @@ class SCF:
     (smearing 0) or Gaussian (1) occupations at inverse temperature beta.
     ``solver`` picks how the Poisson and Kerker equations are solved:
     "spectral" (FFT, exact to rounding) or "aar" (the reference's iteration
-    to poisson_tol / precond_tol)."""
+    to poisson_tol / precond_tol).  ``gram`` ("fp32" or "bf16x3") picks the
+    Gram products of the reduced-precision passes."""
 
     def __init__(self, H, nband, beta, xc="GGA_PBE", nspin=1, kpts=None, kpt_weights=None, smearing=0,
                  cheb_degree=20, rho_trigger=4, nchefsi=1, tol=1e-6, max_iter=100, min_iter=2, poisson_tol=0.0,
                  precond_tol=0.0, mixing_history=7, mixing_param=0.3, kerker=True, fp64_switch_residual=0.0,
-                 low_precision="fp32", solver="spectral"):
+                 low_precision="fp32", solver="spectral", gram="fp32"):
         self.H = H
         self.kpts = None if kpts is None else np.ascontiguousarray(kpts, float).reshape(-1, 3)
         nk = 1 if self.kpts is None else len(self.kpts)
@@ -69,7 +70,8 @@ class SCF:
                       smearing, XC[xc] if isinstance(xc, str) else int(xc), float(tol), max_iter, min_iter,
                       float(poisson_tol), float(precond_tol), mixing_history, float(mixing_param), int(kerker),
                       float(fp64_switch_residual), {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision],
-                      {"spectral": L.SOLVE_SPECTRAL, "aar": L.SOLVE_AAR}[solver])
+                      {"spectral": L.SOLVE_SPECTRAL, "aar": L.SOLVE_AAR}[solver],
+                      {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3}[gram])
         kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
         self._s = ctypes.c_void_p()
         L.check(L.lib().lynx_scf_create(ctypes.byref(self._s), H.ctx, ctypes.byref(p),
