@@ -427,7 +427,10 @@ struct NonorthArgs {
     A ihz;               // 1 / h_z
 };
 
-template <class S, int R, int RP, int NT, int HP, int W>
+// PD: planes of global loads in flight ahead of the plane being consumed
+// (1: the next plane; 2: the two next planes, for short-row grids whose
+// steps are latency-bound)
+template <class S, int R, int RP, int NT, int HP, int W, int PD = 1>
 __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename Tr<S>::A> &q, const StepIO<typename Tr<S>::A> io, const int w)
 {
     using A = typename Tr<S>::A;
@@ -535,7 +538,7 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
 
     int buf = 0;
     auto step = [&](int f, StepLoads<S, RP, HP> &cur, StepLoads<S, RP, HP> &nxt) {
-        issue(f + 1, nxt);
+        issue(f + PD, nxt);
         S *LX = lds0 + buf * 2 * plane;
         S *LG = LX + plane;
         const S *LGp = lds0 + (buf ^ 1) * 2 * plane + plane;   // g of plane f - 1
@@ -627,14 +630,17 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
         buf ^= 1;
     };
 
-    StepLoads<S, RP, HP> la, lb;
+    // a ring of PD + 1 load sets: the step on plane f consumes its set and
+    // refills the one the previous step consumed with plane f + PD
+    StepLoads<S, RP, HP> lring[PD + 1];
     int f = z0 - R;
-    issue(f, la);
-    for (; f + 1 <= f_end; f += 2) {
-        step(f, la, lb);
-        step(f + 1, lb, la);
+#pragma unroll
+    for (int d = 0; d < PD; ++d) issue(f + d, lring[d]);
+    for (; f <= f_end; f += PD + 1) {
+#pragma unroll
+        for (int d = 0; d <= PD; ++d)
+            if (f + d <= f_end) step(f + d, lring[d], lring[(d + PD) % (PD + 1)]);
     }
-    if (f <= f_end) step(f, la, lb);
 }
 
 template <class S, int R, int RP, int NT, int HP, int W>
@@ -1341,7 +1347,8 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
     const StepIO<A> io{r.buf[ix], r.buf[iold], const_cast<void *>(r.buf[iout]), r.ld[ix], r.ld[iold], r.ld[iout],
                        coef[3 * u], coef[3 * u + 1], coef[3 * u + 2]};
     if constexpr (KIND == 1) {
-        stencil_nonorth_body<S, R, RP, NT, HP, W>(r.q, io, w - r.item0);
+        // batched units are small grids: two planes of loads in flight
+        stencil_nonorth_body<S, R, RP, NT, HP, W, 2>(r.q, io, w - r.item0);
     } else if constexpr (KIND == 2 || KIND == 3) {
         if constexpr (!Tr<S>::C)
             stencil_vacc_body<S, R, RP, NT, HP, W, 2, KIND == 3 ? 2 : int(16 / sizeof(S))>(r.q.b, io, w - r.item0);
