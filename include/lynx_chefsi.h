@@ -521,6 +521,20 @@ lynx_status lynx_batch_set_graph(lynx_batch *batch, int enable);
 /* Device time of the last lynx_batch_filter (HIP events around the graph
  * launch or the eager launches) and the number of kernel launches it holds. */
 lynx_status lynx_batch_last_timing(const lynx_batch *batch, double *ms, int *launches);
+/* One CheFSI pass for every unit of the batch: lynx_chefsi_solve on each
+ * unit's block psi[u] (device memory, leading dimension ldpsi[u], ncol[u]
+ * orbitals), done together.  The filter is the batch's graph-replayed
+ * filter; H times the orthonormalised blocks is one batched step; the
+ * Gram products and rotations are batched GEMMs and the Cholesky
+ * factorisations, triangular inverses and subspace eigensolves batched
+ * rocSOLVER calls, over the units of equal (Nd, ncol, ldpsi).  dtype is
+ * LYNX_F64 or LYNX_C128.  eigvals receives sum(ncol) values, unit after
+ * unit, ascending within a unit; lambda_cutoff_out[u] (optional) the
+ * unit's eig[ncol - 1] + 0.1.  Replaces the per-(spin, k-point) loop of
+ * solve / solve_kpt calls (SCF.cpp:391-411, EigenSolver.cpp:738-842). */
+lynx_status lynx_batch_solve(lynx_batch *batch, int dtype, void *const *psi, const long *ldpsi, const int *ncol,
+                             double *eigvals, const double *lambda_cutoff, const double *eig_min,
+                             const double *eig_max, int degree, double *lambda_cutoff_out);
 
 #ifdef __cplusplus
 }

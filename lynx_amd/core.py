@@ -468,6 +468,25 @@ class BatchFilter:
         self._check(L.lib().lynx_batch_filter(self._b, L.dtype_code(dtype), xp, ld, yp, ld, L.iptr(nc), int(degree),
                                               L.dptr(lc), L.dptr(lo), L.dptr(hi), L.MEM_DEVICE))
 
+    def solve_device(self, dtype, ptrs, lds, ncols, degree, lambda_cutoff, eig_min, eig_max):
+        """One CheFSI pass for every unit (lynx_batch_solve) on device blocks
+        (lists of addresses, leading dimensions and column counts), in
+        float64 or complex128; returns (eigenvalues per unit, lambda_cutoff
+        per unit)."""
+        n = len(self.units)
+        pp = (ctypes.c_void_p * n)(*ptrs)
+        ld = (ctypes.c_long * n)(*lds)
+        nc = np.asarray(ncols, np.int32)
+        lc = np.broadcast_to(np.asarray(lambda_cutoff, float), (n,)).copy()
+        lo = np.broadcast_to(np.asarray(eig_min, float), (n,)).copy()
+        hi = np.broadcast_to(np.asarray(eig_max, float), (n,)).copy()
+        eig = np.zeros(int(nc.sum()))
+        lco = np.zeros(n)
+        self._check(L.lib().lynx_batch_solve(self._b, L.dtype_code(dtype), pp, ld, L.iptr(nc), L.dptr(eig),
+                                             L.dptr(lc), L.dptr(lo), L.dptr(hi), int(degree), L.dptr(lco)))
+        off = np.concatenate([[0], np.cumsum(nc)])
+        return [eig[off[i]:off[i + 1]] for i in range(n)], lco
+
     def last_timing(self):
         ms = ctypes.c_double(); nl = ctypes.c_int()
         self._check(L.lib().lynx_batch_last_timing(self._b, ctypes.byref(ms), ctypes.byref(nl)))

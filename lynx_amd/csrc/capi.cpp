@@ -1833,6 +1833,7 @@ struct lynx_batch {
         double kpt[3];
         DevBuf ph64, ph32;     // image phases e^{-i k.R} of this unit's k-point
         DevBuf work, dx, dy;   // recurrence work block; staging for host buffers
+        DevBuf sy;             // lynx_batch_solve: the filtered block, then H psi, then the rotated block
         long epoch = -1;       // the context's set-up epoch the phases were built for
     };
     std::vector<Unit> units;
@@ -1842,6 +1843,13 @@ struct lynx_batch {
     hipGraphExec_t graph = nullptr;
     int graph_degree = 0;
     bool use_graph = true;
+    // lynx_batch_solve: the H psi step's plan (its own coefficient row, so
+    // the filter graph's table is never resized), a BLAS handle on the
+    // batch's stream and per-group dense work
+    std::vector<long> hkey;
+    gpu::BatchPlan *hplan = nullptr;
+    rocblas_handle blas = nullptr;
+    DevBuf dmat, dhs, dev, dwork, dinfo, dptrs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0;
     int last_launches = 0;
@@ -1850,9 +1858,12 @@ struct lynx_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph) (void)hipGraphExecDestroy(graph);
         gpu::batch_plan_destroy(plan);
+        gpu::batch_plan_destroy(hplan);
+        if (blas) (void)rocblas_destroy_handle(blas);
         for (auto &u : units) {
-            u.ph64.release(); u.ph32.release(); u.work.release(); u.dx.release(); u.dy.release();
+            u.ph64.release(); u.ph32.release(); u.work.release(); u.dx.release(); u.dy.release(); u.sy.release();
         }
+        for (DevBuf *x : {&dmat, &dhs, &dev, &dwork, &dinfo, &dptrs}) x->release();
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -2100,5 +2111,239 @@ lynx_status lynx_batch_filter(lynx_batch *b, int dtype, void *const *x, const lo
         HIP_OK(hipEventElapsedTime(&ms, b->e0, b->e1));
         b->last_ms = ms;
         b->last_launches = degree * gpu::batch_launches_per_step(b->plan);
+    });
+}
+
+namespace {
+
+// The operator part of batch unit u (grid, Bloch phases, Veff, projectors)
+// with three block buffers.
+gpu::BatchUnit batch_unit(lynx_batch::Unit &U, int dtype, const void *b0, long l0, const void *b1, long l1,
+                          const void *b2, long l2, int ncol)
+{
+    lynx_ctx *c = U.ctx;
+    const bool single = gpu::elem_single(dtype), cplx = gpu::elem_complex(dtype);
+    gpu::BatchUnit B;
+    const double k0[3] = {0, 0, 0};
+    B.g = geom_at(c, cplx ? U.kpt : k0);
+    B.veff = single ? c->veff32.p : c->veff64.p;
+    if (c->has_nl) {
+        B.nl = single ? c->nl32 : c->nl64;
+        B.nl.phase = single ? U.ph32.p : U.ph64.p;
+    }
+    B.buf[0] = b0; B.buf[1] = b1; B.buf[2] = b2;
+    B.ld[0] = l0; B.ld[1] = l1; B.ld[2] = l2;
+    B.ncol = ncol;
+    return B;
+}
+
+void check_infos(lynx_batch *b, const rocblas_int *dinfo, int m, const std::vector<int> &units, int status,
+                 const char *what)
+{
+    std::vector<rocblas_int> info(m);
+    HIP_OK(hipMemcpyAsync(info.data(), dinfo, m * sizeof(rocblas_int), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < m; ++i)
+        if (info[i] != 0)
+            fail(status, std::string(what) + " failed for batch unit " + std::to_string(units[i]) + " (info=" +
+                             std::to_string(info[i]) + ")");
+}
+
+}  // namespace
+
+lynx_status lynx_batch_solve(lynx_batch *b, int dtype, void *const *psi, const long *ldpsi, const int *ncol,
+                             double *eigvals, const double *lambda_c, const double *eig_min, const double *eig_max,
+                             int degree, double *lambda_c_out)
+{
+    if (!b) return LYNX_E_ARG;
+    return bguarded(b, [&] {
+        if (dtype != LYNX_F64 && dtype != LYNX_C128) fail(LYNX_E_ARG, "lynx_batch_solve runs in FP64 or complex128");
+        if (!psi || !ncol || !eigvals || !lambda_c || !eig_min || !eig_max) fail(LYNX_E_ARG, "null batch argument");
+        const int n = (int)b->units.size();
+        const bool cplx = dtype == LYNX_C128;
+        const size_t es = gpu::elem_size(dtype);
+        std::vector<void *> ys(n);
+        std::vector<long> lys(n), lps(n), eoff(n + 1, 0);
+        for (int u = 0; u < n; ++u) {
+            lynx_batch::Unit &U = b->units[u];
+            const long nd = U.ctx->nd;
+            if (ncol[u] <= 0 || !psi[u]) fail(LYNX_E_ARG, "bad block of batch unit " + std::to_string(u));
+            lps[u] = (ldpsi && ldpsi[u]) ? ldpsi[u] : nd;
+            if (lps[u] < nd) fail(LYNX_E_ARG, "leading dimension smaller than Nd");
+            if (ncol[u] > nd) fail(LYNX_E_ARG, "more orbitals than grid points in batch unit " + std::to_string(u));
+            lys[u] = padded_ld(nd);
+            ys[u] = U.sy.get((size_t)lys[u] * ncol[u] * es);
+            eoff[u + 1] = eoff[u] + ncol[u];
+        }
+        // 1. Y = p(H) psi for every unit (EigenSolver.cpp:558-619)
+        {
+            const lynx_status fs = lynx_batch_filter(b, dtype, psi, lps.data(), ys.data(), lys.data(), ncol, degree,
+                                                     lambda_c, eig_min, eig_max, LYNX_MEM_DEVICE);
+            if (fs != LYNX_OK) fail(fs, "batch filter: " + b->err);
+        }
+        if (!b->blas) {
+            BLAS_OK(rocblas_create_handle(&b->blas));
+            BLAS_OK(rocblas_set_stream(b->blas, b->stream));
+        }
+        hipStream_t st = b->stream;
+        // units of equal (Nd, ncol, ldpsi, dV) form one batched group
+        std::map<std::vector<long>, std::vector<int>> groups;
+        for (int u = 0; u < n; ++u) {
+            const lynx_ctx *c = b->units[u].ctx;
+            long dvbits;
+            std::memcpy(&dvbits, &c->st.dV, sizeof(dvbits));
+            groups[{(long)c->nd, ncol[u], lps[u], dvbits}].push_back(u);
+        }
+        size_t mat = 0, ev = 0;
+        for (auto &g : groups) {
+            const size_t N = (size_t)g.first[1];
+            mat += g.second.size() * N * N;
+            ev += g.second.size() * N;
+        }
+        char *M = static_cast<char *>(b->dmat.get(mat * es));      // S, then R^-1
+        char *Hm = static_cast<char *>(b->dhs.get(mat * es));      // Hs, then Q
+        b->dev.get(ev * sizeof(double));
+        b->dwork.get(ev * sizeof(double));
+        b->dinfo.get((size_t)n * sizeof(rocblas_int));
+        double *D = b->dev.as<double>(), *E = b->dwork.as<double>();
+        rocblas_int *info = b->dinfo.as<rocblas_int>();
+        // device pointer tables, four per unit: Y, psi, the unit's N x N matrix
+        // in M, the one in Hm (group by group)
+        std::vector<void *> hp(4 * (size_t)n);
+        struct Grp {
+            std::vector<int> units;
+            long nd, ld, ldy;
+            int N;
+            double dV;
+            size_t moff, eoff, poff;
+        };
+        std::vector<Grp> G;
+        {
+            size_t moff = 0, eo = 0, po = 0;
+            for (auto &g : groups) {
+                Grp q;
+                q.units = g.second;
+                q.nd = g.first[0];
+                q.N = (int)g.first[1];
+                q.ld = g.first[2];
+                q.ldy = padded_ld(q.nd);
+                q.dV = b->units[q.units[0]].ctx->st.dV;
+                q.moff = moff; q.eoff = eo; q.poff = po;
+                const int m = (int)q.units.size();
+                for (int i = 0; i < m; ++i) {
+                    const int u = q.units[i];
+                    hp[po + i] = ys[u];
+                    hp[po + m + i] = psi[u];
+                    hp[po + 2 * m + i] = M + (moff + (size_t)i * q.N * q.N) * es;
+                    hp[po + 3 * m + i] = Hm + (moff + (size_t)i * q.N * q.N) * es;
+                }
+                moff += (size_t)m * q.N * q.N;
+                eo += (size_t)m * q.N;
+                po += 4 * (size_t)m;
+                G.push_back(q);
+            }
+        }
+        void **dp = static_cast<void **>(b->dptrs.get(hp.size() * sizeof(void *)));
+        HIP_OK(hipMemcpyAsync(dp, hp.data(), hp.size() * sizeof(void *), hipMemcpyHostToDevice, st));
+        using Z = rocblas_double_complex;
+        auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int mm, int nn, int kk, double alpha, void **A,
+                        long lda, void **B, long ldb, void **C, long ldc, int cnt) {
+            if (cplx) {
+                const Z a(alpha, 0.0), z(0.0, 0.0);
+                BLAS_OK(rocblas_zgemm_batched(b->blas, ta, tb, mm, nn, kk, &a, (const Z *const *)A, (int)lda,
+                                              (const Z *const *)B, (int)ldb, &z, (Z *const *)C, (int)ldc, cnt));
+            } else {
+                const double z = 0.0;
+                BLAS_OK(rocblas_dgemm_batched(b->blas, ta == rocblas_operation_conjugate_transpose
+                                                           ? rocblas_operation_transpose
+                                                           : ta,
+                                              tb, mm, nn, kk, &alpha, (const double *const *)A, (int)lda,
+                                              (const double *const *)B, (int)ldb, &z, (double *const *)C, (int)ldc,
+                                              cnt));
+            }
+        };
+        const rocblas_operation OC = rocblas_operation_conjugate_transpose, ON = rocblas_operation_none;
+        // 2. Cholesky-QR per group: S = dV Y^H Y, R^H R = S, psi = Y R^-1 (EigenSolver.cpp:621-641)
+        for (const Grp &q : G) {
+            const int m = (int)q.units.size(), N = q.N;
+            void **Yp = dp + q.poff, **Pp = dp + q.poff + m, **Mp = dp + q.poff + 2 * m;
+            gemm(OC, ON, N, N, (int)q.nd, q.dV, Yp, q.ldy, Yp, q.ldy, Mp, N, m);
+            void *Mg = M + q.moff * es;
+            const rocblas_stride sN = (rocblas_stride)N * N;
+            if (cplx)
+                BLAS_OK(rocsolver_zpotrf_strided_batched(b->blas, rocblas_fill_upper, N, (Z *)Mg, N, sN, info, m));
+            else
+                BLAS_OK(rocsolver_dpotrf_strided_batched(b->blas, rocblas_fill_upper, N, (double *)Mg, N, sN, info,
+                                                         m));
+            check_infos(b, info, m, q.units, LYNX_E_CHOLESKY, "Cholesky factorisation of the overlap matrix");
+            if (cplx)
+                BLAS_OK(rocsolver_ztrtri_strided_batched(b->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, N,
+                                                         (Z *)Mg, N, sN, info, m));
+            else
+                BLAS_OK(rocsolver_dtrtri_strided_batched(b->blas, rocblas_fill_upper, rocblas_diagonal_non_unit, N,
+                                                         (double *)Mg, N, sN, info, m));
+            check_infos(b, info, m, q.units, LYNX_E_CHOLESKY, "triangular inverse");
+            for (int i = 0; i < m; ++i) gpu::zero_lower(dtype, M + (q.moff + (size_t)i * N * N) * es, N, st);
+            gemm(ON, ON, (int)q.nd, N, N, 1.0, Yp, q.ldy, Mp, N, Pp, q.ld, m);
+        }
+        // 3. H psi into Y for every unit: one batched step (a = 1, no old
+        // term, no shift) from its own plan
+        {
+            std::vector<gpu::BatchUnit> bu(n);
+            std::vector<long> key = {dtype};
+            for (int u = 0; u < n; ++u) {
+                lynx_batch::Unit &U = b->units[u];
+                bu[u] = batch_unit(U, dtype, psi[u], lps[u], ys[u], lys[u], ys[u], lys[u], ncol[u]);
+                for (long v : {(long)(uintptr_t)psi[u], lps[u], (long)(uintptr_t)ys[u], lys[u], (long)ncol[u],
+                               U.ctx->epoch, (long)(uintptr_t)U.ctx->veff64.p, (long)(uintptr_t)U.ph64.p})
+                    key.push_back(v);
+            }
+            if (key != b->hkey || !b->hplan) {
+                gpu::batch_plan_destroy(b->hplan);
+                b->hplan = nullptr;
+                b->hplan = gpu::batch_plan_create(dtype, bu.data(), n, st);
+                b->hkey = key;
+            }
+            std::vector<double> coef((size_t)n * 3, 0.0);
+            for (int u = 0; u < n; ++u) coef[3 * u] = 1.0;
+            gpu::batch_set_coef(b->hplan, coef.data(), 1, st);
+            gpu::batch_step(b->hplan, 0, 0, 3, 1, st);
+        }
+        // 4. Rayleigh-Ritz per group: Hs = dV psi^H (H psi), Hermitian part,
+        // Hs = Q Lambda Q^H, psi = psi Q (EigenSolver.cpp:643-681)
+        std::vector<double> hev(ev);
+        for (const Grp &q : G) {
+            const int m = (int)q.units.size(), N = q.N;
+            void **Yp = dp + q.poff, **Pp = dp + q.poff + m, **Hp = dp + q.poff + 3 * m;
+            gemm(OC, ON, N, N, (int)q.nd, q.dV, Pp, q.ld, Yp, q.ldy, Hp, N, m);
+            for (int i = 0; i < m; ++i) gpu::hermitian_part(dtype, Hm + (q.moff + (size_t)i * N * N) * es, N, st);
+            void *Hg = Hm + q.moff * es;
+            const rocblas_stride sN = (rocblas_stride)N * N;
+            if (cplx)
+                BLAS_OK(rocsolver_zheevd_strided_batched(b->blas, rocblas_evect_original, rocblas_fill_upper, N,
+                                                         (Z *)Hg, N, sN, D + q.eoff, N, E + q.eoff, N, info, m));
+            else
+                BLAS_OK(rocsolver_dsyevd_strided_batched(b->blas, rocblas_evect_original, rocblas_fill_upper, N,
+                                                         (double *)Hg, N, sN, D + q.eoff, N, E + q.eoff, N, info, m));
+            check_infos(b, info, m, q.units, LYNX_E_EIG, "subspace eigensolve");
+            gemm(ON, ON, (int)q.nd, N, N, 1.0, Pp, q.ld, Hp, N, Yp, q.ldy, m);
+            for (int i = 0; i < m; ++i) {
+                const int u = q.units[i];
+                gpu::copy_cols(dtype, ys[u], lys[u], psi[u], lps[u], q.nd, N, st);
+            }
+        }
+        HIP_OK(hipMemcpyAsync(hev.data(), D, ev * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (const Grp &q : G)
+            for (size_t i = 0; i < q.units.size(); ++i) {
+                const int u = q.units[i];
+                const double *src = hev.data() + q.eoff + i * q.N;
+                for (int j = 0; j < q.N; ++j) {
+                    if (!std::isfinite(src[j])) fail(LYNX_E_NONFINITE, "non-finite Ritz value in batch unit " +
+                                                                           std::to_string(u));
+                    eigvals[eoff[u] + j] = src[j];
+                }
+                if (lambda_c_out) lambda_c_out[u] = src[q.N - 1] + 0.1;
+            }
     });
 }
