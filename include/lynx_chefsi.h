@@ -347,7 +347,8 @@ lynx_status lynx_get_stencil_config(const lynx_ctx *ctx, int dtype, int ncol, in
  * with the spectral bounds of SCF::estimate_spectral_bounds
  * (SCF.cpp:883-935): the caller's SCF loop (density, potential, mixing)
  * calls lynx_scf_eig_step once per SCF iteration with the current Veff and
- * residual.  Iteration 0 runs rho_trigger CheFSI passes, later iterations
+ * residual.  Iteration 0 starts a run: Lanczos bounds per spin at the first
+ * k-point, then rho_trigger CheFSI passes; later iterations run
  * nchefsi; after each pass eig_min[s] <- eig(s, k0)[0] (iterations > 0),
  * lambda_cutoff <- max_s eig(s, k0)[N-1] + 0.1, and the occupations and Fermi
  * level are recomputed (Occupation.cpp:53-176).  With fp64_switch_residual
@@ -465,8 +466,18 @@ lynx_status lynx_scf_create(lynx_scf **scf, lynx_ctx *ctx, const lynx_scf_params
                             const double *kpt_weights);
 lynx_status lynx_scf_destroy(lynx_scf *scf);
 const char *lynx_scf_last_error(const lynx_scf *scf);
-/* Runs the SCF loop from the atomic densities and random orbitals to
- * convergence (or max_iter). */
+/* Starting density of the next lynx_scf_run in place of the atomic
+ * densities, e.g. a converged density read back from a restart file
+ * (host pointers, nd = Nx Ny Nz): the total density, and for two spins the
+ * magnetization up - down (NULL: the total split evenly); each spin's
+ * density is (total +- mag) / 2 floored at zero.  rho_total NULL restores
+ * the atomic start.  Replaces SCF::set_initial_density (SCF.cpp:83-107) as
+ * used for DensityIO restarts (SCF.cpp:742-761) and by pylynx's
+ * Calculator::set_initial_density (python/src/Calculator.hpp:96-101). */
+lynx_status lynx_scf_set_initial_density(lynx_scf *scf, const double *rho_total, int nd, const double *mag);
+/* Runs the SCF loop from the atomic densities (or the density given by
+ * lynx_scf_set_initial_density) and random orbitals to convergence (or
+ * max_iter). */
 lynx_status lynx_scf_run(lynx_scf *scf, lynx_scf_result *result);
 /* Per-iteration record of the last run (capacity cap): total energy, density
  * residual, wall seconds, filter precision used; *n the iteration count. */

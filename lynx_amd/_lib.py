@@ -80,6 +80,7 @@ SIGNATURES = {
     "lynx_scf_create": (_I, [ctypes.POINTER(_P), _P, _P, _DP, _DP]),
     "lynx_scf_destroy": (_I, [_P]),
     "lynx_scf_last_error": (ctypes.c_char_p, [_P]),
+    "lynx_scf_set_initial_density": (_I, [_P, _DP, _I, _DP]),
     "lynx_scf_run": (_I, [_P, _P]),
     "lynx_scf_history": (_I, [_P, _I, _DP, _DP, _DP, _IP, _IP]),
     "lynx_scf_get": (_I, [_P, _DP, _DP, _DP, _DP]),

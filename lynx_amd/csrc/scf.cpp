@@ -21,6 +21,7 @@
 
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -205,6 +206,7 @@ struct lynx_scf {
     std::vector<double> eig, occ;
     double fermi = 0;
     bool ran = false;                            // a lynx_scf_run completed
+    std::vector<double> rho_init;                // restart density, layout of rho (empty: atomic start)
     lynx_scf_result res{};
     std::vector<double> h_e, h_err, h_t;
     std::vector<int> h_prec;
@@ -691,6 +693,40 @@ lynx_status lynx_scf_build_potential(lynx_scf *S, const double *rho)
     return st;
 }
 
+lynx_status lynx_scf_set_initial_density(lynx_scf *S, const double *rho_total, int nd, const double *mag)
+{
+    if (!S) return LYNX_E_ARG;
+    if (!rho_total) {
+        S->rho_init.clear();
+        return LYNX_OK;
+    }
+    if (nd != S->nd) {
+        S->err = "lynx_scf_set_initial_density: nd " + std::to_string(nd) + " is not the grid's " +
+                 std::to_string(S->nd);
+        return LYNX_E_ARG;
+    }
+    // SCF::set_initial_density (SCF.cpp:83-107): the total as given; two
+    // spins split as (total +- mag) / 2 floored at zero, or evenly without mag
+    const long n = S->nd;
+    if (S->p.nspin == 1) {
+        S->rho_init.assign(rho_total, rho_total + n);
+        return LYNX_OK;
+    }
+    S->rho_init.assign(3 * n, 0.0);
+    double *t = S->rho_init.data(), *up = t + n, *dn = t + 2 * n;
+    for (long i = 0; i < n; ++i) {
+        t[i] = rho_total[i];
+        if (mag) {
+            up[i] = std::max(0.5 * (rho_total[i] + mag[i]), 0.0);
+            dn[i] = std::max(0.5 * (rho_total[i] - mag[i]), 0.0);
+        } else {
+            up[i] = 0.5 * rho_total[i];
+            dn[i] = 0.5 * rho_total[i];
+        }
+    }
+    return LYNX_OK;
+}
+
 lynx_status lynx_scf_run(lynx_scf *S, lynx_scf_result *out)
 {
     if (!S) return LYNX_E_ARG;
@@ -701,10 +737,16 @@ lynx_status lynx_scf_run(lynx_scf *S, lynx_scf_result *out)
         const long n = S->nd;
         const int ns = S->p.nspin, nk = S->p.nkpt, N = S->p.nband;
         const ElecHost &E = electrostatics(c);
-        // initial density: the isolated-atom superposition (SCF.cpp:705-725),
-        // split evenly between spins
-        HIP_OK(hipMemcpyAsync(S->rho.p, E.rho_at.data(), n * sizeof(double), hipMemcpyHostToDevice, hs));
-        if (ns == 2) {
+        // initial density: a restart density given by
+        // lynx_scf_set_initial_density, else the isolated-atom superposition
+        // (SCF.cpp:705-725) split evenly between spins
+        if (!S->rho_init.empty()) {
+            HIP_OK(hipMemcpyAsync(S->rho.p, S->rho_init.data(), S->rho_init.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, hs));
+        } else {
+            HIP_OK(hipMemcpyAsync(S->rho.p, E.rho_at.data(), n * sizeof(double), hipMemcpyHostToDevice, hs));
+        }
+        if (ns == 2 && S->rho_init.empty()) {
             dcopy(S, dptr(S->rho) + n, dptr(S->rho), n);
             gpu::scf_scale(dptr(S->rho) + n, 0.5, n, hs);
             dcopy(S, dptr(S->rho) + 2 * n, dptr(S->rho) + n, n);

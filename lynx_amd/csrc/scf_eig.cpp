@@ -192,8 +192,9 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
         const lynx_scf_eig_params &p = d->p;
         const int dtype = p.complex_kpt ? LYNX_C128 : LYNX_F64;
         const int N = p.nband;
-        // spectral bounds once, per spin, at the first k-point (SCF.cpp:912-929)
-        if (!d->have_bounds) {
+        // spectral bounds per spin at the first k-point, once per SCF run:
+        // at its first iteration (SCF.cpp:912-929)
+        if (!d->have_bounds || scf_iter == 0) {
             for (int s = 0; s < p.nspin; ++s) {
                 ok(lynx_set_veff(d->ctx, veff + s * d->nd, veff_mem), d, "lynx_set_veff");
                 if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[0]), d, "lynx_set_kpoint");

@@ -8,7 +8,11 @@ builds the grid, atoms and k-points, ``run()`` runs the SCF and returns the
 total energy, with the accessors ``energy()``, ``fermi_energy()``,
 ``n_iterations()``, ``is_converged()``, and ``compute_forces()`` after a
 run.  Defaults follow
-ParameterDefaults.cpp:10-80.  All numerical work goes through the C ABI.
+ParameterDefaults.cpp:10-80.  The "output" keys density_restart_file and
+density_output_file read and write the reference's density file
+(SCF.cpp:742-761, 822-831; lynx_amd.io), ``set_initial_density`` starts
+from a given density, and ``write_results`` writes the reference's result
+JSON.  All numerical work goes through the C ABI.
 """
 from __future__ import annotations
 
@@ -17,9 +21,12 @@ import json
 import math
 import os
 
+import time
+
 import numpy as np
 
 from . import _lib as L
+from . import io
 
 KB = 3.1668115634556e-6    # Ha / K
 EH = 27.211386245988       # eV per Ha
@@ -87,6 +94,22 @@ class SCF:
         self._check(L.lib().lynx_scf_run(self._s, ctypes.byref(r)))
         self.natom = r.natom
         return r.as_dict()
+
+    def set_initial_density(self, rho, mag=None):
+        """Start the next run() from the total density ``rho`` (Nd) and,
+        for two spins, the magnetization ``mag`` (None: split evenly) in
+        place of the atomic densities; ``rho`` None restores the atomic
+        start (SCF::set_initial_density, SCF.cpp:83-107)."""
+        if rho is None:
+            self._check(L.lib().lynx_scf_set_initial_density(self._s, None, 0, None))
+            return
+        r = np.ascontiguousarray(rho, float).ravel()
+        m = None if mag is None else np.ascontiguousarray(mag, float).ravel()
+        if m is not None and m.size != r.size:
+            raise ValueError("set_initial_density: mag and rho differ in size")
+        self._keep = (r, m)
+        self._check(L.lib().lynx_scf_set_initial_density(self._s, L.dptr(r), r.size,
+                                                         None if m is None else L.dptr(m)))
 
     def history(self):
         n = ctypes.c_int()
@@ -179,6 +202,10 @@ class Calculator:
         self._dir = "."
         self._setup = False
         self._result = None
+        self._rho0 = None
+        self._forces = None
+        self._times = {"scf_total": 0.0, "force": 0.0, "stress": 0.0, "walltime_total": 0.0}
+        self._t0 = time.perf_counter()
 
     def load_config(self, json_file):
         with open(json_file) as f:
@@ -216,6 +243,11 @@ class Calculator:
         self.frac = np.array(frac)
         vnl = core.NonlocalProjector().setup_atoms(files, types, self.frac, True)
         self.H = core.Hamiltonian(self.device_id).setup(st, dom, grid, core.HaloExchange(dom, st.FDn()), vnl)
+        self.orthogonal = orth
+        self.shape = n
+        self.latvec = latvec
+        self.h = (grid.dx(), grid.dy(), grid.dz())
+        self.dV = grid.dV()
         zions = []
         for f in files:
             z = ctypes.c_double(); lm = ctypes.c_int(); ll = ctypes.c_int(); npj = ctypes.c_int()
@@ -223,6 +255,7 @@ class Calculator:
             L.check(L.lib().lynx_psp8_info(f.encode(), ctypes.byref(z), ctypes.byref(lm), ctypes.byref(ll),
                                            ctypes.byref(npj), ctypes.byref(rc)))
             zions.append(z.value)
+        self.zions = zions
         self.Ne = int(sum(int(zions[t]) for t in types))
         self.natom = len(types)
         e = c.get("electronic", {})
@@ -232,6 +265,7 @@ class Calculator:
         temp = float(e.get("temperature", -1.0))
         if temp <= 0:
             temp = elec_temp_default(smear)
+        self.temperature_K = temp
         self.beta = 1.0 / (KB * temp)
         self.smearing = 1 if smear == "gaussian" else 0
         self.nstates = int(e.get("Nstates", 0)) or (self.Ne // 2 + 10)
@@ -259,7 +293,10 @@ class Calculator:
             kc, w = np.zeros(3 * nk.value), np.zeros(nk.value)
             L.check(L.lib().lynx_kpoints(K[0], K[1], K[2], L.dptr(sh), L.dptr(lv), cell, L.dptr(kc), None, L.dptr(w),
                                          nk.value, ctypes.byref(nk)))
-            self.kpts, self.kw = kc.reshape(-1, 3), w
+            kr = np.zeros(3 * nk.value)
+            L.check(L.lib().lynx_kpoints(K[0], K[1], K[2], L.dptr(sh), L.dptr(lv), cell, None, L.dptr(kr), None,
+                                         nk.value, ctypes.byref(nk)))
+            self.kpts, self.kw, self.kred = kc.reshape(-1, 3), w, kr.reshape(-1, 3)
         self._setup = True
         return self
 
@@ -270,7 +307,32 @@ class Calculator:
         kw.update(overrides)
         self.scf = SCF(self.H, self.nstates, self.beta, xc=self.xc, nspin=self.nspin, kpts=self.kpts,
                        kpt_weights=self.kw, smearing=self.smearing, cheb_degree=self.cheb_degree, **kw)
+        out = self.config.get("output", {})
+        rho0 = self._rho0
+        restart = out.get("density_restart_file")
+        if restart:
+            # SCF.cpp:742-761: a file that does not fit falls back to the
+            # atomic densities with a warning
+            path = restart if os.path.isabs(restart) else os.path.join(self._dir, restart)
+            got = io.read_density(path, self.shape, self.nspin)
+            if got is None:
+                print(f"WARNING: Failed to read density from {restart}, using atomic density", flush=True)
+            else:
+                rho0 = io.total_and_mag(got)
+                print(f"Density restart: loaded from {restart} (Ne={got.sum() * self.dV:.6f})", flush=True)
+        if rho0 is not None:
+            self.scf.set_initial_density(*rho0)
+        t = time.perf_counter()
         self._result = self.scf.run()
+        self._times["scf_total"] = time.perf_counter() - t
+        self._forces = None
+        dest = out.get("density_output_file")
+        if dest:
+            path = dest if os.path.isabs(dest) else os.path.join(self._dir, dest)
+            if io.write_density(path, self.spin_densities(), self.shape, self.latvec, self.dV):
+                print(f"Density written to {dest}", flush=True)
+            else:
+                print(f"WARNING: Failed to write density to {dest}", flush=True)
         return self._result["etotal"]
 
     def compute_forces(self):
@@ -278,7 +340,44 @@ class Calculator:
         Calculator.compute_forces, python/src/bind_calculator.cpp:104-112)."""
         if self._result is None:
             raise RuntimeError("compute_forces needs a completed run()")
-        return self.scf.forces()
+        t = time.perf_counter()
+        self._forces = self.scf.forces()
+        self._times["force"] = time.perf_counter() - t
+        return self._forces
+
+    def set_initial_density(self, rho, mag=None):
+        """Start the next run() from the total density ``rho`` (Nd) and the
+        magnetization ``mag`` (two spins; None splits evenly) instead of the
+        atomic densities (python/src/bind_calculator.cpp:123-139)."""
+        self._rho0 = None if rho is None else (np.array(rho, float).ravel(),
+                                               None if mag is None else np.array(mag, float).ravel())
+
+    def spin_densities(self):
+        """The density the last run ended on, nspin x Nd (total for one
+        spin, up and down for two): the input density of its last iteration,
+        the one the reference writes to its density file."""
+        _, _, rho, _ = self.scf.state()
+        return rho.reshape(self.nspin, -1)
+
+    def density(self):
+        """Total density of the last run, Nd (pylynx's Calculator.density,
+        python/src/bind_calculator.cpp:177-179)."""
+        r = self.spin_densities()
+        return r[0] if self.nspin == 1 else r[0] + r[1]
+
+    def kpoints_reduced(self):
+        """Reduced coordinates and weights of the k-points in use."""
+        if self.kpts is None:
+            return np.zeros((1, 3)), np.ones(1)
+        return self.kred, self.kw
+
+    def write_results(self, path, test_name="", input_file=""):
+        """The reference's result JSON of the last run (lynx_amd.io.ResultWriter)."""
+        if self._result is None:
+            raise RuntimeError("write_results needs a completed run()")
+        self._times["walltime_total"] = time.perf_counter() - self._t0
+        io.ResultWriter.from_calculator(self, test_name=test_name, input_file=input_file, forces=self._forces,
+                                        timing=self._times).write(path)
 
     # accessors of pylynx's Calculator
     def is_setup(self): return self._setup

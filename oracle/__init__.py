@@ -871,8 +871,9 @@ class ScfEigStage:
         nd = op.grid.nd
         veff = np.asarray(veff, float).reshape(self.nspin, nd)
         nk = len(self.kpts)
-        if self.bounds is None:
-            # SCF.cpp:912-929: Lanczos per spin at the first k-point
+        if self.bounds is None or scf_iter == 0:
+            # SCF.cpp:912-929: Lanczos per spin at the first k-point, once
+            # per SCF run (at its first iteration)
             self.eig_min = np.zeros(self.nspin)
             self.eig_max = np.zeros(self.nspin)
             for s in range(self.nspin):

@@ -211,3 +211,35 @@ def test_pbe_gradient_expansions():
         sig = t2 * (2 * phi * ks * rho) ** 2
         H = osc.fc_pbe(rho, sig, zeta) / rho - osc.ec_unif(rs, zeta, 1, mod=True)
         assert np.allclose(H / t2, 0.066725 * phi ** 3, rtol=2e-4)
+
+
+def test_initial_density_split():
+    # SCF::set_initial_density (SCF.cpp:83-107) by hand: up/down are
+    # (total +- mag) / 2 floored at zero, the total kept as given
+    t, m = np.array([1.0, 0.2, 0.3]), np.array([0.4, 0.5, -0.1])
+    r = osc.ScfOracle.initial_density(2, t, m)
+    assert np.array_equal(r[0], t)
+    assert np.allclose(r[1], [0.7, 0.35, 0.1], rtol=0, atol=1e-16)
+    assert np.allclose(r[2], [0.3, 0.0, 0.2], rtol=0, atol=1e-16)
+    r = osc.ScfOracle.initial_density(2, t)
+    assert np.array_equal(r[1], 0.5 * t) and np.array_equal(r[2], 0.5 * t)
+    assert len(osc.ScfOracle.initial_density(1, t, m)) == 1
+
+
+def test_restart_density_runs():
+    # the atomic density handed in as a restart density reproduces the
+    # default run exactly; the converged density handed back in converges
+    # in fewer iterations to the same energy
+    g = _grid(12)
+    ps = [oracle.load_psp8(psp("C.pot"))]
+    beta = 1 / (3.1668115634556e-6 * 1000)
+    mk = lambda: osc.ScfOracle(g, ps, [0] * 8, DIAMOND, 20, beta, xc=2, tol=1e-6)  # noqa: E731
+    a = mk()
+    ra = a.run()
+    b = mk()
+    b.run(rho0=b.E.rho_at.copy())
+    assert [h["etotal"] for h in a.history] == [h["etotal"] for h in b.history]
+    c = mk()
+    rc = c.run(rho0=a.rho_in[0])
+    assert rc["converged"] and len(c.history) < len(a.history)
+    assert abs(rc["etotal"] - ra["etotal"]) / 8 < 1e-7
