@@ -370,6 +370,11 @@ const gpu::StepCfg &step_config(lynx_ctx *c, int e, const void *x, long ldx, voi
         (void)hipEventDestroy(e1);
         c->tune_log[key] = times;
     }
+    if (!gpu::stencil_shape_fits(e, c->st.nx, best)) {
+        // untuned: the first shape that fits the rows (long rows need 512 threads)
+        const auto cands = step_candidates(c, e);
+        if (!cands.empty()) best = cands.front();
+    }
     return c->tuned[key] = best;
 }
 

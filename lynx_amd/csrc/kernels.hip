@@ -1568,6 +1568,32 @@ int nonorth_max_rows(int pitch, int nt)
     return (int)((kLdsBytes / (4 * (long)sizeof(S)) - nt) / pitch) - 2 * kMaxR;
 }
 
+// LDS pitch of the scalar and non-orthogonal stencil tiles for rows of nx
+// points: 64, 128, or 256 for long rows (nx + 2R > 128; 512-thread shapes,
+// with kLongHalo halo slots per thread for up to 256-point rows)
+inline int tile_pitch(int nx) { return nx + 2 * kMaxR <= 64 ? 64 : (nx + 2 * kMaxR <= 128 ? 128 : 256); }
+constexpr int kLongHalo = 6;
+
+// rows per band of the scalar body within the LDS: two planes of
+// pitch * (rows + 2R) + NT elements
+template <class S>
+int step_max_rows(int pitch, int nt)
+{
+    return (int)((kLdsBytes / (2 * (long)sizeof(S)) - nt) / pitch) - 2 * kMaxR;
+}
+
+// Whether rows of nx points fit the scalar / non-orthogonal tiles of NT
+// threads and RP points per thread (row pitch, halo slots, LDS).
+template <class S>
+bool step_rows_fit(int nx, int nt, int rp, bool nonorth)
+{
+    const int pitch = tile_pitch(nx);
+    if (nx + 2 * kMaxR > 256 || nx > (long)nt * rp) return false;
+    const int hb = pitch == 256 ? (nt >= 512 ? kLongHalo : 0) : (nt >= 512 ? 3 : 6);
+    if (2 * kMaxR * nx > nt * hb) return false;
+    return (nonorth ? nonorth_max_rows<S>(pitch, nt) : step_max_rows<S>(pitch, nt)) >= 1;
+}
+
 template <class S, int RP, int NT>
 void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *old, long ldo,
                     void *out, long ldout, const void *veff, int ncol, double a, double b, double s,
@@ -1581,12 +1607,14 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
     const long pts = (long)NT * RP;
-    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB || g.nx + 2 * kMaxR > 128)
+    if (!step_rows_fit<S>(g.nx, NT, RP, false))
         fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
+    const int pitch = tile_pitch(g.nx);
+    const int max_rows = step_max_rows<S>(pitch, NT);
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
     p.tyr = (g.ny + nb - 1) / nb;
-    while ((long)p.tyr * g.nx > pts) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
+    while ((long)p.tyr * g.nx > pts || p.tyr > max_rows) { ++nb; p.tyr = (g.ny + nb - 1) / nb; }
     p.nbands = (g.ny + p.tyr - 1) / p.tyr;
     nzch = std::max(1, std::min(nzch, g.nz));
     p.zch = (g.nz + nzch - 1) / nzch;
@@ -1600,15 +1628,16 @@ void launch_step_rp(const StencilGeom &g, const void *x, long ldx, const void *o
         for (int sd = 0; sd < 2; ++sd)
             for (int q = 0; q < 2; ++q) p.ph[ax][sd][q] = (A)g.phase[ax][sd][q];
     const int blocks = ((p.nwork + 7) / 8) * 8;
-    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
     const size_t shm = 2 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
     if (pitch == 64) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, p);
         else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, p);
-    } else {
+    } else if (pitch == 128) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, p);
         else hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, p);
+    } else if constexpr (NT >= 512) {
+        hipLaunchKernelGGL((k_stencil_step<S, kMaxR, RP, NT, kLongHalo, 256>), dim3(blocks), dim3(NT), shm, st, p);
     }
 }
 
@@ -1621,14 +1650,14 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     constexpr int HS = NT >= 512 ? 2 : 3;
     constexpr int HB = NT >= 512 ? 3 : 6;
     const long pts = (long)NT * RP;
-    if (g.nx > pts || 2 * kMaxR * g.nx > NT * HB || g.nx + 2 * kMaxR > 128)
+    if (!step_rows_fit<S>(g.nx, NT, RP, true))
         fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
     NonorthArgs<A> q;
     StepArgs<A> &p = q.b;
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const A *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
-    const int pitch = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+    const int pitch = tile_pitch(g.nx);
     const int max_rows = nonorth_max_rows<S>(pitch, NT);
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
@@ -1661,9 +1690,12 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     if (pitch == 64) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, q);
         else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, q);
-    } else {
+    } else if (pitch == 128) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 128>), dim3(blocks), dim3(NT), shm, st, q);
         else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 128>), dim3(blocks), dim3(NT), shm, st, q);
+    } else if constexpr (NT >= 512 && sizeof(S) <= 8) {
+        // long rows; complex128 planes of pitch 256 do not fit the LDS (step_rows_fit)
+        hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, kLongHalo, 256>), dim3(blocks), dim3(NT), shm, st, q);
     }
 }
 
@@ -1773,8 +1805,7 @@ bool stencil_shape_fits(int e, int nx, const StepCfg &cfg)
                 ok = VW > 0 && lpr <= lanes && 2 * kMaxR * lpr <= nt * hb && nx + 2 * XO <= 128 && nx >= 2 * XO;
             }
         } else {
-            const long pts = (long)nt * (base_rp<S>() / std::max(1, cfg.rp_div));
-            ok = nx <= pts && 2 * kMaxR * nx <= nt * (nt >= 512 ? 3 : 6) && nx + 2 * kMaxR <= 128;
+            ok = step_rows_fit<S>(nx, nt, base_rp<S>() / std::max(1, cfg.rp_div), false);
         }
     });
     return ok;
