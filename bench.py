@@ -397,6 +397,42 @@ def run_mixed_si216():
     return res
 
 
+def run_si216_passes(reps=3):
+    """Per-pass times of the Si216 eigen stage (the workload of
+    run_mixed_si216): FP64, and the reduced-precision pass with the FP32
+    filter and split-BF16x3 Gram products and rotations.  Each mode runs SCF
+    iteration 0 and one warm iteration 1, pauses 0.5 s (the pause separates
+    the timed passes in a kernel trace), then times `reps` iteration-1
+    passes."""
+    from systems import si216_mixed
+    from lynx_amd import core
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    s = si216_mixed()
+    nb, ne, beta = 480, 4.0 * 216, 1.0 / (3.166811563e-6 * 1000.0)
+    X0 = np.random.default_rng(7).uniform(-0.5, 0.5, (nb, s.nd))
+    out = {}
+    for name, kw in (("fp64", {}), ("fp32_bf16x3", dict(fp64_switch_residual=1e-4, low_precision="fp32",
+                                                       gram="bf16x3"))):
+        H = s.hamiltonian(0)
+        drv = core.ScfEigenStage(H, nb, ne, beta, **kw)
+        d = DeviceArray.from_numpy(X0)
+        drv.step_device(0, 1.0, s.veff, [d.value], s.nd)
+        drv.step_device(1, 1e-3, s.veff, [d.value], s.nd)
+        synchronize()
+        time.sleep(0.5)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            drv.step_device(1, 1e-3, s.veff, [d.value], s.nd)
+            synchronize()
+            ts.append(time.perf_counter() - t0)
+        out[name + "_pass_ms"] = round(1e3 * min(ts), 2)
+        d.free()
+        del drv, H
+    out["speedup"] = round(out["fp64_pass_ms"] / out["fp32_bf16x3_pass_ms"], 3)
+    return out
+
+
 KB_HA = 3.1668115634556e-6   # Boltzmann constant, Ha/K
 
 
@@ -624,7 +660,12 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time legs (Si8 and Si216)")
     ap.add_argument("--stencil-shape", default=None, metavar="VEC,THREADS,DIV,ZCHUNKS",
                     help="force the FP64 stencil launch shape instead of the autotuned one (e.g. 1,512,2,1)")
+    ap.add_argument("--si216-passes", action="store_true",
+                    help="only time single Si216 CheFSI passes (FP64 and the mixed pass) and print them")
     args = ap.parse_args()
+    if args.si216_passes:
+        print(json.dumps({"si216_passes": run_si216_passes()}), flush=True)
+        return
     shape = tuple(int(v) for v in args.stencil_shape.split(",")) if args.stencil_shape else None
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -281,7 +281,9 @@ lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision);
  * runs them as split-BF16x3 products on the matrix cores: each operand split
  * into a bfloat16 head and tail, hi.hi + hi.lo + lo.hi with FP32
  * accumulation per K chunk, chunks summed in FP64 (about 4e-6 relative
- * against FP64; FP32 GEMMs about 2e-7).  Complex blocks, and real blocks
+ * against FP64; FP32 GEMMs about 2e-7), and the pass's two rotations (by
+ * the inverse Cholesky factor and by the Ritz vectors) likewise, from the
+ * FP64 factors, the second written straight to the FP64 psi.  Complex blocks, and real blocks
  * whose columns are not 16-byte aligned, use FP32 GEMMs either way.  No
  * reference counterpart: the reference forms these products in FP64
  * (EigenSolver.cpp:621-660). */
@@ -297,6 +299,18 @@ lynx_status lynx_get_gram_precision(const lynx_ctx *ctx, int *mode);
  * and project_hamiltonian (EigenSolver.cpp:621-660). */
 lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const void *b, int ldb, int n, void *c,
                       int mem);
+/* c = a q for a block of n columns (Nd rows, leading dimensions >= Nd; a
+ * and c in host or device memory per mem, in dtype's precision) and an
+ * n x n matrix q in host memory, FP64 (real dtypes) or complex128 (complex
+ * ones), column-major: the rotation of EigenSolver::rotate_orbitals and of
+ * the Cholesky step (EigenSolver.cpp:622-641, 674-681), through the product
+ * the solve uses for dtype: FP64 / complex128 GEMMs; FP32 GEMMs of q
+ * rounded to the block's precision; or, for real FP32 blocks with
+ * LYNX_GRAM_BF16X3, split-BF16x3 matrix-core products of the block and the
+ * FP64 q (the reduced-precision pass uses the same setting for its two
+ * rotations). */
+lynx_status lynx_rotate(lynx_ctx *ctx, int dtype, const void *a, int lda, const double *q, int n, void *c, int ldc,
+                        int mem);
 /* Number of lynx_set_veff calls made on the context so far (accepted or
  * refused).  A caller that skips re-uploading an unchanged potential keeps
  * the generation of its own last upload and uploads again when another

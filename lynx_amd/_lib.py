@@ -67,6 +67,7 @@ SIGNATURES = {
     "lynx_set_gram_precision": (_I, [_P, _I]),
     "lynx_get_gram_precision": (_I, [_P, _IP]),
     "lynx_gram": (_I, [_P, _I, _P, _I, _P, _I, _I, _P, _I]),
+    "lynx_rotate": (_I, [_P, _I, _P, _I, _DP, _I, _P, _I, _I]),
     "lynx_get_veff_generation": (_I, [_P, ctypes.POINTER(_L)]),
     "lynx_solve_count": (_I, [_P, ctypes.POINTER(_L)]),
     "lynx_fermi_level": (_I, [_I, _I, _I, _DP, _DP, _D, _D, _I, _DP, _DP]),

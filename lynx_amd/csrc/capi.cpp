@@ -1357,17 +1357,35 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
                         return false;
                     }
                 };
-                void *Rs = ctx->s3.get((size_t)nband * nband * gpu::elem_size(es1));
+                // the rotations of real blocks run as split-BF16x3 matrix-core
+                // products with the Gram products' setting, from the FP64
+                // factors directly; otherwise FP32 GEMMs of FP32 copies
+                const bool rot3 = es1 == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3;
+                void *Rs = rot3 ? ctx->s3.get(gpu::rotate_bf16x3_work(nband, nband))
+                                : ctx->s3.get((size_t)nband * nband * gpu::elem_size(es1));
+                auto rot = [&](const void *A, const void *Q64, void *C, long ldc, bool c_double) {
+                    if (rot3) {
+                        gpu::rotate_bf16x3(static_cast<const float *>(A), ldw, nd, static_cast<const double *>(Q64),
+                                           nband, nband, nband, C, ldc, c_double, Rs, ctx->stream);
+                        return;
+                    }
+                    gpu::convert_cols(dtype, Q64, nband, es1, Rs, nband, nband, nband, ctx->stream);
+                    if (!c_double) {
+                        rotate(ctx, es1, A, ldw, Rs, nband, C, ldc);
+                        return;
+                    }
+                    void *tmp = A == Xs ? Ys : Xs;    // the other single-precision block is free
+                    rotate(ctx, es1, A, ldw, Rs, nband, tmp, ldw);
+                    gpu::convert_cols(es1, tmp, ldw, dtype, C, ldc, nd, nband, ctx->stream);
+                };
                 void *Cs = Xs, *Free = Ys;
                 if (!chol(Ys)) return false;
-                gpu::convert_cols(dtype, ctx->s1.p, nband, es1, Rs, nband, nband, nband, ctx->stream);
-                rotate(ctx, es1, Ys, ldw, Rs, nband, Cs, ldw);
+                rot(Ys, ctx->s1.p, Cs, ldw, false);
                 const double spread = triangular_diag_ratio(ctx, dtype, nband);
                 if (spread > kCholQrFp64Ratio) return false;
                 if (spread > kCholQr2Ratio) {
                     if (!chol(Cs)) return false;
-                    gpu::convert_cols(dtype, ctx->s1.p, nband, es1, Rs, nband, nband, nband, ctx->stream);
-                    rotate(ctx, es1, Cs, ldw, Rs, nband, Free, ldw);
+                    rot(Cs, ctx->s1.p, Free, ldw, false);
                     std::swap(Cs, Free);
                     ++ctx->cholqr2_passes;
                 }
@@ -1386,10 +1404,8 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
                 }
                 if (!(eigvals[0] >= eig_min - 0.05 * (eig_max - eig_min)) || !(eigvals[nband - 1] <= eig_max))
                     return false;
-                // a12: psi = Cs Q
-                gpu::convert_cols(dtype, H, nband, es1, Rs, nband, nband, nband, ctx->stream);
-                rotate(ctx, es1, Cs, ldw, Rs, nband, Free, ldw);
-                gpu::convert_cols(es1, Free, ldw, dtype, P, ldP, nd, nband, ctx->stream);
+                // a12: psi = Cs Q, written to the FP64 psi
+                rot(Cs, H, P, ldP, true);
                 return true;
             };
             if (fp32_pass()) {
@@ -1735,6 +1751,52 @@ lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const vo
         void *C = ctx->s2.get((size_t)n * n * ews);
         gram(ctx, e, da, lda, db, ldb, n, ctx->st.dV, C);
         HIP_OK(hipMemcpyAsync(c, C, (size_t)n * n * ews, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+lynx_status lynx_rotate(lynx_ctx *ctx, int dtype, const void *a, int lda, const double *q, int n, void *c, int ldc,
+                        int mem)
+{
+    if (!ctx || !a || !q || !c || n < 1) return LYNX_E_ARG;
+    return guarded(ctx, [&] {
+        need_grid(ctx);
+        if (dtype != LYNX_F64 && dtype != LYNX_C128 && dtype != LYNX_F32 && dtype != LYNX_C64)
+            fail(LYNX_E_ARG, "lynx_rotate: dtype must be f64, c128, f32 or c64");
+        if (lda < ctx->nd || ldc < ctx->nd) fail(LYNX_E_ARG, "lynx_rotate: leading dimension below Nd");
+        if (mem != LYNX_MEM_HOST && mem != LYNX_MEM_DEVICE)
+            fail(LYNX_E_ARG, "lynx_rotate: mem must be LYNX_MEM_HOST or LYNX_MEM_DEVICE");
+        const int e = dtype, ew = wide_of(e);
+        const size_t es = gpu::elem_size(e), ews = gpu::elem_size(ew);
+        const long nd = ctx->nd;
+        const void *A = a;
+        void *C = c;
+        long la = lda, lc = ldc;
+        if (mem == LYNX_MEM_HOST) {
+            const long ld = padded_ld(nd);
+            void *dA = ctx->wA.get((size_t)ld * n * es);
+            HIP_OK(hipMemcpy2DAsync(dA, ld * es, a, (size_t)lda * es, nd * es, n, hipMemcpyHostToDevice,
+                                    ctx->stream));
+            A = dA;
+            C = ctx->wB.get((size_t)ld * n * es);
+            la = lc = ld;
+        }
+        void *Q = ctx->s2.get((size_t)n * n * ews);
+        HIP_OK(hipMemcpyAsync(Q, q, (size_t)n * n * ews, hipMemcpyHostToDevice, ctx->stream));
+        if (e == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3) {
+            void *W = ctx->s3.get(gpu::rotate_bf16x3_work(n, n));
+            gpu::rotate_bf16x3(static_cast<const float *>(A), la, nd, static_cast<const double *>(Q), n, n, n, C, lc,
+                               false, W, ctx->stream);
+        } else if (e != ew) {
+            void *Qs = ctx->s3.get((size_t)n * n * es);
+            gpu::convert_cols(ew, Q, n, e, Qs, n, n, n, ctx->stream);
+            rotate(ctx, e, A, la, Qs, n, C, lc);
+        } else {
+            rotate(ctx, e, A, la, Q, n, C, lc);
+        }
+        if (mem == LYNX_MEM_HOST)
+            HIP_OK(hipMemcpy2DAsync(c, (size_t)ldc * es, C, lc * es, nd * es, n, hipMemcpyDeviceToHost,
+                                    ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
     });
 }

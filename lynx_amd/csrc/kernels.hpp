@@ -118,6 +118,14 @@ void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void 
 int gram_bf16x3(const float *A, long lda, const float *B, long ldb, long K, int N, float *part, int nsplit,
                 hipStream_t st);
 int gram_bf16x3_splits(long K, int N);
+// Split-BF16x3 rotation on the matrix cores (kernels_mfma.hip): C (M x N) =
+// A (M x K) Q (K x N) for a real FP32 block A (column-major, ld lda) and an
+// FP64 Q (ld ldq), both split into bfloat16 hi + lo, hi.hi + hi.lo + lo.hi
+// accumulated in FP32; C is written as float, or as double when c_double.
+// work: rotate_bf16x3_work(K, N) bytes of device memory (Q's split halves).
+size_t rotate_bf16x3_work(int K, int N);
+void rotate_bf16x3(const float *A, long lda, long M, const double *Q, int ldq, int K, int N, void *C, long ldc,
+                   bool c_double, void *work, hipStream_t st);
 // Small dense helpers on n x n column-major matrices.
 void hermitian_part(int e, void *H, int n, hipStream_t st);
 void zero_lower(int e, void *H, int n, hipStream_t st);

@@ -112,6 +112,12 @@ __device__ inline bf16x8 frag(const unsigned short *s, int col, int k)
     return *reinterpret_cast<const bf16x8 *>(s + col * kPitch + k);
 }
 
+// eight bf16 of row r (pitch 16 + 8) from k on, for the rotation's LDS rows
+__device__ inline bf16x8 frag_p(const unsigned short *s, int r, int k)
+{
+    return *reinterpret_cast<const bf16x8 *>(s + r * (16 + 8) + k);
+}
+
 // Tile pair of workgroup b: all tiles_n^2 pairs, or for A^T A only the
 // pairs tm <= tn (row-major over the upper triangle); the lower tiles of
 // A^T A are written as the transposes of the upper ones.
@@ -207,7 +213,188 @@ __global__ void __launch_bounds__(kThreads) k_gram_bf16x3(const float *__restric
     }
 }
 
+// ---------------------------------------------------------------- rotation --
+// C (M x N) = A (M x K) Q (K x N) for a real FP32 block A (M grid points,
+// column-major, ld lda) and an FP64 Q, as split-BF16x3 products: computed as
+// C^T = Q^T A^T, so that the matrix cores' output rows run along the grid
+// points and each 32-lane half-wave stores 32 consecutive points of one
+// column (coalesced).  Q is split once (k_split_q) into bfloat16 hi and lo,
+// n-major with k contiguous and zero padding to Kp x Npad; A is split while
+// it is staged.  A workgroup of 4 * NS waves owns 128 grid points and up to
+// 160 * NS columns; wave w takes the 32-point block w % 4 and the five
+// 32-column blocks of strip w / 4.  Per 16-deep K block, the first four
+// waves stage A (lane = grid point, eight K values, each load coalesced
+// across the lanes) and all waves stage Q, both as (hi, lo) rows with k
+// contiguous; the next block's loads are in flight during the products.
+constexpr int kRB = 128;             // grid points per workgroup
+constexpr int kRK = 16;              // K per LDS block (one MFMA step)
+constexpr int kRP = kRK + 8;         // bf16 per LDS row, padded
+constexpr int kRNB = 5;              // 32-column blocks per wave
+
+__global__ void k_split_q(const double *__restrict__ Q, int ldq, int K, int N, int Kp, int Npad,
+                          unsigned short *__restrict__ qh, unsigned short *__restrict__ ql)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)Npad * Kp) return;
+    const int n = (int)(i / Kp), k = (int)(i % Kp);
+    const double q = (n < N && k < K) ? Q[k + (long)n * ldq] : 0.0;
+    const unsigned short h = bf16_bits((float)q);
+    qh[i] = h;
+    ql[i] = bf16_bits((float)(q - (double)bf16_value(h)));
+}
+
+template <int NS, bool DOUT>
+__global__ void __launch_bounds__(256 * NS) k_rotate_bf16x3(const float *__restrict__ A, long lda, long M, int K,
+                                                           const unsigned short *__restrict__ qh,
+                                                           const unsigned short *__restrict__ ql, int Kp, int N,
+                                                           void *__restrict__ Cv, long ldc, int mtiles, int per_xcd)
+{
+    constexpr int NT = 256 * NS;
+    constexpr int NC = 160 * NS;                    // columns per workgroup
+    constexpr int QCH = NC * 2 * (kRK / 8);         // 16-byte (hi or lo) chunks of Q per K block
+    constexpr int QPT = (QCH + NT - 1) / NT;
+    const int item = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (item >= mtiles) return;
+    __shared__ __attribute__((aligned(16))) unsigned short sAh[kRB * kRP], sAl[kRB * kRP];
+    __shared__ __attribute__((aligned(16))) unsigned short sQh[NC * kRP], sQl[NC * kRP];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const long m0 = (long)item * kRB;
+    const int n0 = blockIdx.y * NC;                 // first column of this workgroup
+    const int Npad = (N + 31) / 32 * 32;
+    const int mb = (w & 3) * 32, ns = w >> 2;       // this wave's 32 points and column strip
+    // columns of the strip that exist (blocks of 32 beyond N are skipped)
+    int nblk = 0;
+#pragma unroll
+    for (int j = 0; j < kRNB; ++j)
+        if (n0 + (ns * kRNB + j) * 32 < Npad) nblk = j + 1;
+
+    // A staging: thread t < 256 holds grid point m0 + t % 128, K values
+    // 8 (t / 128) .. + 7 of the block
+    const long am = m0 + (t & 127);
+    const int akg = (t >> 7) & 1;
+    float av[8];
+    uint4 qv[QPT];
+    auto load = [&](int k0) {
+        if (t < 256) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = k0 + 8 * akg + i;
+                av[i] = (am < M && k < K) ? A[am + (long)k * lda] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QPT; ++q) {
+            const int c = t + q * NT;
+            if (c < QCH) {
+                // chunk c: column n = c / 4 of this workgroup, half (hi, lo) and k offset
+                const int n = c >> 2, part = (c >> 1) & 1, ko = (c & 1) * 8;
+                const int gn = n0 + n;
+                const unsigned short *src = part ? ql : qh;
+                qv[q] = gn < Npad ? *reinterpret_cast<const uint4 *>(src + (long)gn * Kp + k0 + ko)
+                                  : make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto store = [&]() {
+        if (t < 256) {
+            unsigned short h[8], lo[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                h[i] = bf16_bits(av[i]);
+                lo[i] = bf16_bits(av[i] - bf16_value(h[i]));
+            }
+            uint4 a, b;
+            a.x = h[0] | ((unsigned)h[1] << 16); a.y = h[2] | ((unsigned)h[3] << 16);
+            a.z = h[4] | ((unsigned)h[5] << 16); a.w = h[6] | ((unsigned)h[7] << 16);
+            b.x = lo[0] | ((unsigned)lo[1] << 16); b.y = lo[2] | ((unsigned)lo[3] << 16);
+            b.z = lo[4] | ((unsigned)lo[5] << 16); b.w = lo[6] | ((unsigned)lo[7] << 16);
+            *reinterpret_cast<uint4 *>(sAh + (t & 127) * kRP + 8 * akg) = a;
+            *reinterpret_cast<uint4 *>(sAl + (t & 127) * kRP + 8 * akg) = b;
+        }
+#pragma unroll
+        for (int q = 0; q < QPT; ++q) {
+            const int c = t + q * NT;
+            if (c < QCH) {
+                const int n = c >> 2, part = (c >> 1) & 1, ko = (c & 1) * 8;
+                *reinterpret_cast<uint4 *>((part ? sQl : sQh) + n * kRP + ko) = qv[q];
+            }
+        }
+    };
+
+    floatx16 acc[kRNB];
+#pragma unroll
+    for (int j = 0; j < kRNB; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[j][r] = 0.0f;
+    const int kl = 8 * (l >> 5);
+    load(0);
+    for (int k0 = 0; k0 < Kp; k0 += kRK) {
+        store();
+        __syncthreads();
+        if (k0 + kRK < Kp) load(k0 + kRK);
+        // B operand: A^T (k x point), lane supplies A[point mb + l % 32][k]
+        const bf16x8 bh = frag_p(sAh, mb + (l & 31), kl), blo = frag_p(sAl, mb + (l & 31), kl);
+#pragma unroll
+        for (int j = 0; j < kRNB; ++j) {
+            if (j < nblk) {
+                // A operand: Q^T (column x k), lane supplies Q[k][column]
+                const int cn = (ns * kRNB + j) * 32 + (l & 31);
+                const bf16x8 qhf = frag_p(sQh, cn, kl), qlf = frag_p(sQl, cn, kl);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qhf, bh, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qhf, blo, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qlf, bh, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[column][point]: lane l holds column 8 (r / 4) + 4 (l / 32) + r % 4
+    // of the block and point l % 32
+    const long m = m0 + mb + (l & 31);
+#pragma unroll
+    for (int j = 0; j < kRNB; ++j) {
+        if (j >= nblk) continue;
+        const int cb = n0 + (ns * kRNB + j) * 32;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int n = cb + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+            if (m < M && n < N) {
+                if constexpr (DOUT) static_cast<double *>(Cv)[m + (long)n * ldc] = (double)acc[j][r];
+                else static_cast<float *>(Cv)[m + (long)n * ldc] = acc[j][r];
+            }
+        }
+    }
+}
+
 }  // namespace
+
+size_t rotate_bf16x3_work(int K, int N)
+{
+    const int Kp = (K + kRK - 1) / kRK * kRK, Npad = (N + 31) / 32 * 32;
+    return (size_t)Kp * Npad * 2 * sizeof(unsigned short);
+}
+
+void rotate_bf16x3(const float *A, long lda, long M, const double *Q, int ldq, int K, int N, void *C, long ldc,
+                   bool c_double, void *work, hipStream_t st)
+{
+    if (M < 1 || K < 1 || N < 1) throw std::runtime_error("rotate_bf16x3: empty product");
+    const int Kp = (K + kRK - 1) / kRK * kRK, Npad = (N + 31) / 32 * 32;
+    unsigned short *qh = static_cast<unsigned short *>(work);
+    unsigned short *ql = qh + (size_t)Kp * Npad;
+    const long nq = (long)Kp * Npad;
+    k_split_q<<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(Q, ldq, K, N, Kp, Npad, qh, ql);
+    const int mtiles = (int)((M + kRB - 1) / kRB), per_xcd = (mtiles + 7) / 8;
+    // column strips of 160 per workgroup (one to three), more workgroup rows beyond 480
+    const int ns = std::min(3, (Npad + 159) / 160);
+    const dim3 grid(8 * per_xcd, (Npad + 160 * ns - 1) / (160 * ns));
+    auto go = [&](auto kern, int nt) {
+        hipLaunchKernelGGL(kern, grid, dim3(nt), 0, st, A, lda, M, K, qh, ql, Kp, N, C, ldc, mtiles, per_xcd);
+    };
+    if (ns == 1) c_double ? go(k_rotate_bf16x3<1, true>, 256) : go(k_rotate_bf16x3<1, false>, 256);
+    else if (ns == 2) c_double ? go(k_rotate_bf16x3<2, true>, 512) : go(k_rotate_bf16x3<2, false>, 512);
+    else c_double ? go(k_rotate_bf16x3<3, true>, 768) : go(k_rotate_bf16x3<3, false>, 768);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("k_rotate_bf16x3: ") + hipGetErrorString(e));
+}
 
 int gram_bf16x3_splits(long K, int N)
 {
