@@ -500,7 +500,16 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
         hys[h] = ys;
     }
 
-    const int f_end = z1 - 1 + R;
+    // A workgroup that owns the whole periodic z range marches the nz planes
+    // once (no 2R wrapped planes): the R outputs at each end then lack the
+    // contributions of the planes across the z boundary.  The first R steps
+    // leave the bottom outputs' wrapped part (planes 0..R-1 seen from
+    // nz-R..nz-1, Bloch factor e^{+ikL}) as partial results in Y, the next R
+    // steps store the top outputs 0..R-1 without their wrapped part, and
+    // after the march the ring holds the rest of both, which is added to
+    // those lines of Y (the planes nz-R..nz-1 seen from 0..R-1 with e^{-ikL}).
+    const bool wrapz = p.nzch == 1 && nz >= 2 * R;
+    const int f_end = wrapz ? z1 - 1 : z1 - 1 + R;
     // every load of plane f (owned points and halo rows, z-wrapped) and of
     // the output plane f - R (Y_{k-2}, Veff), clamped into range
     auto issue = [&](int f, StepLoads<S, RP, HP> &L) {
@@ -622,6 +631,13 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
                 S res = scl(A(-0.5) * io.a, acc[r][0]);
                 if (has_old) res = fm(io.b, cur.old[r], res);
                 if (qv[r]) st_at(Yz, qoff[r], res);
+            } else if (wrapz && qv[r]) {
+                // bottom output nz + f - R: its contributions from planes
+                // 0..f, across the boundary
+                const int zo = (nz + f - R) * nxny;
+                S res = scl(A(-0.5) * io.a, phased(acc[r][0], p.ph[2][1][0], p.ph[2][1][1]));
+                if (has_old) res = fm(io.b, ld_at(O + zo, qoff[r]), res);
+                st_at(Y + zo, qoff[r], res);
             }
 #pragma unroll
             for (int j = 0; j + 1 < NA; ++j) acc[r][j] = acc[r][j + 1];
@@ -633,13 +649,41 @@ __device__ __forceinline__ void stencil_nonorth_body(const NonorthArgs<typename 
     // a ring of PD + 1 load sets: the step on plane f consumes its set and
     // refills the one the previous step consumed with plane f + PD
     StepLoads<S, RP, HP> lring[PD + 1];
-    int f = z0 - R;
+    int f = wrapz ? z0 : z0 - R;
 #pragma unroll
     for (int d = 0; d < PD; ++d) issue(f + d, lring[d]);
     for (; f <= f_end; f += PD + 1) {
 #pragma unroll
         for (int d = 0; d <= PD; ++d)
             if (f + d <= f_end) step(f + d, lring[d], lring[(d + PD) % (PD + 1)]);
+    }
+    if (!wrapz) return;
+    // the xy term of the last plane (its g plane is complete after a barrier)
+    __syncthreads();
+    {
+        const S *LGp = lds0 + (buf ^ 1) * 2 * plane + plane;
+#pragma unroll
+        for (int r = 0; r < RP; ++r) {
+            const S *g = LGp + qli[r];
+            S mxy = zero<S>();
+#pragma unroll
+            for (int k = 1; k <= R; ++k) mxy = fm(q.w1[k], add(g[k], scl(A(-1), g[-k])), mxy);
+            acc[r][R - 1] = fm(q.axy_hy, mxy, acc[r][R - 1]);
+        }
+    }
+    // ring slot j holds output nz - R + j: the rest of the bottom outputs
+    // (j < R), and for j >= R the planes nz-R..nz-1 seen from top output
+    // j - R across the boundary
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+        if (!qv[r]) continue;
+#pragma unroll
+        for (int j = 0; j < 2 * R; ++j) {
+            const int o = j < R ? nz - R + j : j - R;
+            const S c = j < R ? acc[r][j] : phased(acc[r][j], p.ph[2][0][0], p.ph[2][0][1]);
+            S *Yo = Y + o * nxny;
+            st_at(Yo, qoff[r], fm(A(-0.5) * io.a, c, ld_at(Yo, qoff[r])));
+        }
     }
 }
 
