@@ -2014,6 +2014,7 @@ struct BatchPlan {
         int nitems = 0;
         int max_tyr = 0;
         bool small_halo = true;   // fewer halo slots per thread suffice for every unit of the group
+        int nt = 256;             // threads per workgroup (192 for short-row non-orthogonal groups)
     };
     struct NGroup {
         int npm = 8;
@@ -2053,15 +2054,32 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
 {
     using A = typename Tr<S>::A;
     std::vector<BatchRec<A>> recs(n);
-    // stencil groups by (orth, pitch), nonlocal groups by npm
+    // stencil groups by (orth, pitch, threads), nonlocal groups by npm
     std::vector<int> sgi(n), ngi(n, -1);
-    std::vector<int> kind(n);
+    std::vector<int> kind(n), ntu(n, kBatchNT);
+    // lanes of a non-orthogonal tile with nt threads that hold a point
+    auto nonorth_fill = [&](const StencilGeom &g, int nt, int W) {
+        const long pts = (long)nt * batch_rp_nonorth<S>();
+        const int max_rows = nonorth_max_rows<S>(W, nt);
+        if (max_rows < 1 || g.nx > pts) return 0.0;
+        int nb = (int)std::max<long>(1, std::min<long>(((long)g.nx * g.ny + pts - 1) / pts, g.ny));
+        int tyr = (g.ny + nb - 1) / nb;
+        while ((long)tyr * g.nx > pts || tyr > max_rows) { ++nb; tyr = (g.ny + nb - 1) / nb; }
+        nb = (g.ny + tyr - 1) / tyr;
+        return (double)g.nx * g.ny / ((double)nb * pts);
+    };
     for (int u = 0; u < n; ++u) {
         const StencilGeom &g = units[u].g;
         if (g.nx + 2 * kMaxR > 128) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
         kind[u] = g.orth ? 0 : 1;
         int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
         bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
+        // short non-orthogonal rows: 192 threads (two halo slots each) when
+        // they leave fewer lanes idle than 256, e.g. 8 rows of 24 points
+        if (!g.orth && 2 * kMaxR * g.nx <= 192 * 2 && nonorth_fill(g, 192, W) > nonorth_fill(g, kBatchNT, W)) {
+            ntu[u] = 192;
+            sh = true;
+        }
         if constexpr (!Tr<S>::C) {
             // kind 2: 16-byte runs; kind 3: 8-byte runs of two floats for
             // rows whose length is not a multiple of four
@@ -2086,9 +2104,10 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         }
         int k = -1;
         for (size_t i = 0; i < P->sg.size(); ++i)
-            if (P->sg[i].kind == kind[u] && P->sg[i].W == W && P->sg[i].small_halo == sh) k = (int)i;
+            if (P->sg[i].kind == kind[u] && P->sg[i].W == W && P->sg[i].small_halo == sh && P->sg[i].nt == ntu[u])
+                k = (int)i;
         if (k < 0) {
-            P->sg.push_back({kind[u], W, nullptr, 0, 0, sh});
+            P->sg.push_back({kind[u], W, nullptr, 0, 0, sh, ntu[u]});
             k = (int)P->sg.size() - 1;
         }
         sgi[u] = k;
@@ -2124,11 +2143,12 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
             const int nb = (g.ny + rpb - 1) / rpb;
             p.tyr = (g.ny + nb - 1) / nb;
         } else {
-            const long pts = (long)kBatchNT * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
-            if (g.nx > pts || 2 * kMaxR * g.nx > kBatchNT * 6)
+            const int nt = ntu[u];
+            const long pts = (long)nt * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
+            if (g.nx > pts || 2 * kMaxR * g.nx > nt * (nt == kBatchNT ? 6 : 2))
                 fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
             p.lpr = 0;
-            const int max_rows = g.orth ? g.ny : nonorth_max_rows<S>(P->sg[sgi[u]].W, kBatchNT);
+            const int max_rows = g.orth ? g.ny : nonorth_max_rows<S>(P->sg[sgi[u]].W, nt);
             int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
             nb = std::max(1, std::min(nb, g.ny));
             p.tyr = (g.ny + nb - 1) / nb;
@@ -2216,6 +2236,15 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
             else if constexpr (sizeof(S) == 4)
                 hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 3>), dim3(blocks), dim3(kBatchNT),
                                    shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+        }
+        return;
+    }
+    if (G.kind == 1 && G.nt == 192) {
+        // short non-orthogonal rows, two halo slots per thread
+        if constexpr (W == 64) {
+            const size_t shm = 4 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 192);
+            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 192, 2, W, 1>), dim3(blocks),
+                               dim3(192), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
         }
         return;
     }
