@@ -399,8 +399,9 @@ def run_mixed_si216():
 
 def run_si216_passes(reps=3):
     """Per-pass times of the Si216 eigen stage (the workload of
-    run_mixed_si216): FP64, and the reduced-precision pass with the FP32
-    filter and split-BF16x3 Gram products and rotations.  Each mode runs SCF
+    run_mixed_si216): FP64; the reduced-precision pass with the FP32 filter
+    and split-BF16x3 Gram products (FP32 rotations); the same with the
+    rotations as split-BF16x3 products too.  Each mode runs SCF
     iteration 0 and one warm iteration 1, pauses 0.5 s (the pause separates
     the timed passes in a kernel trace), then times `reps` iteration-1
     passes."""
@@ -411,8 +412,9 @@ def run_si216_passes(reps=3):
     nb, ne, beta = 480, 4.0 * 216, 1.0 / (3.166811563e-6 * 1000.0)
     X0 = np.random.default_rng(7).uniform(-0.5, 0.5, (nb, s.nd))
     out = {}
-    for name, kw in (("fp64", {}), ("fp32_bf16x3", dict(fp64_switch_residual=1e-4, low_precision="fp32",
-                                                       gram="bf16x3"))):
+    for name, kw in (("fp64", {}),
+                     ("fp32_bf16x3", dict(fp64_switch_residual=1e-4, low_precision="fp32", gram="bf16x3")),
+                     ("fp32_bf16x3_rot", dict(fp64_switch_residual=1e-4, low_precision="fp32", gram="bf16x3_rot"))):
         H = s.hamiltonian(0)
         drv = core.ScfEigenStage(H, nb, ne, beta, **kw)
         d = DeviceArray.from_numpy(X0)
@@ -430,6 +432,7 @@ def run_si216_passes(reps=3):
         d.free()
         del drv, H
     out["speedup"] = round(out["fp64_pass_ms"] / out["fp32_bf16x3_pass_ms"], 3)
+    out["speedup_rot"] = round(out["fp64_pass_ms"] / out["fp32_bf16x3_rot_pass_ms"], 3)
     return out
 
 

@@ -281,13 +281,16 @@ lynx_status lynx_get_filter_precision(const lynx_ctx *ctx, int *precision);
  * runs them as split-BF16x3 products on the matrix cores: each operand split
  * into a bfloat16 head and tail, hi.hi + hi.lo + lo.hi with FP32
  * accumulation per K chunk, chunks summed in FP64 (about 4e-6 relative
- * against FP64; FP32 GEMMs about 2e-7), and the pass's two rotations (by
- * the inverse Cholesky factor and by the Ritz vectors) likewise, from the
- * FP64 factors, the second written straight to the FP64 psi.  Complex blocks, and real blocks
+ * against FP64; FP32 GEMMs about 2e-7); the rotations stay FP32 GEMMs.
+ * LYNX_GRAM_BF16X3_ROT also runs the pass's two rotations (by the inverse
+ * Cholesky factor and by the Ritz vectors) as split-BF16x3 products from
+ * the FP64 factors, the second written straight to the FP64 psi: faster per
+ * pass, but their 2^-17 relative error in psi can lengthen the
+ * reduced-precision phase of an SCF.  Complex blocks, and real blocks
  * whose columns are not 16-byte aligned, use FP32 GEMMs either way.  No
  * reference counterpart: the reference forms these products in FP64
  * (EigenSolver.cpp:621-660). */
-enum lynx_gram_precision { LYNX_GRAM_FP32 = 0, LYNX_GRAM_BF16X3 = 1 };
+enum lynx_gram_precision { LYNX_GRAM_FP32 = 0, LYNX_GRAM_BF16X3 = 1, LYNX_GRAM_BF16X3_ROT = 2 };
 lynx_status lynx_set_gram_precision(lynx_ctx *ctx, int mode);
 lynx_status lynx_get_gram_precision(const lynx_ctx *ctx, int *mode);
 /* c = dV a^H b (n x n, FP64 for real dtypes, complex128 for complex ones,
@@ -306,9 +309,8 @@ lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const vo
  * the Cholesky step (EigenSolver.cpp:622-641, 674-681), through the product
  * the solve uses for dtype: FP64 / complex128 GEMMs; FP32 GEMMs of q
  * rounded to the block's precision; or, for real FP32 blocks with
- * LYNX_GRAM_BF16X3, split-BF16x3 matrix-core products of the block and the
- * FP64 q (the reduced-precision pass uses the same setting for its two
- * rotations). */
+ * LYNX_GRAM_BF16X3_ROT, split-BF16x3 matrix-core products of the block and
+ * the FP64 q (the reduced-precision pass's rotations under that setting). */
 lynx_status lynx_rotate(lynx_ctx *ctx, int dtype, const void *a, int lda, const double *q, int n, void *c, int ldc,
                         int mem);
 /* Number of lynx_set_veff calls made on the context so far (accepted or
@@ -387,8 +389,9 @@ typedef struct {
     int smearing;                 /* 0 Fermi-Dirac, 1 Gaussian */
     double fp64_switch_residual;  /* 0: FP64 filter throughout */
     int low_precision;            /* before the switch: LYNX_FILTER_FP32 (or 0) or LYNX_FILTER_BF16 */
-    int gram_precision;           /* Gram products of the reduced-precision passes: LYNX_GRAM_FP32 (0) or
-                                     LYNX_GRAM_BF16X3 (split-BF16x3 on the matrix cores) */
+    int gram_precision;           /* Gram products of the reduced-precision passes: LYNX_GRAM_FP32 (0),
+                                     LYNX_GRAM_BF16X3 (split-BF16x3 on the matrix cores) or
+                                     LYNX_GRAM_BF16X3_ROT (the rotations too) */
 } lynx_scf_eig_params;
 lynx_status lynx_scf_eig_create(lynx_scf_eig **driver, lynx_ctx *ctx, const lynx_scf_eig_params *params,
                                 const double *kpts, const double *kpt_weights);

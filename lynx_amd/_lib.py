@@ -18,7 +18,7 @@ F64, C128, F32, C64, BF16 = 0, 1, 2, 3, 4
 FILTER_FP64, FILTER_FP32, FILTER_BF16 = 0, 1, 2
 XC_LDA_PZ, XC_LDA_PW, XC_GGA_PBE = 0, 1, 2
 SOLVE_SPECTRAL, SOLVE_AAR = 0, 1
-GRAM_FP32, GRAM_BF16X3 = 0, 1
+GRAM_FP32, GRAM_BF16X3, GRAM_BF16X3_ROT = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 CELL_ORTHOGONAL, CELL_NONORTHOGONAL = 0, 1
 

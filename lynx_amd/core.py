@@ -521,7 +521,7 @@ class ScfEigenStage:
     (``low_precision``: "fp32", or "bf16" for bfloat16 filter storage) until
     the SCF residual drops below it, with the Gram products of those passes
     as FP32 GEMMs or split-BF16x3 MFMA products (``gram``: "fp32" or
-    "bf16x3") (lynx_scf_eig_*)."""
+    "bf16x3"; "bf16x3_rot" also rotates that way) (lynx_scf_eig_*)."""
 
     def __init__(self, H, nband, nelectron, beta, kpts=None, kpt_weights=None, nspin=1, cheb_degree=20,
                  rho_trigger=4, nchefsi=1, smearing=0, fp64_switch_residual=0.0, low_precision="fp32", gram="fp32"):
@@ -533,7 +533,7 @@ class ScfEigenStage:
         p = _ScfEigParams(nspin, nk, int(self.complex_), nband, cheb_degree, rho_trigger, nchefsi,
                           float(nelectron), float(beta), smearing, float(fp64_switch_residual),
                           {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision],
-                          {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3}[gram])
+                          {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3, "bf16x3_rot": L.GRAM_BF16X3_ROT}[gram])
         kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
         self._d = ctypes.c_void_p()
         L.check(L.lib().lynx_scf_eig_create(ctypes.byref(self._d), H.ctx, ctypes.byref(p),

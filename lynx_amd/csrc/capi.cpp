@@ -575,7 +575,7 @@ constexpr double kCholQrFp64Ratio = 100.0;
 void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, int n, double alpha, void *C)
 {
     const long nd = c->nd;
-    if (e == gpu::F32 && c->gram_precision == LYNX_GRAM_BF16X3 && lda % 4 == 0 && ldb % 4 == 0 &&
+    if (e == gpu::F32 && c->gram_precision != LYNX_GRAM_FP32 && lda % 4 == 0 && ldb % 4 == 0 &&
         reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) {
         const int ns = gpu::gram_bf16x3_splits(nd, n);
         void *W = c->gwork.get((size_t)ns * n * n * sizeof(float));
@@ -1357,10 +1357,10 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
                         return false;
                     }
                 };
-                // the rotations of real blocks run as split-BF16x3 matrix-core
-                // products with the Gram products' setting, from the FP64
-                // factors directly; otherwise FP32 GEMMs of FP32 copies
-                const bool rot3 = es1 == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3;
+                // with LYNX_GRAM_BF16X3_ROT the rotations of real blocks run as
+                // split-BF16x3 matrix-core products from the FP64 factors
+                // directly; otherwise as FP32 GEMMs of FP32 copies
+                const bool rot3 = es1 == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3_ROT;
                 void *Rs = rot3 ? ctx->s3.get(gpu::rotate_bf16x3_work(nband, nband))
                                 : ctx->s3.get((size_t)nband * nband * gpu::elem_size(es1));
                 auto rot = [&](const void *A, const void *Q64, void *C, long ldc, bool c_double) {
@@ -1701,8 +1701,8 @@ lynx_status lynx_set_filter_precision(lynx_ctx *ctx, int precision)
 lynx_status lynx_set_gram_precision(lynx_ctx *ctx, int mode)
 {
     if (!ctx) return LYNX_E_ARG;
-    if (mode != LYNX_GRAM_FP32 && mode != LYNX_GRAM_BF16X3) {
-        ctx->err = "Gram precision must be LYNX_GRAM_FP32 or LYNX_GRAM_BF16X3";
+    if (mode != LYNX_GRAM_FP32 && mode != LYNX_GRAM_BF16X3 && mode != LYNX_GRAM_BF16X3_ROT) {
+        ctx->err = "Gram precision must be LYNX_GRAM_FP32, LYNX_GRAM_BF16X3 or LYNX_GRAM_BF16X3_ROT";
         return LYNX_E_ARG;
     }
     ctx->gram_precision = mode;
@@ -1783,7 +1783,7 @@ lynx_status lynx_rotate(lynx_ctx *ctx, int dtype, const void *a, int lda, const 
         }
         void *Q = ctx->s2.get((size_t)n * n * ews);
         HIP_OK(hipMemcpyAsync(Q, q, (size_t)n * n * ews, hipMemcpyHostToDevice, ctx->stream));
-        if (e == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3) {
+        if (e == gpu::F32 && ctx->gram_precision == LYNX_GRAM_BF16X3_ROT) {
             void *W = ctx->s3.get(gpu::rotate_bf16x3_work(n, n));
             gpu::rotate_bf16x3(static_cast<const float *>(A), la, nd, static_cast<const double *>(Q), n, n, n, C, lc,
                                false, W, ctx->stream);

@@ -62,8 +62,9 @@ class SCF:
     (smearing 0) or Gaussian (1) occupations at inverse temperature beta.
     ``solver`` picks how the Poisson and Kerker equations are solved:
     "spectral" (FFT, exact to rounding) or "aar" (the reference's iteration
-    to poisson_tol / precond_tol).  ``gram`` ("fp32" or "bf16x3") picks the
-    Gram products of the reduced-precision passes."""
+    to poisson_tol / precond_tol).  ``gram`` ("fp32", "bf16x3" or
+    "bf16x3_rot") picks the Gram products (and with "bf16x3_rot" the
+    rotations) of the reduced-precision passes."""
 
     def __init__(self, H, nband, beta, xc="GGA_PBE", nspin=1, kpts=None, kpt_weights=None, smearing=0,
                  cheb_degree=20, rho_trigger=4, nchefsi=1, tol=1e-6, max_iter=100, min_iter=2, poisson_tol=0.0,
@@ -78,7 +79,7 @@ class SCF:
                       float(poisson_tol), float(precond_tol), mixing_history, float(mixing_param), int(kerker),
                       float(fp64_switch_residual), {"fp32": L.FILTER_FP32, "bf16": L.FILTER_BF16}[low_precision],
                       {"spectral": L.SOLVE_SPECTRAL, "aar": L.SOLVE_AAR}[solver],
-                      {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3}[gram])
+                      {"fp32": L.GRAM_FP32, "bf16x3": L.GRAM_BF16X3, "bf16x3_rot": L.GRAM_BF16X3_ROT}[gram])
         kw = None if kpt_weights is None else np.ascontiguousarray(kpt_weights, float)
         self._s = ctypes.c_void_p()
         L.check(L.lib().lynx_scf_create(ctypes.byref(self._s), H.ctx, ctypes.byref(p),

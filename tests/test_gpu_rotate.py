@@ -4,8 +4,8 @@ measured against the entry's scale |a_i| |q_j| (row i of a, column j of q):
 
 - FP64 blocks: rocBLAS GEMM, 1e-14;
 - FP32 blocks with LYNX_GRAM_FP32: q rounded to FP32, FP32 GEMM, 5e-7;
-- FP32 blocks with LYNX_GRAM_BF16X3 (the hand-written split-BF16x3 MFMA
-  kernel, q split from FP64): 2e-5 (each operand is held to about 2^-17,
+- FP32 blocks with LYNX_GRAM_BF16X3_ROT (the hand-written split-BF16x3
+  MFMA kernel, q split from FP64): 2e-5 (each operand is held to about 2^-17,
   the products accumulate in FP32); exactly bfloat16 a and q (the tails
   vanish) to the FP32 accumulation alone, 5e-7;
 - shapes that exercise the kernel's edges: one, two and three 160-column
@@ -67,11 +67,11 @@ def test_rotation_matches_fp64(grid, n):
     e64 = err(_rotate(L, H, L.F64, A, Q), A, Q)
     A32 = A.astype(np.float32)
     out = {}
-    for mode in (L.GRAM_FP32, L.GRAM_BF16X3):
+    for mode in (L.GRAM_FP32, L.GRAM_BF16X3, L.GRAM_BF16X3_ROT):
         L.check(L.lib().lynx_set_gram_precision(H.ctx, mode), H.ctx)
         out[mode] = _rotate(L, H, L.F32, A32, Q)
     e_fp32 = err(out[L.GRAM_FP32], A32, Q)
-    e_bf3 = err(out[L.GRAM_BF16X3], A32, Q)
+    e_bf3 = err(out[L.GRAM_BF16X3_ROT], A32, Q)
     # exactly bfloat16 operands: only the FP32 accumulation rounds
     Ab, Qb = bf16_round(A32), bf16_round(Q).astype(np.float64)
     e_exact = err(_rotate(L, H, L.F32, Ab, Qb), Ab, Qb)
@@ -82,7 +82,8 @@ def test_rotation_matches_fp64(grid, n):
     assert e_fp32 < 5e-7
     assert e_bf3 < 2e-5
     assert e_exact < 5e-7
-    assert np.array_equal(dev, out[L.GRAM_BF16X3])
-    # the split changes what it should: the bf16x3 result is not the FP32 GEMM's
-    assert not np.array_equal(out[L.GRAM_FP32], out[L.GRAM_BF16X3])
+    assert np.array_equal(dev, out[L.GRAM_BF16X3_ROT])
+    # LYNX_GRAM_BF16X3 leaves the rotation an FP32 GEMM; the split changes it
+    assert np.array_equal(out[L.GRAM_FP32], out[L.GRAM_BF16X3])
+    assert not np.array_equal(out[L.GRAM_FP32], out[L.GRAM_BF16X3_ROT])
     L.check(L.lib().lynx_set_gram_precision(H.ctx, L.GRAM_FP32), H.ctx)
