@@ -39,13 +39,16 @@ def test_eigen_stage_matches_oracle_fp64(si8_16):
         assert np.abs(occ - ref.occ).max() < 1e-7
 
 
-def test_eigen_stage_fp32_until_residual(si8_16):
+@pytest.mark.parametrize("gram", ["fp32", "bf16x3", "bf16x3_rot"])
+def test_eigen_stage_fp32_until_residual(si8_16, gram):
+    # the reduced-precision pass with FP32 Gram products and rotations, with
+    # split-BF16x3 Gram products, and with split-BF16x3 rotations too
     from lynx_amd import core
     s = si8_16
     H64, H32 = s.hamiltonian(), s.hamiltonian()
     X0 = s.random_block(NB, 82)
     d64 = core.ScfEigenStage(H64, NB, NE, BETA)
-    dmx = core.ScfEigenStage(H32, NB, NE, BETA, fp64_switch_residual=1e-4)
+    dmx = core.ScfEigenStage(H32, NB, NE, BETA, fp64_switch_residual=1e-4, gram=gram)
     p64, pmx = [X0.copy(order="F")], [X0.copy(order="F")]
     for it, res in enumerate((1.0, 1e-3, 1e-5, 1e-6)):
         e64, _, ef64, pr64 = d64.step(it, res, s.veff, p64)
