@@ -349,8 +349,9 @@ lynx_status lynx_get_grid_dims(const lynx_ctx *ctx, int dims[3]);
  * not 16-byte aligned); (0, 0, 0, 0) returns to tuning.  Rows of more than
  * 116 points (up to 244) run on 512-thread shapes only, without vector runs
  * (a 256-point LDS row pitch), and not for complex128 blocks on
- * non-orthogonal cells (LYNX_E_UNSUPPORTED); batches keep the 116-point
- * limit.  lynx_get_stencil_config
+ * non-orthogonal cells (LYNX_E_UNSUPPORTED); batched units with such rows
+ * run in 512-thread groups of the same pitch, under the same exclusion.
+ * lynx_get_stencil_config
  * reports the shape in use and, in tune_ms (capacity *n_tune on input, count
  * on output), the timed candidates in order vector_runs 0..3 x row alignment
  * 0..1 (none for 0) x threads

@@ -2014,7 +2014,7 @@ struct BatchPlan {
         int nitems = 0;
         int max_tyr = 0;
         bool small_halo = true;   // fewer halo slots per thread suffice for every unit of the group
-        int nt = 256;             // threads per workgroup (192 for short-row non-orthogonal groups)
+        int nt = 256;             // threads per workgroup (192 for short-row non-orthogonal groups, 512 for long rows)
     };
     struct NGroup {
         int npm = 8;
@@ -2070,13 +2070,21 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
     };
     for (int u = 0; u < n; ++u) {
         const StencilGeom &g = units[u].g;
-        if (g.nx + 2 * kMaxR > 128) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
+        if (g.nx + 2 * kMaxR > 256) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
         kind[u] = g.orth ? 0 : 1;
-        int W = g.nx + 2 * kMaxR <= 64 ? 64 : 128;
+        int W = tile_pitch(g.nx);
         bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
-        // short non-orthogonal rows: 192 threads (two halo slots each) when
-        // they leave fewer lanes idle than 256, e.g. 8 rows of 24 points
-        if (!g.orth && 2 * kMaxR * g.nx <= 192 * 2 && nonorth_fill(g, 192, W) > nonorth_fill(g, kBatchNT, W)) {
+        if (W == 256) {
+            // long rows: 512 threads, a 256-point pitch and kLongHalo halo
+            // slots per thread, scalar bodies (no vector runs)
+            if (!g.orth && sizeof(S) > 8)
+                fail(8, "complex128 non-orthogonal grids with rows over 116 points are not supported");
+            ntu[u] = 512;
+            sh = false;
+        } else if (!g.orth && 2 * kMaxR * g.nx <= 192 * 2 && nonorth_fill(g, 192, W) > nonorth_fill(g, kBatchNT, W)) {
+            // short non-orthogonal rows: 192 threads (two halo slots each)
+            // when they leave fewer lanes idle than 256, e.g. 8 rows of 24
+            // points
             ntu[u] = 192;
             sh = true;
         }
@@ -2145,10 +2153,13 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         } else {
             const int nt = ntu[u];
             const long pts = (long)nt * (g.orth ? batch_rp_orth<S>() : batch_rp_nonorth<S>());
-            if (g.nx > pts || 2 * kMaxR * g.nx > nt * (nt == kBatchNT ? 6 : 2))
+            const int hslots = nt == 512 ? kLongHalo : (nt == kBatchNT ? 6 : 2);
+            if (g.nx > pts || 2 * kMaxR * g.nx > nt * hslots)
                 fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
             p.lpr = 0;
-            const int max_rows = g.orth ? g.ny : nonorth_max_rows<S>(P->sg[sgi[u]].W, nt);
+            const int pitch = P->sg[sgi[u]].W;
+            const int max_rows = g.orth ? step_max_rows<S>(pitch, nt) : nonorth_max_rows<S>(pitch, nt);
+            if (max_rows < 1) fail(8, "grid rows longer than the batched stencil kernel's tile are not supported");
             int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
             nb = std::max(1, std::min(nb, g.ny));
             p.tyr = (g.ny + nb - 1) / nb;
@@ -2239,7 +2250,17 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
         }
         return;
     }
-    if (G.kind == 1 && G.nt == 192) {
+    if constexpr (W == 256) {
+        // long rows: 512 threads, scalar bodies
+        const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 512);
+        if (G.kind == 0)
+            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), 512, kLongHalo, W, 0>), dim3(blocks),
+                               dim3(512), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+        else if constexpr (sizeof(S) <= 8)
+            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 512, kLongHalo, W, 1>), dim3(blocks),
+                               dim3(512), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+        return;
+    } else if (G.kind == 1 && G.nt == 192) {
         // short non-orthogonal rows, two halo slots per thread
         if constexpr (W == 64) {
             const size_t shm = 4 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 192);
@@ -2344,6 +2365,8 @@ void batch_step(BatchPlan *P, int step, int ix, int iold, int iout, hipStream_t 
             if (G.W == 64) {
                 if (G.small_halo) launch_stencil_batch<S, 64, 3>(P, G, cf, ix, iold, iout, st);
                 else launch_stencil_batch<S, 64, 6>(P, G, cf, ix, iold, iout, st);
+            } else if (G.W == 256) {
+                launch_stencil_batch<S, 256, kLongHalo>(P, G, cf, ix, iold, iout, st);
             } else {
                 if (G.small_halo) launch_stencil_batch<S, 128, 3>(P, G, cf, ix, iold, iout, st);
                 else launch_stencil_batch<S, 128, 6>(P, G, cf, ix, iold, iout, st);

@@ -10,7 +10,11 @@ against the oracle.
   vector, 120 x 24 x 24): real FP64, FP32 and complex64 against the oracle;
   complex128 planes of that pitch do not fit the LDS, and the library
   refuses them with LYNX_E_UNSUPPORTED;
-- 256-thread shapes forced on long rows are refused the same way.
+- 256-thread shapes forced on long rows are refused the same way;
+- batches (lynx_batch_*) holding long-row units (512-thread groups with a
+  256-point pitch) beside short-row ones: every unit's filter against the
+  oracle's filter of that unit alone, FP64 / complex128 to 1e-11 and FP32 /
+  complex64 to 2e-5; complex128 non-orthogonal long rows refused.
 """
 import numpy as np
 import pytest
@@ -105,3 +109,74 @@ def test_long_rows_nonorthogonal():
                                   L.MEM_HOST)
         assert st == L.E_UNSUPPORTED, shape
     L.check(L.lib().lynx_set_stencil_config(H.ctx, 0, 0, 0, 0), H.ctx)
+
+
+def _bounds(i):
+    return 0.3 + 0.05 * i, -0.6 - 0.02 * i, 60.0 + 3.0 * i
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 2e-5)])
+def test_long_rows_batch_real(dtype, tol):
+    # 120- and 200-point orthogonal rows, a 120-point non-orthogonal cell and
+    # a short-row unit in one batch
+    from lynx_amd import core
+    lv = A_SI * FCC_PRIMITIVE.copy()
+    lv[0] *= 5.0
+    systems = [System((120, 24, 20), A_SI, DIAMOND), System((200, 16, 16), A_SI, DIAMOND),
+               System((120, 24, 24), A_SI, SI2, latvec=lv), System((20, 20, 20), A_SI, DIAMOND)]
+    for s in systems[:2]:
+        s.latvec = np.diag([A_SI * s.n[0] / 24, A_SI * s.n[1] / 24, A_SI * s.n[2] / 24])
+    Hs = []
+    for s in systems:
+        H = s.hamiltonian()
+        H.set_veff(s.veff)
+        Hs.append(H)
+    bf = core.BatchFilter([(H, None) for H in Hs])
+    blocks = [s.random_block(2 + i % 2, 90 + i) for i, s in enumerate(systems)]
+    deg = 7
+    lc, lo, hi = zip(*[_bounds(i) for i in range(len(systems))])
+    got = bf.filter(blocks, deg, lc, lo, hi, dtype=dtype)
+    for i, s in enumerate(systems):
+        ref = s.oracle_op().cheb_filter(blocks[i], s.veff, lc[i], lo[i], hi[i], deg)
+        assert rel(got[i], ref) < tol, (i, s.n)
+    bf.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_long_rows_batch_kpoints(dtype, tol):
+    # Bloch-phased k-points of an orthogonal cell with 120-point rows and,
+    # in complex64, of the stretched non-orthogonal cell
+    from lynx_amd import core, _lib as L
+    n = (120, 24, 20)
+    latvec = np.diag([A_SI * n[0] / 24, A_SI * n[1] / 24, A_SI * n[2] / 24])
+    s = System(n, A_SI, DIAMOND)
+    s.latvec = latvec
+    H = s.hamiltonian()
+    H.set_veff(s.veff)
+    units = [(s, H, kvec(latvec, r)) for r in ((0.0, 0.0, 0.0), (0.25, -0.125, 0.375))]
+    lv = A_SI * FCC_PRIMITIVE.copy()
+    lv[0] *= 5.0
+    sn = System((120, 24, 24), A_SI, SI2, latvec=lv)
+    Hn = sn.hamiltonian()
+    Hn.set_veff(sn.veff)
+    kn = kvec(lv, (0.25, 0.125, -0.25))
+    if dtype == np.complex64:
+        units.append((sn, Hn, kn))
+    else:
+        with pytest.raises(L.LynxError):
+            bad = core.BatchFilter([(Hn, kn)])
+            try:
+                bad.filter([sn.random_block(2, 99, complex_=True)], 5, [0.3], [-0.6], [60.0], dtype=dtype)
+            finally:
+                bad.close()
+    bf = core.BatchFilter([(h, k) for _, h, k in units])
+    blocks = [u[0].random_block(3, 100 + i, complex_=True) for i, u in enumerate(units)]
+    deg = 7
+    lc, lo, hi = zip(*[_bounds(i) for i in range(len(units))])
+    got = bf.filter(blocks, deg, lc, lo, hi, dtype=dtype)
+    for i, (sy, _, k) in enumerate(units):
+        op = sy.oracle_op()
+        op.set_kpoint(k)
+        ref = op.cheb_filter(blocks[i], sy.veff, lc[i], lo[i], hi[i], deg)
+        assert rel(got[i], ref) < tol, (i, k)
+    bf.close()
