@@ -1245,64 +1245,37 @@ __global__ void __launch_bounds__(256) k_nl_gather(NlArgs<typename Tr<S>::A> p, 
     nl_gather_body<S, NPM, SPLIT, T>(p, X, ldx, beta, ncol, blockIdx.x, blockIdx.y);
 }
 
-// atoms with more sphere points than this are gathered by four waves
+// atoms with more sphere points than this are gathered by four waves (one
+// operator) or by parts (batches, k_nl_gather_part_batch)
 constexpr int kGatherSplitPts = 1024;
 
 // -------------------------------------------------------- nonlocal scatter --
 // Owner-computes: one lane per covered grid point, one wave per column; sums
 // chi . beta over every (image, point) entry at that grid point, so no two
 // lanes write the same address and the result is deterministic.
-template <class S, int NPM>
-__device__ __forceinline__ void nl_scatter_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ beta,
-                                                S *__restrict__ Y, long ldy, int ncol, typename Tr<S>::A a,
-                                                int cblock, int cgroup)
+// Many columns per wave: each entry's metadata and projector row are loaded
+// once and reused for the wave's cpw columns (at most CPWMAX), so per column
+// only the atom's beta row (shared by the wave's neighbouring points) and y
+// itself move.  Per column the entries are summed in list order, then
+// y += a * sum.
+template <class S, int NPM, int CPWMAX, class T>
+__device__ __forceinline__ void nl_scatter_mc_body(const NlArgs<typename Tr<S>::A> &p, const S *__restrict__ beta,
+                                                   T *__restrict__ Y, long ldy, int ncol, typename Tr<S>::A a,
+                                                   int cblock, int cgroup, int cpw)
 {
     using A = typename Tr<S>::A;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int u = cblock * 64 + lane;
-    const int n = cgroup * 4 + wave;
-    if (u >= p.n_cov || n >= ncol) return;
-    const S *bn = beta + (long)n * p.n_atom * NPM;
-    S acc = zero<S>();
-    for (int e = p.cov_off[u]; e < p.cov_off[u + 1]; ++e) {
-        const int pt = p.ent_pt[e];
-        const A *ch = p.chi + (long)pt * NPM;
-        const S *bt = bn + (long)p.pt_atom[pt] * NPM;
-        S s = zero<S>();
-#pragma unroll
-        for (int j = 0; j < NPM; ++j) s = fm(ch[j], bt[j], s);
-        if constexpr (Tr<S>::C) {
-            const int im = p.pt_img[pt];
-            s = cmul(s, p.phase[2 * im], -p.phase[2 * im + 1]);
-        }
-        acc = add(acc, s);
-    }
-    const long gi = (long)n * ldy + p.cov_gpos[u];
-    Y[gi] = fm(a, acc, Y[gi]);
-}
-
-// Many columns per wave: the point's entry metadata and projector row are
-// loaded once and reused for CPW columns, so per column only the atom's
-// beta row (shared by the wave's neighbouring points) and y itself move.
-template <class S, int NPM, int CPW, class T>
-__global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A> p, const S *__restrict__ beta,
-                                                       T *__restrict__ Y, long ldy, int ncol,
-                                                       typename Tr<S>::A a, const typename Tr<S>::A *acoef)
-{
-    using A = typename Tr<S>::A;
-    if (acoef) a = acoef[0];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int u = blockIdx.x * 64 + lane;
-    const int c0 = (blockIdx.y * 4 + wave) * CPW;
+    const int c0 = (cgroup * 4 + wave) * cpw;
     if (u >= p.n_cov || c0 >= ncol) return;
-    const int c1 = min(c0 + CPW, ncol);
+    const int nc = min(cpw, ncol - c0);
     const int e0 = p.cov_off[u], e1 = p.cov_off[u + 1];
     const long g = p.cov_gpos[u];
     const long nb = (long)p.n_atom * NPM;
     if (e1 - e0 == 1) {
-        // one entry (spheres do not overlap here): keep its row in registers
+        // one entry (spheres do not overlap here): each column's y is
+        // updated as soon as its product is formed
         const int pt = p.ent_pt[e0];
         A ch[NPM];
 #pragma unroll
@@ -1314,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A>
             pr = p.phase[2 * im];
             pi = -p.phase[2 * im + 1];
         }
-        for (int c = c0; c < c1; ++c) {
+        for (int c = c0; c < c0 + nc; ++c) {
             const S *bt = beta + c * nb + boff;
             S s = zero<S>();
 #pragma unroll
@@ -1325,33 +1298,49 @@ __global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A>
         }
         return;
     }
-    for (int c = c0; c < c1; ++c) {
-        const S *bn = beta + c * nb;
-        S acc = zero<S>();
-        for (int e = e0; e < e1; ++e) {
-            const int pt = p.ent_pt[e];
-            const A *chp = p.chi + (long)pt * NPM;
-            const S *bt = bn + (long)p.pt_atom[pt] * NPM;
-            S s = zero<S>();
+    S acc[CPWMAX];
 #pragma unroll
-            for (int j = 0; j < NPM; ++j) s = fm(chp[j], bt[j], s);
-            if constexpr (Tr<S>::C) {
-                const int im = p.pt_img[pt];
-                s = cmul(s, p.phase[2 * im], -p.phase[2 * im + 1]);
-            }
-            acc = add(acc, s);
+    for (int k = 0; k < CPWMAX; ++k) acc[k] = zero<S>();
+    for (int e = e0; e < e1; ++e) {
+        const int pt = p.ent_pt[e];
+        A ch[NPM];
+#pragma unroll
+        for (int j = 0; j < NPM; ++j) ch[j] = p.chi[(long)pt * NPM + j];
+        const long boff = (long)p.pt_atom[pt] * NPM;
+        A pr = A(1), pi = A(0);
+        if constexpr (Tr<S>::C) {
+            const int im = p.pt_img[pt];
+            pr = p.phase[2 * im];
+            pi = -p.phase[2 * im + 1];
         }
-        const long gi = (long)c * ldy + g;
-        Y[gi] = StoreAs<T>::from(fm(a, acc, to_compute(Y[gi])));
+#pragma unroll
+        for (int k = 0; k < CPWMAX; ++k) {
+            if (k < nc) {
+                const S *bt = beta + (c0 + k) * nb + boff;
+                S s = zero<S>();
+#pragma unroll
+                for (int j = 0; j < NPM; ++j) s = fm(ch[j], bt[j], s);
+                if constexpr (Tr<S>::C) s = cmul(s, pr, pi);
+                acc[k] = add(acc[k], s);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < CPWMAX; ++k) {
+        if (k < nc) {
+            const long gi = (long)(c0 + k) * ldy + g;
+            Y[gi] = StoreAs<T>::from(fm(a, acc[k], to_compute(Y[gi])));
+        }
     }
 }
 
-template <class S, int NPM>
-__global__ void __launch_bounds__(256) k_nl_scatter(NlArgs<typename Tr<S>::A> p,
-                                                    const S *__restrict__ beta, S *__restrict__ Y,
-                                                    long ldy, int ncol, typename Tr<S>::A a)
+template <class S, int NPM, int CPW, class T>
+__global__ void __launch_bounds__(256) k_nl_scatter_mc(NlArgs<typename Tr<S>::A> p, const S *__restrict__ beta,
+                                                       T *__restrict__ Y, long ldy, int ncol,
+                                                       typename Tr<S>::A a, const typename Tr<S>::A *acoef)
 {
-    nl_scatter_body<S, NPM>(p, beta, Y, ldy, ncol, a, blockIdx.x, blockIdx.y);
+    if (acoef) a = acoef[0];
+    nl_scatter_mc_body<S, NPM, CPW, T>(p, beta, Y, ldy, ncol, a, blockIdx.x, blockIdx.y, CPW);
 }
 
 // ------------------------------------------------------- batched launches --
@@ -1369,8 +1358,17 @@ struct BatchRec {
     int item0;                  // first stencil work item
     int g_item0, s_item0;       // first gather / scatter block
     int ncol;
+    int s_cpw;                  // scatter columns per wave
     long beta_off;              // first beta element of the unit
     NlArgs<A> nl;
+    // gather by parts (units with large spheres): sphere point ranges
+    // [pb, pe) of at most kGatherPartPts points of one atom, the first part
+    // of each atom (n_atom + 1), and the unit's first partial sum
+    const int2 *parts;
+    const int *atom_part;
+    int nparts;
+    int r_item0;                // first reduction block
+    long part_off;
 };
 
 // KIND: 0 orthogonal (scalar body), 1 non-orthogonal, 2 orthogonal with
@@ -1417,6 +1415,103 @@ __global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename
                                   r.ncol, lb % nl.n_atom, lb / nl.n_atom);
 }
 
+// Gather by parts, for units whose atoms have large spheres: one workgroup
+// per (part of at most kGatherPartPts sphere points of one atom, group of
+// kGatherPartCols columns).  The part's x values (Bloch phase applied) are
+// staged in LDS with point-contiguous loads; then the lanes are laid out
+// projector-major (lane j of each NPM-lane subgroup holds projector j) and
+// the subgroups take interleaved points, so one coalesced load fetches a
+// point's whole projector row, used for every column of the group.  The
+// subgroup sums are added in a fixed order through LDS and stored as the
+// part's partial sums; k_nl_gather_reduce_batch adds the parts of each atom.
+constexpr int kGatherPartPts = 256;
+constexpr int kGatherPartCols = 8;
+
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_gather_part_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
+                                                              const int *__restrict__ item_unit, int nitems,
+                                                              S *__restrict__ partial, int ix)
+{
+    using A = typename Tr<S>::A;
+    constexpr int CG = kGatherPartCols, PP = kGatherPartPts, NSUB = 256 / NPM;
+    static_assert(CG * PP >= NSUB * CG * NPM, "the staging buffer holds the subgroup sums");
+    __shared__ S xs[CG * PP];
+    const int w = blockIdx.x;
+    if (w >= nitems) return;
+    const int u = item_unit[w];
+    const BatchRec<A> &r = recs[u];
+    const NlArgs<A> nl = r.nl;
+    const int lb = w - r.g_item0;
+    const int q = lb % r.nparts;
+    const int c0 = (lb / r.nparts) * CG;
+    const int ncol = r.ncol;
+    const int2 pr = r.parts[q];
+    const int np = pr.y - pr.x;
+    const S *__restrict__ X = reinterpret_cast<const S *>(r.buf[ix]);
+    const long ldx = r.ld[ix];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < CG * PP; i += 256) {
+        const int k = i / PP, e = i % PP;
+        S v = zero<S>();
+        if (e < np && c0 + k < ncol) {
+            v = X[(long)(c0 + k) * ldx + nl.pt_gpos[pr.x + e]];
+            if constexpr (Tr<S>::C) {
+                const int im = nl.pt_img[pr.x + e];
+                v = cmul(v, nl.phase[2 * im], nl.phase[2 * im + 1]);
+            }
+        }
+        xs[i] = v;
+    }
+    __syncthreads();
+    const int j = tid % NPM, sub = tid / NPM;
+    S acc[CG];
+#pragma unroll
+    for (int k = 0; k < CG; ++k) acc[k] = zero<S>();
+    for (int e = sub; e < np; e += NSUB) {
+        const A ch = nl.chi[(long)(pr.x + e) * NPM + j];
+#pragma unroll
+        for (int k = 0; k < CG; ++k) acc[k] = fm(ch, xs[k * PP + e], acc[k]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CG; ++k) xs[(sub * CG + k) * NPM + j] = acc[k];
+    __syncthreads();
+    for (int i = tid; i < CG * NPM; i += 256) {
+        const int k = i / NPM, jj = i % NPM;
+        if (c0 + k >= ncol) continue;
+        S t = zero<S>();
+        for (int sg = 0; sg < NSUB; ++sg) t = add(t, xs[(sg * CG + k) * NPM + jj]);
+        partial[r.part_off + ((long)q * ncol + c0 + k) * NPM + jj] = t;
+    }
+}
+
+// beta[n][atom][j] = gamma[atom][j] * (sum of the atom's part sums, in part
+// order); one thread per (column, atom, projector) of a unit.
+template <class S, int NPM>
+__global__ void __launch_bounds__(256) k_nl_gather_reduce_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
+                                                                const int *__restrict__ item_unit, int nitems,
+                                                                const S *__restrict__ partial, S *__restrict__ beta)
+{
+    using A = typename Tr<S>::A;
+    const int w = blockIdx.x;
+    if (w >= nitems) return;
+    const int u = item_unit[w];
+    const BatchRec<A> &r = recs[u];
+    const NlArgs<A> nl = r.nl;
+    const long i = (long)(w - r.r_item0) * 256 + threadIdx.x;
+    if (i >= (long)r.ncol * nl.n_atom * NPM) return;
+    const int jj = (int)(i % NPM);
+    const int at = (int)((i / NPM) % nl.n_atom);
+    const int c = (int)(i / ((long)NPM * nl.n_atom));
+    S t = zero<S>();
+    for (int q = r.atom_part[at]; q < r.atom_part[at + 1]; ++q)
+        t = add(t, partial[r.part_off + ((long)q * r.ncol + c) * NPM + jj]);
+    beta[r.beta_off + i] = scl(nl.gamma[at * NPM + jj], t);
+}
+
+// batched scatter: columns per wave at most (one workgroup covers 4 cpw)
+constexpr int kBatchScatterCpw = 4;
+
 template <class S, int NPM>
 __global__ void __launch_bounds__(256) k_nl_scatter_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
                                                           const int *__restrict__ item_unit, int nitems,
@@ -1431,8 +1526,9 @@ __global__ void __launch_bounds__(256) k_nl_scatter_batch(const BatchRec<typenam
     const NlArgs<A> nl = r.nl;
     const int lb = w - r.s_item0;
     const int ncb = (nl.n_cov + 63) / 64;
-    nl_scatter_body<S, NPM>(nl, beta + r.beta_off, reinterpret_cast<S *>(const_cast<void *>(r.buf[iout])),
-                            r.ld[iout], r.ncol, coef[3 * u], lb % ncb, lb / ncb);
+    nl_scatter_mc_body<S, NPM, kBatchScatterCpw, S>(nl, beta + r.beta_off,
+                                                    reinterpret_cast<S *>(const_cast<void *>(r.buf[iout])),
+                                                    r.ld[iout], r.ncol, coef[3 * u], lb % ncb, lb / ncb, r.s_cpw);
 }
 
 // ------------------------------------------------------ Gram reduction --
@@ -2018,13 +2114,16 @@ struct BatchPlan {
     };
     struct NGroup {
         int npm = 8;
-        bool split = false;     // gather with four waves per (atom, column)
-        int *gitems = nullptr, *sitems = nullptr;
-        int ngitems = 0, nsitems = 0;
+        bool split = false;     // gather by parts (k_nl_gather_part_batch), then a reduction
+        int *gitems = nullptr, *sitems = nullptr, *ritems = nullptr;
+        int ngitems = 0, nsitems = 0, nritems = 0;
     };
     std::vector<SGroup> sg;
     std::vector<NGroup> ng;
     void *beta = nullptr;
+    void *partial = nullptr;    // part sums of the gather by parts
+    int2 *parts = nullptr;      // part point ranges, all units
+    int *atom_part = nullptr;   // first part of each atom, all units
     void *coef = nullptr;
     int coef_steps = 0;
 };
@@ -2134,8 +2233,13 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
             ngi[u] = j;
         }
     }
-    std::vector<std::vector<int>> sitems(P->sg.size()), gitems(P->ng.size()), citems(P->ng.size());
+    std::vector<std::vector<int>> sitems(P->sg.size()), gitems(P->ng.size()), citems(P->ng.size()),
+        ritems(P->ng.size());
     long beta_off = 0;
+    std::vector<int2> parts_h;
+    std::vector<int> atom_part_h;
+    std::vector<long> part_first(n, -1), range_first(n, -1);
+    long part_elems = 0;
     for (int u = 0; u < n; ++u) {
         const BatchUnit &U = units[u];
         const StencilGeom &g = U.g;
@@ -2199,18 +2303,62 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         r.nl = NlArgs<A>{nl.atom_pt_off, nl.pt_gpos, nl.pt_img, nl.pt_atom, nl.cov_off, nl.cov_gpos, nl.ent_pt,
                          (const A *)nl.chi, (const A *)nl.gamma, (const A *)nl.phase, nl.n_atom, nl.n_cov, nl.npm};
         r.beta_off = beta_off;
-        r.g_item0 = r.s_item0 = 0;
+        r.g_item0 = r.s_item0 = r.r_item0 = 0;
+        r.s_cpw = 1;
+        r.parts = nullptr;
+        r.atom_part = nullptr;
+        r.nparts = 0;
+        r.part_off = 0;
         if (ngi[u] >= 0) {
-            const int cg = (U.ncol + 3) / 4;
             auto &gi = gitems[ngi[u]];
             r.g_item0 = (int)gi.size();
-            const int gcols = P->ng[ngi[u]].split ? U.ncol : cg;   // (atom, column or column group)
-            for (int w = 0; w < nl.n_atom * gcols; ++w) gi.push_back(u);
+            if (P->ng[ngi[u]].split) {
+                // parts of at most kGatherPartPts points of one atom
+                std::vector<int> off(nl.n_atom + 1);
+                hip_check(hipMemcpy(off.data(), nl.atom_pt_off, off.size() * sizeof(int), hipMemcpyDeviceToHost),
+                          "copy");
+                part_first[u] = (long)atom_part_h.size();
+                range_first[u] = (long)parts_h.size();
+                for (int at = 0; at < nl.n_atom; ++at) {
+                    atom_part_h.push_back(r.nparts);
+                    for (int pb = off[at]; pb < off[at + 1]; pb += kGatherPartPts) {
+                        parts_h.push_back(make_int2(pb, std::min(pb + kGatherPartPts, off[at + 1])));
+                        ++r.nparts;
+                    }
+                }
+                atom_part_h.push_back(r.nparts);
+                r.part_off = part_elems;
+                part_elems += (long)r.nparts * U.ncol * nl.npm;
+                const int groups = (U.ncol + kGatherPartCols - 1) / kGatherPartCols;
+                for (int w = 0; w < r.nparts * groups; ++w) gi.push_back(u);
+                auto &ri = ritems[ngi[u]];
+                r.r_item0 = (int)ri.size();
+                const long nred = (long)U.ncol * nl.n_atom * nl.npm;
+                for (long w = 0; w < (nred + 255) / 256; ++w) ri.push_back(u);
+            } else {
+                // (atom, group of four columns)
+                const int cg = (U.ncol + 3) / 4;
+                for (int w = 0; w < nl.n_atom * cg; ++w) gi.push_back(u);
+            }
+            // scatter: column groups of 4 cpw columns, cpw chosen so the
+            // groups divide the columns evenly (24 columns: two groups of 12)
+            const int sgroups = (U.ncol + 4 * kBatchScatterCpw - 1) / (4 * kBatchScatterCpw);
+            r.s_cpw = (U.ncol + 4 * sgroups - 1) / (4 * sgroups);
             auto &ci = citems[ngi[u]];
             r.s_item0 = (int)ci.size();
-            for (int w = 0; w < ((nl.n_cov + 63) / 64) * cg; ++w) ci.push_back(u);
+            for (int w = 0; w < ((nl.n_cov + 63) / 64) * sgroups; ++w) ci.push_back(u);
             beta_off += (long)U.ncol * nl.n_atom * nl.npm;
         }
+    }
+    if (!parts_h.empty()) {
+        P->parts = dev_upload(parts_h, st);
+        P->atom_part = dev_upload(atom_part_h, st);
+        hip_check(hipMalloc(&P->partial, std::max<size_t>(1, part_elems * sizeof(S))), "hipMalloc");
+        for (int u = 0; u < n; ++u)
+            if (part_first[u] >= 0) {
+                recs[u].parts = P->parts + range_first[u];
+                recs[u].atom_part = P->atom_part + part_first[u];
+            }
     }
     P->recs = dev_upload(recs, st);
     for (size_t i = 0; i < P->sg.size(); ++i) {
@@ -2222,6 +2370,8 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         P->ng[i].gitems = dev_upload(gitems[i], st);
         P->ng[i].nsitems = (int)citems[i].size();
         P->ng[i].sitems = dev_upload(citems[i], st);
+        P->ng[i].nritems = (int)ritems[i].size();
+        P->ng[i].ritems = dev_upload(ritems[i], st);
     }
     hip_check(hipMalloc(&P->beta, std::max<size_t>(1, beta_off * sizeof(S))), "hipMalloc");
     hip_check(hipStreamSynchronize(st), "batch upload");
@@ -2287,10 +2437,13 @@ void launch_nl_batch(const BatchPlan *P, const BatchPlan::NGroup &G, const void 
     const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
     if (gather) {
         if (G.ngitems > 0) {
-            if (G.split)
-                hipLaunchKernelGGL((k_nl_gather_batch<S, NPM, true>), dim3(G.ngitems), dim3(256), 0, st, recs,
-                                   G.gitems, G.ngitems, static_cast<S *>(P->beta), ix);
-            else
+            if (G.split) {
+                hipLaunchKernelGGL((k_nl_gather_part_batch<S, NPM>), dim3(G.ngitems), dim3(256), 0, st, recs,
+                                   G.gitems, G.ngitems, static_cast<S *>(P->partial), ix);
+                hipLaunchKernelGGL((k_nl_gather_reduce_batch<S, NPM>), dim3(G.nritems), dim3(256), 0, st, recs,
+                                   G.ritems, G.nritems, static_cast<const S *>(P->partial),
+                                   static_cast<S *>(P->beta));
+            } else
                 hipLaunchKernelGGL((k_nl_gather_batch<S, NPM, false>), dim3(G.ngitems), dim3(256), 0, st, recs,
                                    G.gitems, G.ngitems, static_cast<S *>(P->beta), ix);
         }
@@ -2323,8 +2476,12 @@ void batch_plan_destroy(BatchPlan *P)
     for (auto &g : P->ng) {
         (void)hipFree(g.gitems);
         (void)hipFree(g.sitems);
+        (void)hipFree(g.ritems);
     }
     (void)hipFree(P->beta);
+    (void)hipFree(P->partial);
+    (void)hipFree(P->parts);
+    (void)hipFree(P->atom_part);
     (void)hipFree(P->coef);
     delete P;
 }
@@ -2380,7 +2537,12 @@ void batch_step(BatchPlan *P, int step, int ix, int iold, int iout, hipStream_t 
     });
 }
 
-int batch_launches_per_step(const BatchPlan *P) { return (int)(P->sg.size() + 2 * P->ng.size()); }
+int batch_launches_per_step(const BatchPlan *P)
+{
+    int n = (int)(P->sg.size() + 2 * P->ng.size());
+    for (const auto &G : P->ng) n += G.split && G.ngitems > 0;   // the part sums' reduction
+    return n;
+}
 
 void gram_reduce(int e, const void *work, int nsplit, int n, double alpha, void *C, hipStream_t st)
 {

@@ -76,6 +76,33 @@ def test_heterogeneous_batch_matches_oracle(dtype, tol):
     bf.close()
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5), (np.float64, 1e-11),
+                                       (np.float32, 2e-5)])
+def test_batch_gather_by_parts(dtype, tol):
+    # atoms with more than 1024 sphere points (BaTiO3 at 20^3: the La
+    # stand-in has 1955 over its images, Zr 1221) are gathered by parts of
+    # 256 points and 8 columns, then the parts are summed per atom; 3 and 11
+    # columns (a partial column group), two k-points or Gamma twice
+    from lynx_amd import core
+    from systems import batio3
+    s = batio3(0, n=20)
+    H = s.hamiltonian()
+    H.set_veff(s.veff)
+    cplx = np.dtype(dtype).kind == "c"
+    ks = [kvec(s, (0.25, -0.125, 0.375)), kvec(s, (0.0, 0.5, 0.25))] if cplx else [None, None]
+    bf = core.BatchFilter([(H, k) for k in ks])
+    blocks = [s.random_block(3, 120, complex_=cplx), s.random_block(11, 121, complex_=cplx)]
+    deg = 6
+    lc, lo, hi = zip(*[_bounds(i) for i in range(2)])
+    got = bf.filter(blocks, deg, lc, lo, hi, dtype=dtype)
+    op = s.oracle_op()
+    for i, k in enumerate(ks):
+        op.set_kpoint(k if k is not None else (0.0, 0.0, 0.0))
+        ref = op.cheb_filter(blocks[i], s.veff, lc[i], lo[i], hi[i], deg)
+        assert rel(got[i], ref) < tol, (i, k)
+    bf.close()
+
+
 def test_graph_replays_match_eager_launches(fcc_kbatch):
     # the same plan run eagerly and from the captured graph, several calls
     # in a row with different inputs and bounds and other GPU work between
