@@ -2147,6 +2147,9 @@ T *dev_upload(const std::vector<T> &v, hipStream_t st)
 template <class S> constexpr int batch_rp_orth() { return base_rp<S>() / 2 > 0 ? base_rp<S>() / 2 : 1; }
 template <class S> constexpr int batch_rp_nonorth() { return base_rp<S>() / 4 > 0 ? base_rp<S>() / 4 : 1; }
 constexpr int kBatchNT = 256;
+// LDS row pitch of batched non-orthogonal units with rows of at most
+// 40 - 2R points
+constexpr int kShortNonorthPitch = 40;
 
 template <class S>
 void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
@@ -2172,6 +2175,7 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         if (g.nx + 2 * kMaxR > 256) fail(8, "grid rows longer than the fused stencil kernel's tile are not supported");
         kind[u] = g.orth ? 0 : 1;
         int W = tile_pitch(g.nx);
+        if (!g.orth && g.nx + 2 * kMaxR <= kShortNonorthPitch) W = kShortNonorthPitch;
         bool sh = 2 * kMaxR * g.nx <= kBatchNT * 3;
         if (W == 256) {
             // long rows: 512 threads, a 256-point pitch and kLongHalo halo
@@ -2385,48 +2389,42 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
     const int blocks = ((G.nitems + 7) / 8) * 8;
     const auto *recs = static_cast<const BatchRec<A> *>(P->recs);
     const A *cf = static_cast<const A *>(coef);
-    if (G.kind == 2 || G.kind == 3) {
+    auto go = [&](auto kern, int nt, size_t shm) {
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(nt), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+    };
+    if constexpr (W == 256) {
+        // long rows: 512 threads, scalar bodies
+        const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 512);
+        if (G.kind == 0) go(k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), 512, kLongHalo, W, 0>, 512, shm);
+        else if constexpr (sizeof(S) <= 8)
+            go(k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 512, kLongHalo, W, 1>, 512, shm);
+    } else if constexpr (W == kShortNonorthPitch) {
+        // short non-orthogonal rows: the tight pitch leaves room for more
+        // workgroups per CU (the body holds four planes of this pitch)
+        const size_t shm = 4 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + G.nt);
+        if (G.nt == 192) go(k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 192, 2, W, 1>, 192, shm);
+        else go(k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, 1>, kBatchNT, shm);
+    } else if (G.kind == 2 || G.kind == 3) {
         if constexpr (!Tr<S>::C) {
             // vector kinds: halo slots counted in runs (2 or 3 per thread)
             constexpr int HV = HP == 3 ? 2 : 3;
             const int VW = G.kind == 3 ? 2 : int(16 / sizeof(S));
             const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT * VW);
-            if (G.kind == 2)
-                hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>), dim3(blocks), dim3(kBatchNT),
-                                   shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
-            else if constexpr (sizeof(S) == 4)
-                hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 3>), dim3(blocks), dim3(kBatchNT),
-                                   shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+            if (G.kind == 2) go(k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>, kBatchNT, shm);
+            else if constexpr (sizeof(S) == 4) go(k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 3>, kBatchNT, shm);
         }
-        return;
-    }
-    if constexpr (W == 256) {
-        // long rows: 512 threads, scalar bodies
-        const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 512);
-        if (G.kind == 0)
-            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), 512, kLongHalo, W, 0>), dim3(blocks),
-                               dim3(512), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
-        else if constexpr (sizeof(S) <= 8)
-            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 512, kLongHalo, W, 1>), dim3(blocks),
-                               dim3(512), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
-        return;
     } else if (G.kind == 1 && G.nt == 192) {
         // short non-orthogonal rows, two halo slots per thread
         if constexpr (W == 64) {
             const size_t shm = 4 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + 192);
-            hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 192, 2, W, 1>), dim3(blocks),
-                               dim3(192), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
+            go(k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), 192, 2, W, 1>, 192, shm);
         }
-        return;
+    } else {
+        // the non-orthogonal body uses two (plane, g-plane) pairs
+        const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
+        if (G.kind == 0) go(k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, 0>, kBatchNT, shm);
+        else go(k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, 1>, kBatchNT, shm);
     }
-    // the non-orthogonal body uses two (plane, g-plane) pairs
-    const size_t shm = (G.kind == 1 ? 4 : 2) * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT);
-    if (G.kind == 0)
-        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_orth<S>(), kBatchNT, HP, W, 0>), dim3(blocks),
-                           dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
-    else
-        hipLaunchKernelGGL((k_stencil_batch<S, kMaxR, batch_rp_nonorth<S>(), kBatchNT, HP, W, 1>), dim3(blocks),
-                           dim3(kBatchNT), shm, st, recs, G.items, G.nitems, cf, ix, iold, iout);
 }
 
 template <class S, int NPM>
@@ -2519,7 +2517,10 @@ void batch_step(BatchPlan *P, int step, int ix, int iold, int iout, hipStream_t 
             else launch_nl_batch<S, 32>(P, G, cf, ix, iout, true, st);
         }
         for (const auto &G : P->sg) {
-            if (G.W == 64) {
+            if (G.W == kShortNonorthPitch) {
+                if (G.small_halo) launch_stencil_batch<S, kShortNonorthPitch, 3>(P, G, cf, ix, iold, iout, st);
+                else launch_stencil_batch<S, kShortNonorthPitch, 6>(P, G, cf, ix, iold, iout, st);
+            } else if (G.W == 64) {
                 if (G.small_halo) launch_stencil_batch<S, 64, 3>(P, G, cf, ix, iold, iout, st);
                 else launch_stencil_batch<S, 64, 6>(P, G, cf, ix, iold, iout, st);
             } else if (G.W == 256) {
