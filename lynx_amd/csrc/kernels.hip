@@ -1399,7 +1399,9 @@ __global__ void __launch_bounds__(NT) k_stencil_batch(const BatchRec<typename Tr
     }
 }
 
-template <class S, int NPM, bool SPLIT>
+// Batched gather of units whose atoms have small spheres: one workgroup per
+// (atom, group of four columns), one wave per column.
+template <class S, int NPM>
 __global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename Tr<S>::A> *__restrict__ recs,
                                                          const int *__restrict__ item_unit, int nitems,
                                                          S *__restrict__ beta, int ix)
@@ -1411,7 +1413,7 @@ __global__ void __launch_bounds__(256) k_nl_gather_batch(const BatchRec<typename
     const BatchRec<A> &r = recs[u];
     const NlArgs<A> nl = r.nl;
     const int lb = w - r.g_item0;
-    nl_gather_body<S, NPM, SPLIT>(nl, reinterpret_cast<const S *>(r.buf[ix]), r.ld[ix], beta + r.beta_off,
+    nl_gather_body<S, NPM, false>(nl, reinterpret_cast<const S *>(r.buf[ix]), r.ld[ix], beta + r.beta_off,
                                   r.ncol, lb % nl.n_atom, lb / nl.n_atom);
 }
 
@@ -1504,8 +1506,10 @@ __global__ void __launch_bounds__(256) k_nl_gather_reduce_batch(const BatchRec<t
     const int at = (int)((i / NPM) % nl.n_atom);
     const int c = (int)(i / ((long)NPM * nl.n_atom));
     S t = zero<S>();
-    for (int q = r.atom_part[at]; q < r.atom_part[at + 1]; ++q)
-        t = add(t, partial[r.part_off + ((long)q * r.ncol + c) * NPM + jj]);
+    const int q0 = r.atom_part[at], q1 = r.atom_part[at + 1];
+    // loads issued four parts ahead; the sum stays in part order
+#pragma unroll 4
+    for (int q = q0; q < q1; ++q) t = add(t, partial[r.part_off + ((long)q * r.ncol + c) * NPM + jj]);
     beta[r.beta_off + i] = scl(nl.gamma[at * NPM + jj], t);
 }
 
@@ -2442,7 +2446,7 @@ void launch_nl_batch(const BatchPlan *P, const BatchPlan::NGroup &G, const void 
                                    G.ritems, G.nritems, static_cast<const S *>(P->partial),
                                    static_cast<S *>(P->beta));
             } else
-                hipLaunchKernelGGL((k_nl_gather_batch<S, NPM, false>), dim3(G.ngitems), dim3(256), 0, st, recs,
+                hipLaunchKernelGGL((k_nl_gather_batch<S, NPM>), dim3(G.ngitems), dim3(256), 0, st, recs,
                                    G.gitems, G.ngitems, static_cast<S *>(P->beta), ix);
         }
     } else if (G.nsitems > 0) {
