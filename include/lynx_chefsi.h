@@ -319,8 +319,9 @@ lynx_status lynx_rotate(lynx_ctx *ctx, int dtype, const void *a, int lda, const 
  * caller (the eigen stage, another wrapper) has set a potential since.  No
  * reference counterpart. */
 lynx_status lynx_get_veff_generation(const lynx_ctx *ctx, long *generation);
-/* Number of lynx_chefsi_solve calls completed on the context (each CheFSI
- * pass over one block).  No reference counterpart (a diagnostic). */
+/* Number of CheFSI passes over one block completed on the context
+ * (lynx_chefsi_solve calls, and each batch unit of lynx_batch_solve on it).
+ * No reference counterpart (a diagnostic). */
 lynx_status lynx_solve_count(const lynx_ctx *ctx, long *count);
 /* Counters of the FP32 passes of this context.  *count: passes redone in
  * FP64 from the unchanged psi, because the filtered block was too
@@ -401,7 +402,10 @@ const char *lynx_scf_eig_last_error(const lynx_scf_eig *driver);
 /* veff: nspin x Nd doubles (host); psi[s * nkpt + k]: blocks of nband
  * columns (ld, mem as lynx_chefsi_solve), updated in place; eig and occ:
  * nspin x nkpt x nband; *fermi the Fermi level; *precision_used the filter
- * precision of this step (LYNX_FILTER_*). */
+ * precision of this step (LYNX_FILTER_*).  FP64 passes over several
+ * complex k-points on device blocks solve each spin's k-points together
+ * through one lynx_batch_solve (created on first use, owned by the
+ * driver); other passes loop over the k-points with lynx_chefsi_solve. */
 lynx_status lynx_scf_eig_step(lynx_scf_eig *driver, int scf_iter, double scf_residual, const double *veff,
                               void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
                               int *precision_used);

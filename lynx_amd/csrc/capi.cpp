@@ -2412,5 +2412,6 @@ lynx_status lynx_batch_solve(lynx_batch *b, int dtype, void *const *psi, const l
                 }
                 if (lambda_c_out) lambda_c_out[u] = src[q.N - 1] + 0.1;
             }
+        for (int u = 0; u < n; ++u) ++b->units[u].ctx->solves;
     });
 }

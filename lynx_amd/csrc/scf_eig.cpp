@@ -26,6 +26,7 @@ struct lynx_scf_eig {
     double fermi = 0.0;
     int last_precision = LYNX_FILTER_FP64;
     std::string err;
+    lynx_batch *batch = nullptr;   // one spin's k-points: FP64 passes on device blocks
 };
 
 namespace {
@@ -155,6 +156,7 @@ lynx_status lynx_scf_eig_create(lynx_scf_eig **out, lynx_ctx *ctx, const lynx_sc
 
 lynx_status lynx_scf_eig_destroy(lynx_scf_eig *d)
 {
+    if (d && d->batch) (void)lynx_batch_destroy(d->batch);
     delete d;
     return LYNX_OK;
 }
@@ -211,9 +213,34 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
         ok(lynx_set_filter_precision(d->ctx, d->last_precision), d, "lynx_set_filter_precision");
         ok(lynx_set_gram_precision(d->ctx, p.gram_precision), d, "lynx_set_gram_precision");
         const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
+        // FP64 passes over several k-points of device blocks: the k-points of
+        // a spin solved together (lynx_batch_solve: the same pass per
+        // k-point, with batched launches, GEMMs and rocSOLVER calls)
+        const bool batched = !single && p.complex_kpt && p.nkpt > 1 && mem == LYNX_MEM_DEVICE;
+        if (batched && !d->batch) {
+            std::vector<lynx_ctx *> ctxs(p.nkpt, d->ctx);
+            const lynx_status bs = lynx_batch_create(&d->batch, p.nkpt, ctxs.data(), d->kpts.data());
+            if (bs != LYNX_OK) {
+                d->batch = nullptr;
+                throw Fail{bs, "lynx_batch_create failed"};
+            }
+        }
         for (int pass = 0; pass < passes; ++pass) {
             for (int s = 0; s < p.nspin; ++s) {
                 ok(lynx_set_veff(d->ctx, veff + s * d->nd, veff_mem), d, "lynx_set_veff");
+                if (batched) {
+                    const int nk = p.nkpt;
+                    std::vector<void *> ps(psi + (long)s * nk, psi + (long)(s + 1) * nk);
+                    std::vector<long> lds(nk, ld);
+                    std::vector<int> nc(nk, N);
+                    std::vector<double> lc(nk, d->lambda_c), lo(nk, d->eig_min[s]), hi(nk, d->eig_max[s]);
+                    const lynx_status bs = lynx_batch_solve(d->batch, dtype, ps.data(), lds.data(), nc.data(),
+                                                            eig + (long)s * nk * N, lc.data(), lo.data(), hi.data(),
+                                                            p.cheb_degree, nullptr);
+                    if (bs != LYNX_OK)
+                        throw Fail{bs, std::string("lynx_batch_solve: ") + lynx_batch_last_error(d->batch)};
+                    continue;
+                }
                 for (int k = 0; k < p.nkpt; ++k) {
                     if (p.complex_kpt) ok(lynx_set_kpoint(d->ctx, &d->kpts[3 * k]), d, "lynx_set_kpoint");
                     ok(lynx_chefsi_solve(d->ctx, dtype, psi[s * p.nkpt + k], ld, N, eig + (long)(s * p.nkpt + k) * N,
