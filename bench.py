@@ -495,6 +495,41 @@ def run_scf_si8(with_cpu):
     return out
 
 
+def run_scf_si2_kpoints():
+    """SCF wall-time with k-points: Si2 in the FCC primitive cell (a = 10.26
+    Bohr, C.pot for Si), 24^3 grid, the 6x6x6 Monkhorst-Pack grid reduced by
+    time reversal to 112 k-points, 12 states per k-point, GGA-PBE,
+    Fermi-Dirac 1000 K, degree 20, FP64, orbitals from the glibc rand()
+    stream, to a 1e-6 density residual.  Each FP64 pass solves the 112
+    k-points together (lynx_batch_solve).  One untimed iteration tunes the
+    launch shapes."""
+    from systems import si2_fcc, fcc_kpoints
+    from lynx_amd.scf import SCF
+    from lynx_amd.hipmem import synchronize
+    s = si2_fcc(24)
+    H = s.hamiltonian(0)
+    ks, w = fcc_kpoints(s)
+    beta = 1.0 / (KB_HA * 1000.0)
+    kw = dict(xc="GGA_PBE", tol=1e-6, kpts=ks, kpt_weights=w)
+    warm = SCF(H, 12, beta, max_iter=1, **kw)
+    warm.run()
+    warm.close()
+    S = SCF(H, 12, beta, **kw)
+    synchronize()
+    t0 = time.perf_counter()
+    r = S.run()
+    synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": "Si2 FCC primitive cell a=10.26 Bohr, 24^3, FD12, 6x6x6 MP -> %d k-points, 12 states, "
+                       "GGA-PBE, Fermi-Dirac 1000 K, degree 20, FP64, SCF to 1e-6 density residual; C.pot stands "
+                       "in for Si" % len(ks),
+           "scf_wall_s": round(wall, 4), "iterations": r["iterations"], "converged": bool(r["converged"]),
+           "etotal_per_atom_Ha": r["etotal"] / 2}
+    out.update(_scf_history_summary(S))
+    S.close()
+    return out
+
+
 def run_scf_si216():
     """SCF wall-time on Si216 (the 3x3x3 supercell, a = 30.78 Bohr,
     sigma = 0.03 Bohr seed 3, 90^3 grid, 480 states, GGA-PBE at Gamma,
@@ -699,6 +734,8 @@ def main():
     if not args.no_scf:
         variants["scf_si8"] = run_scf_si8(not args.no_cpu_baseline)
         print(f"[bench] Si8 SCF: {variants['scf_si8']}", file=sys.stderr, flush=True)
+        variants["scf_si2_kpoints"] = run_scf_si2_kpoints()
+        print(f"[bench] Si2 k-point SCF: {variants['scf_si2_kpoints']}", file=sys.stderr, flush=True)
     if not args.no_si216:
         variants["gram_si216"] = run_gram_si216()
         print(f"[bench] Si216 Gram: {variants['gram_si216']}", file=sys.stderr, flush=True)
