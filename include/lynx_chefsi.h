@@ -402,10 +402,13 @@ const char *lynx_scf_eig_last_error(const lynx_scf_eig *driver);
 /* veff: nspin x Nd doubles (host); psi[s * nkpt + k]: blocks of nband
  * columns (ld, mem as lynx_chefsi_solve), updated in place; eig and occ:
  * nspin x nkpt x nband; *fermi the Fermi level; *precision_used the filter
- * precision of this step (LYNX_FILTER_*).  FP64 passes over several
- * complex k-points on device blocks solve each spin's k-points together
- * through one lynx_batch_solve (created on first use, owned by the
- * driver); other passes loop over the k-points with lynx_chefsi_solve. */
+ * precision of this step (LYNX_FILTER_*).  Passes over several complex
+ * k-points on device blocks solve each spin's k-points together through
+ * one lynx_batch_solve (created on first use, owned by the driver): FP64
+ * passes as lynx_chefsi_solve runs them, reduced-precision passes with the
+ * filter in complex64 and the subspace steps in complex128 (gram_precision
+ * does not apply to them).  Host blocks loop over the k-points with
+ * lynx_chefsi_solve. */
 lynx_status lynx_scf_eig_step(lynx_scf_eig *driver, int scf_iter, double scf_residual, const double *veff,
                               void *const *psi, int ld, int mem, double *eig, double *occ, double *fermi,
                               int *precision_used);
@@ -555,6 +558,13 @@ lynx_status lynx_batch_filter(lynx_batch *batch, int dtype, void *const *x, cons
                               const double *eig_min, const double *eig_max, int mem);
 /* 1 (default): capture the filter into a HIP graph and replay it; 0: launch eagerly. */
 lynx_status lynx_batch_set_graph(lynx_batch *batch, int enable);
+/* Filter precision of lynx_batch_solve: LYNX_FILTER_FP64 (default) or
+ * LYNX_FILTER_FP32, the filter on single-precision copies of the blocks
+ * (float / complex64) widened back before the FP64 / complex128 subspace
+ * steps: the reduced-precision pass, batched (lynx_set_filter_precision
+ * is its single-context counterpart; no reference counterpart, the
+ * reference filters in FP64). */
+lynx_status lynx_batch_set_filter_precision(lynx_batch *batch, int precision);
 /* Device time of the last lynx_batch_filter (HIP events around the graph
  * launch or the eager launches) and the number of kernel launches it holds. */
 lynx_status lynx_batch_last_timing(const lynx_batch *batch, double *ms, int *launches);

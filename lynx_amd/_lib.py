@@ -94,6 +94,7 @@ SIGNATURES = {
     "lynx_batch_filter": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_L), ctypes.POINTER(_P),
                                ctypes.POINTER(_L), _IP, _I, _DP, _DP, _DP, _I]),
     "lynx_batch_set_graph": (_I, [_P, _I]),
+    "lynx_batch_set_filter_precision": (_I, [_P, _I]),
     "lynx_batch_last_timing": (_I, [_P, _DP, _IP]),
     "lynx_batch_solve": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_L), _IP, _DP, _DP, _DP, _DP, _I, _DP]),
     "lynx_set_stencil_config": (_I, [_P, _I, _I, _I, _I]),

@@ -1901,6 +1901,7 @@ struct lynx_batch {
         DevBuf ph64, ph32;     // image phases e^{-i k.R} of this unit's k-point
         DevBuf work, dx, dy;   // recurrence work block; staging for host buffers
         DevBuf sy;             // lynx_batch_solve: the filtered block, then H psi, then the rotated block
+        DevBuf sx32, sy32;     // lynx_batch_solve with the FP32 filter: single-precision copies
         long epoch = -1;       // the context's set-up epoch the phases were built for
     };
     std::vector<Unit> units;
@@ -1910,6 +1911,7 @@ struct lynx_batch {
     hipGraphExec_t graph = nullptr;
     int graph_degree = 0;
     bool use_graph = true;
+    int filter_precision = LYNX_FILTER_FP64;   // of lynx_batch_solve's filter
     // lynx_batch_solve: the H psi step's plan (its own coefficient row, so
     // the filter graph's table is never resized), a BLAS handle on the
     // batch's stream and per-group dense work
@@ -1929,6 +1931,7 @@ struct lynx_batch {
         if (blas) (void)rocblas_destroy_handle(blas);
         for (auto &u : units) {
             u.ph64.release(); u.ph32.release(); u.work.release(); u.dx.release(); u.dy.release(); u.sy.release();
+            u.sx32.release(); u.sy32.release();
         }
         for (DevBuf *x : {&dmat, &dhs, &dev, &dwork, &dinfo, &dptrs}) x->release();
         if (e0) (void)hipEventDestroy(e0);
@@ -2016,6 +2019,17 @@ lynx_status lynx_batch_destroy(lynx_batch *b)
 }
 
 const char *lynx_batch_last_error(const lynx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+lynx_status lynx_batch_set_filter_precision(lynx_batch *b, int precision)
+{
+    if (!b) return LYNX_E_ARG;
+    if (precision != LYNX_FILTER_FP64 && precision != LYNX_FILTER_FP32) {
+        b->err = "lynx_batch_set_filter_precision: LYNX_FILTER_FP64 or LYNX_FILTER_FP32";
+        return LYNX_E_ARG;
+    }
+    b->filter_precision = precision;
+    return LYNX_OK;
+}
 
 lynx_status lynx_batch_set_graph(lynx_batch *b, int enable)
 {
@@ -2242,8 +2256,25 @@ lynx_status lynx_batch_solve(lynx_batch *b, int dtype, void *const *psi, const l
             ys[u] = U.sy.get((size_t)lys[u] * ncol[u] * es);
             eoff[u + 1] = eoff[u] + ncol[u];
         }
-        // 1. Y = p(H) psi for every unit (EigenSolver.cpp:558-619)
-        {
+        // 1. Y = p(H) psi for every unit (EigenSolver.cpp:558-619); with the
+        // FP32 filter on single-precision copies, widened back into Y
+        if (b->filter_precision == LYNX_FILTER_FP32) {
+            const int e1 = cplx ? LYNX_C64 : LYNX_F32;
+            const size_t es1 = gpu::elem_size(e1);
+            std::vector<void *> xs(n), y1(n);
+            for (int u = 0; u < n; ++u) {
+                lynx_batch::Unit &U = b->units[u];
+                HIP_OK(hipStreamSynchronize(U.ctx->stream));   // psi as the context's stream left it
+                xs[u] = U.sx32.get((size_t)lys[u] * ncol[u] * es1);
+                y1[u] = U.sy32.get((size_t)lys[u] * ncol[u] * es1);
+                gpu::convert_cols(dtype, psi[u], lps[u], e1, xs[u], lys[u], U.ctx->nd, ncol[u], b->stream);
+            }
+            const lynx_status fs = lynx_batch_filter(b, e1, xs.data(), lys.data(), y1.data(), lys.data(), ncol, degree,
+                                                     lambda_c, eig_min, eig_max, LYNX_MEM_DEVICE);
+            if (fs != LYNX_OK) fail(fs, "batch filter: " + b->err);
+            for (int u = 0; u < n; ++u)
+                gpu::convert_cols(e1, y1[u], lys[u], dtype, ys[u], lys[u], b->units[u].ctx->nd, ncol[u], b->stream);
+        } else {
             const lynx_status fs = lynx_batch_filter(b, dtype, psi, lps.data(), ys.data(), lys.data(), ncol, degree,
                                                      lambda_c, eig_min, eig_max, LYNX_MEM_DEVICE);
             if (fs != LYNX_OK) fail(fs, "batch filter: " + b->err);

@@ -213,10 +213,12 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
         ok(lynx_set_filter_precision(d->ctx, d->last_precision), d, "lynx_set_filter_precision");
         ok(lynx_set_gram_precision(d->ctx, p.gram_precision), d, "lynx_set_gram_precision");
         const double spin_fac = p.nspin == 1 ? 2.0 : 1.0;
-        // FP64 passes over several k-points of device blocks: the k-points of
-        // a spin solved together (lynx_batch_solve: the same pass per
-        // k-point, with batched launches, GEMMs and rocSOLVER calls)
-        const bool batched = !single && p.complex_kpt && p.nkpt > 1 && mem == LYNX_MEM_DEVICE;
+        // several k-points of device blocks: the k-points of a spin solved
+        // together (lynx_batch_solve: the same pass per k-point, with
+        // batched launches, GEMMs and rocSOLVER calls).  Reduced-precision
+        // passes run its filter in complex64 and the subspace steps in
+        // complex128.
+        const bool batched = p.complex_kpt && p.nkpt > 1 && mem == LYNX_MEM_DEVICE;
         if (batched && !d->batch) {
             std::vector<lynx_ctx *> ctxs(p.nkpt, d->ctx);
             const lynx_status bs = lynx_batch_create(&d->batch, p.nkpt, ctxs.data(), d->kpts.data());
@@ -224,6 +226,11 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
                 d->batch = nullptr;
                 throw Fail{bs, "lynx_batch_create failed"};
             }
+        }
+        if (batched) {
+            const lynx_status bs =
+                lynx_batch_set_filter_precision(d->batch, single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64);
+            if (bs != LYNX_OK) throw Fail{bs, "lynx_batch_set_filter_precision failed"};
         }
         for (int pass = 0; pass < passes; ++pass) {
             for (int s = 0; s < p.nspin; ++s) {

@@ -116,6 +116,33 @@ def test_eigen_stage_kpoints_on_device_blocks(si8_16):
         d.free()
 
 
+def test_eigen_stage_kpoints_on_device_blocks_mixed(si8_16):
+    # reduced-precision passes over k-points on device blocks: the batched
+    # filter in complex64 and the subspace steps in complex128, within
+    # 1e-5 Ha of the FP64 stage (host blocks, per-k-point loop); FP64 after
+    # the switch
+    from systems import A_SI
+    from lynx_amd import core
+    from lynx_amd.hipmem import DeviceArray
+    s = si8_16
+    kp = 2 * np.pi / A_SI * np.array([[0.0, 0.0, 0.0], [0.25, -0.125, 0.375], [0.5, 0.25, 0.0]])
+    w = np.array([0.25, 0.5, 0.25])
+    H64, Hmx = s.hamiltonian(), s.hamiltonian()
+    X0 = s.random_block(NB, 90, complex_=True)
+    d64 = core.ScfEigenStage(H64, NB, NE, BETA, kpts=kp, kpt_weights=w)
+    dmx = core.ScfEigenStage(Hmx, NB, NE, BETA, kpts=kp, kpt_weights=w, fp64_switch_residual=1e-4)
+    p64 = [X0.copy(order="F") for _ in range(3)]
+    dev = [DeviceArray.from_numpy(np.ascontiguousarray(X0.T)) for _ in range(3)]
+    for it, res in enumerate((1.0, 1e-3, 1e-5)):
+        e64, _, ef64, _ = d64.step(it, res, s.veff, p64)
+        emx, _, efmx, prmx = dmx.step_device(it, res, s.veff, [d.value for d in dev], s.nd)
+        assert prmx == (1 if res > 1e-4 or it == 0 else 0)
+        assert np.abs(emx - e64).max() < 1e-5, it
+        assert abs(efmx - ef64) < 1e-5
+    for d in dev:
+        d.free()
+
+
 def test_eigen_stage_two_spins_matches_oracle(si8_16):
     # nspin = 2 with spin 1's Veff shifted by a constant: per-spin Lanczos
     # bounds, lambda_c from the larger last eigenvalue over spins, eig_min
