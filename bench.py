@@ -499,10 +499,11 @@ def run_scf_si2_kpoints():
     """SCF wall-time with k-points: Si2 in the FCC primitive cell (a = 10.26
     Bohr, C.pot for Si), 24^3 grid, the 6x6x6 Monkhorst-Pack grid reduced by
     time reversal to 112 k-points, 12 states per k-point, GGA-PBE,
-    Fermi-Dirac 1000 K, degree 20, FP64, orbitals from the glibc rand()
-    stream, to a 1e-6 density residual.  Each FP64 pass solves the 112
-    k-points together (lynx_batch_solve).  One untimed iteration tunes the
-    launch shapes."""
+    Fermi-Dirac 1000 K, degree 20, orbitals from the glibc rand() stream, to
+    a 1e-6 density residual; FP64 throughout, and with the filter in
+    complex64 until the residual falls below 1e-4.  Each pass solves the
+    112 k-points together (lynx_batch_solve).  One untimed iteration per
+    mode tunes the launch shapes."""
     from systems import si2_fcc, fcc_kpoints
     from lynx_amd.scf import SCF
     from lynx_amd.hipmem import synchronize
@@ -510,23 +511,28 @@ def run_scf_si2_kpoints():
     H = s.hamiltonian(0)
     ks, w = fcc_kpoints(s)
     beta = 1.0 / (KB_HA * 1000.0)
-    kw = dict(xc="GGA_PBE", tol=1e-6, kpts=ks, kpt_weights=w)
-    warm = SCF(H, 12, beta, max_iter=1, **kw)
-    warm.run()
-    warm.close()
-    S = SCF(H, 12, beta, **kw)
-    synchronize()
-    t0 = time.perf_counter()
-    r = S.run()
-    synchronize()
-    wall = time.perf_counter() - t0
     out = {"workload": "Si2 FCC primitive cell a=10.26 Bohr, 24^3, FD12, 6x6x6 MP -> %d k-points, 12 states, "
-                       "GGA-PBE, Fermi-Dirac 1000 K, degree 20, FP64, SCF to 1e-6 density residual; C.pot stands "
-                       "in for Si" % len(ks),
-           "scf_wall_s": round(wall, 4), "iterations": r["iterations"], "converged": bool(r["converged"]),
-           "etotal_per_atom_Ha": r["etotal"] / 2}
-    out.update(_scf_history_summary(S))
-    S.close()
+                       "GGA-PBE, Fermi-Dirac 1000 K, degree 20, SCF to 1e-6 density residual; C.pot stands "
+                       "in for Si" % len(ks), "modes": {}}
+    for name, thr in (("fp64", 0.0), ("fp32_until_1e-4", 1e-4)):
+        kw = dict(xc="GGA_PBE", tol=1e-6, kpts=ks, kpt_weights=w, fp64_switch_residual=thr)
+        warm = SCF(H, 12, beta, max_iter=1, **kw)
+        warm.run()
+        warm.close()
+        S = SCF(H, 12, beta, **kw)
+        synchronize()
+        t0 = time.perf_counter()
+        r = S.run()
+        synchronize()
+        wall = time.perf_counter() - t0
+        m = {"scf_wall_s": round(wall, 4), "iterations": r["iterations"], "converged": bool(r["converged"]),
+             "etotal_per_atom_Ha": r["etotal"] / 2}
+        m.update(_scf_history_summary(S))
+        S.close()
+        out["modes"][name] = m
+    a, b = out["modes"]["fp64"], out["modes"]["fp32_until_1e-4"]
+    out["speedup_scf_wall_fp32_until_1e-4"] = round(a["scf_wall_s"] / b["scf_wall_s"], 3)
+    out["etotal_err_per_atom_Ha_fp32_until_1e-4"] = abs(a["etotal_per_atom_Ha"] - b["etotal_per_atom_Ha"])
     return out
 
 
