@@ -179,7 +179,8 @@ def run_kpoint_batch(steps, warmup):
     launch per kernel and step; against the same filter per k-point in
     complex128 (the FP64 complex path) and in complex64, one call each.
     Then one whole CheFSI pass in complex128 (lynx_batch_solve against one
-    lynx_chefsi_solve per k-point, best of two from the same start)."""
+    lynx_chefsi_solve per k-point, best of two from the same start), and the
+    batched pass with its filter in complex64."""
     from systems import si2_fcc
     from lynx_amd import _lib as L, core
     from lynx_amd.hipmem import DeviceArray, synchronize
@@ -234,7 +235,8 @@ def run_kpoint_batch(steps, warmup):
     # solve (filter, Cholesky-QR, H psi, Rayleigh-Ritz) against one
     # lynx_chefsi_solve per k-point, from the same start blocks
     solve = {}
-    for mode in ("batched", "per_kpoint"):
+    for mode in ("batched", "batched_fp32_filter", "per_kpoint"):
+        bf.set_filter_precision("fp32" if mode == "batched_fp32_filter" else "fp64")
         blocks = [DeviceArray.from_numpy(x.T.copy()) for x in X]
         best, eig = 1e9, None
         for rep in range(2):
@@ -242,7 +244,7 @@ def run_kpoint_batch(steps, warmup):
                 d.upload(X[i].T.copy())
             synchronize()
             t0 = time.perf_counter()
-            if mode == "batched":
+            if mode.startswith("batched"):
                 eig, _ = bf.solve_device(np.complex128, [d.value for d in blocks], [nd] * nk, [nb] * nk, DEGREE,
                                          [lc] * nk, [lo] * nk, [hi] * nk)
             else:
@@ -259,6 +261,8 @@ def run_kpoint_batch(steps, warmup):
         for d in blocks:
             d.free()
     eig_diff = max(float(np.abs(a - b).max()) for a, b in zip(solve["batched"][1], solve["per_kpoint"][1]))
+    eig_diff32 = max(float(np.abs(a - b).max())
+                     for a, b in zip(solve["batched_fp32_filter"][1], solve["per_kpoint"][1]))
     bf.close()
     return {"workload": "Si2 FCC primitive cell a=10.26 Bohr, 24^3 grid, 6x6x6 MP -> %d k-points, %d complex64 "
                         "orbitals per k-point, degree %d, bounds from %d Lanczos steps at k-point 0"
@@ -272,7 +276,9 @@ def run_kpoint_batch(steps, warmup):
             "chefsi_pass_c128": {"batched_ms": round(solve["batched"][0], 3),
                                  "per_kpoint_ms": round(solve["per_kpoint"][0], 3),
                                  "speedup": round(solve["per_kpoint"][0] / solve["batched"][0], 2),
-                                 "max_abs_ritz_value_diff_Ha": eig_diff}}
+                                 "max_abs_ritz_value_diff_Ha": eig_diff,
+                                 "batched_fp32_filter_ms": round(solve["batched_fp32_filter"][0], 3),
+                                 "max_abs_ritz_value_diff_Ha_fp32_filter": eig_diff32}}
 
 
 def run_hetero_batch(steps, warmup):

@@ -468,6 +468,12 @@ class BatchFilter:
         self._check(L.lib().lynx_batch_filter(self._b, L.dtype_code(dtype), xp, ld, yp, ld, L.iptr(nc), int(degree),
                                               L.dptr(lc), L.dptr(lo), L.dptr(hi), L.MEM_DEVICE))
 
+    def set_filter_precision(self, precision):
+        """"fp64" (default) or "fp32": the filter of solve_device on
+        single-precision copies (lynx_batch_set_filter_precision)."""
+        self._check(L.lib().lynx_batch_set_filter_precision(
+            self._b, {"fp64": L.FILTER_FP64, "fp32": L.FILTER_FP32}[precision]))
+
     def solve_device(self, dtype, ptrs, lds, ncols, degree, lambda_cutoff, eig_min, eig_max):
         """One CheFSI pass for every unit (lynx_batch_solve) on device blocks
         (lists of addresses, leading dimensions and column counts), in
