@@ -213,6 +213,130 @@ __global__ void __launch_bounds__(kThreads) k_gram_bf16x3(const float *__restric
     }
 }
 
+// ------------------------------------------------------------- FP64 Gram --
+// C = A^T B for FP64 blocks on v_mfma_f64_16x16x4f64 (lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16],
+// v = 0..3; checked on gfx950).  Same decomposition as k_gram_bf16x3: one
+// workgroup per (tile pair, K chunk), for A^T A only the tiles on and above
+// the diagonal (written to both halves), one FP64 partial product per chunk
+// summed in a fixed order by gram_reduce.  A workgroup of four waves owns a
+// 128 x 128 tile; wave w takes the 64 x 64 quarter (w / 2, w % 2), 4 x 4
+// products per 4-deep step, each fragment read once from LDS and used four
+// times.  Per 32-deep block the tile's 128 columns of A and of B are staged
+// in LDS with k contiguous (33 doubles per column, so the 16 lanes reading
+// one k of 16 columns hit distinct banks); the next block's loads are in
+// flight while the current block's products run.
+constexpr int kDT = 128;          // C tile edge
+constexpr int kDKB = 32;          // K rows per LDS block
+constexpr int kDP = kDKB + 1;     // doubles per LDS column
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+// This thread's share of a 32-deep block of a 128-column tile: 16
+// consecutive K rows (from k0 + koff, masked by kend and the column count)
+// of column c = t / 2.
+struct DSlab {
+    double v[kDKB / 2];
+};
+
+__device__ inline void load_dslab(const double *__restrict__ X, long ldx, long k0, long kend, int col0, int N, int t,
+                                  DSlab &s)
+{
+    const int c = t >> 1;
+    const int koff = (t & 1) * (kDKB / 2);
+    const int gc = col0 + c;
+    const long r0 = k0 + koff;
+    if (gc < N && r0 + kDKB / 2 <= kend) {
+        const double2 *p = reinterpret_cast<const double2 *>(X + (long)gc * ldx + r0);
+#pragma unroll
+        for (int q = 0; q < kDKB / 4; ++q) {
+            const double2 f = p[q];
+            s.v[2 * q] = f.x;
+            s.v[2 * q + 1] = f.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kDKB / 2; ++i) s.v[i] = (gc < N && r0 + i < kend) ? X[(long)gc * ldx + r0 + i] : 0.0;
+    }
+}
+
+__device__ inline void store_dslab(const DSlab &s, double *lds, int t)
+{
+    double *d = lds + (t >> 1) * kDP + (t & 1) * (kDKB / 2);
+#pragma unroll
+    for (int i = 0; i < kDKB / 2; ++i) d[i] = s.v[i];
+}
+
+__global__ void __launch_bounds__(256) k_gram_f64(const double *__restrict__ A, long lda,
+                                                  const double *__restrict__ B, long ldb, long K, int N, long kch,
+                                                  int tiles_n, int same, int pairs, int items, int per_xcd,
+                                                  double *__restrict__ part)
+{
+    const int item = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (item >= items) return;
+    const int chunk = item / pairs, pair = item % pairs;
+    __shared__ double sA[kDT * kDP], sB[kDT * kDP];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    int tm, tn;
+    tile_of(pair, tiles_n, same, tm, tn);
+    const int m0 = tm * kDT, n0 = tn * kDT;
+    const long kbeg = (long)chunk * kch, kend = std::min(K, kbeg + kch);
+    const bool one = same && tm == tn;          // B's tile is A's
+    const double *sb = one ? sA : sB;
+    const double *Bsrc = same ? A : B;
+    const long ldbs = same ? lda : ldb;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+    const int lr = l & 15, lk = l >> 4;
+    double4v acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = double4v{0.0, 0.0, 0.0, 0.0};
+    DSlab xa, xb;
+    if (kbeg < kend) {
+        load_dslab(A, lda, kbeg, kend, m0, N, t, xa);
+        if (!one) load_dslab(Bsrc, ldbs, kbeg, kend, n0, N, t, xb);
+    }
+    for (long k0 = kbeg; k0 < kend; k0 += kDKB) {
+        store_dslab(xa, sA, t);
+        if (!one) store_dslab(xb, sB, t);
+        __syncthreads();
+        if (k0 + kDKB < kend) {
+            load_dslab(A, lda, k0 + kDKB, kend, m0, N, t, xa);
+            if (!one) load_dslab(Bsrc, ldbs, k0 + kDKB, kend, n0, N, t, xb);
+        }
+#pragma unroll
+        for (int ks = 0; ks < kDKB; ks += 4) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = sA[(wm + 16 * i + lr) * kDP + ks + lk];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = sb[(wn + 16 * j + lr) * kDP + ks + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double *P = part + (size_t)chunk * N * N;
+    const bool mirror = same && tm != tn;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + wn + 16 * j + lr;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm + 16 * i + lk + 4 * v;
+                if (m < N && n < N) {
+                    P[m + (size_t)n * N] = acc[i][j][v];
+                    if (mirror) P[n + (size_t)m * N] = acc[i][j][v];
+                }
+            }
+        }
+}
+
 // ---------------------------------------------------------------- rotation --
 // C (M x N) = A (M x K) Q (K x N) for a real FP32 block A (M grid points,
 // column-major, ld lda) and an FP64 Q, as split-BF16x3 products: computed as
@@ -424,6 +548,39 @@ int gram_bf16x3(const float *A, long lda, const float *B, long ldb, long K, int 
                                                     part);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("k_gram_bf16x3: ") + hipGetErrorString(e));
+    return used;
+}
+
+int gram_f64_splits(long K, int N)
+{
+    const int tiles = (N + kDT - 1) / kDT;
+    const int pairs = tiles * (tiles + 1) / 2;
+    const long want = std::max(1L, (768L + pairs - 1) / pairs);
+    return (int)std::max(1L, std::min(want, (K + kDKB - 1) / kDKB));
+}
+
+bool gram_f64_fits(const double *A, long lda, const double *B, long ldb)
+{
+    return lda % 2 == 0 && ldb % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(B) % 16 == 0;
+}
+
+int gram_f64(const double *A, long lda, const double *B, long ldb, long K, int N, double *part, int nsplit,
+             hipStream_t st)
+{
+    if (N < 1 || K < 1 || nsplit < 1) throw std::runtime_error("gram_f64: empty product");
+    if (!gram_f64_fits(A, lda, B, ldb)) throw std::runtime_error("gram_f64: columns must be 16-byte aligned");
+    const int tiles = (N + kDT - 1) / kDT;
+    const int same = A == B && lda == ldb ? 1 : 0;
+    const int pairs = same ? tiles * (tiles + 1) / 2 : tiles * tiles;
+    // chunk length: a multiple of the LDS block, so 16-byte loads stay aligned
+    long kch = (K + nsplit - 1) / nsplit;
+    kch = (kch + kDKB - 1) / kDKB * kDKB;
+    const int used = (int)((K + kch - 1) / kch);
+    const int items = pairs * used, per_xcd = (items + 7) / 8;
+    k_gram_f64<<<8 * per_xcd, 256, 0, st>>>(A, lda, B, ldb, K, N, kch, tiles, same, pairs, items, per_xcd, part);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string("k_gram_f64: ") + hipGetErrorString(e));
     return used;
 }
 

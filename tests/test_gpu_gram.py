@@ -1,7 +1,10 @@
 """Gram products through lynx_gram (the overlap / projection product of the
 CheFSI pass) against numpy in FP64 on the same inputs:
 
-- FP64 blocks: rocBLAS split-K GEMMs, 1e-13 relative;
+- FP64 blocks: the overlap A^T A through the hand-written FP64 MFMA kernel
+  (upper tiles, mirrored), a general A^T B through rocBLAS split-K GEMMs,
+  1e-13 relative; one tile, several 128-column tiles with a ragged last
+  one, K not a multiple of the 32-row block, host and device inputs;
 - FP32 blocks, errors per entry relative to dV |a_i| |b_j| (the scale of
   a product of two columns): LYNX_GRAM_FP32 (FP32 GEMMs, FP64 chunk sums)
   5e-6 (1.2e-6 measured on a one-chunk sum of squares); LYNX_GRAM_BF16X3
@@ -86,14 +89,29 @@ def test_split_bf16x3_gram_exact_on_bfloat16_inputs_and_device_memory():
     dA.free(); dB.free()
 
 
-def test_fp64_gram_matches_numpy():
+@pytest.mark.parametrize("grid,n", [(16, 24), ((20, 21, 22), 300), ((24, 25, 26), 129)])
+def test_fp64_gram_matches_numpy(grid, n):
     from lynx_amd import _lib as L
-    s, H = _ctx(16)
-    nd, n = s.nd, 24
-    A = np.asfortranarray(np.random.default_rng(2).uniform(-0.5, 0.5, (nd, n)))
+    from lynx_amd.hipmem import DeviceArray, synchronize
+    s = System(grid, 10.0, np.zeros((0, 3)), with_nonlocal=False)
+    H = s.hamiltonian(0)
+    nd = s.nd
+    rng = np.random.default_rng(2 + n)
+    A = np.asfortranarray(rng.uniform(-0.5, 0.5, (nd, n)))
+    B = np.asfortranarray(rng.uniform(-0.5, 0.5, (nd, n)))
+    dV = abs(np.linalg.det(s.latvec)) / nd
+    for X, Y in ((A, A), (A, B)):
+        c = np.zeros((n, n))
+        L.check(L.lib().lynx_gram(H.ctx, L.F64, L.dptr(X), nd, L.dptr(Y), nd, n, L.dptr(c), L.MEM_HOST), H.ctx)
+        ref = dV * (X.T @ Y)
+        assert np.abs(c.T - ref).max() / np.abs(ref).max() < 1e-13, (grid, n, X is Y)
+    # device input, the overlap
+    dA = DeviceArray.from_numpy(np.ascontiguousarray(A.T))
     c = np.zeros((n, n))
-    L.check(L.lib().lynx_gram(H.ctx, L.F64, L.dptr(A), nd, L.dptr(A), nd, n, L.dptr(c), L.MEM_HOST), H.ctx)
-    ref = s.latvec[0, 0] ** 3 / nd * (A.T @ A)
+    L.check(L.lib().lynx_gram(H.ctx, L.F64, dA.value, nd, dA.value, nd, n, L.dptr(c), L.MEM_DEVICE), H.ctx)
+    synchronize()
+    dA.free()
+    ref = dV * (A.T @ A)
     assert np.abs(c.T - ref).max() / np.abs(ref).max() < 1e-13
 
 
