@@ -297,8 +297,9 @@ lynx_status lynx_get_gram_precision(const lynx_ctx *ctx, int *mode);
  * written to host memory) for two blocks of n columns (leading dimensions
  * >= Nd, in host or device memory per mem) on the context's grid, through
  * the product the solve uses for dtype: for FP64 the overlap (a == b) on a
- * hand-written FP64 matrix-core kernel that computes the upper tiles only,
- * other FP64 / complex128 products as GEMMs; for single precision the FP32
+ * hand-written FP64 matrix-core kernel that computes the upper tiles only
+ * (the solve's projection, whose Hermitian part it takes, too), other FP64 /
+ * complex128 products as GEMMs; for single precision the FP32
  * GEMMs or split-BF16x3 products selected by lynx_set_gram_precision.  The overlap product of EigenSolver::orthogonalize
  * and project_hamiltonian (EigenSolver.cpp:621-660). */
 lynx_status lynx_gram(lynx_ctx *ctx, int dtype, const void *a, int lda, const void *b, int ldb, int n, void *c,

@@ -572,7 +572,10 @@ constexpr double kCholQrFp64Ratio = 100.0;
 // Single-precision operands (the mixed-precision pass) use up to 64 chunks,
 // so that each partial product sums at most a few thousand terms in FP32
 // before the FP64 reduction.  C is FP64 (wide_of(e)).
-void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, int n, double alpha, void *C)
+// hermitian: the caller takes the result's Hermitian part (the projection
+// X^H (H X)), so for FP64 only its upper tiles need forming.
+void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, int n, double alpha, void *C,
+          bool hermitian = false)
 {
     const long nd = c->nd;
     if (e == gpu::F32 && c->gram_precision != LYNX_GRAM_FP32 && lda % 4 == 0 && ldb % 4 == 0 &&
@@ -584,16 +587,18 @@ void gram(lynx_ctx *c, int e, const void *A, long lda, const void *B, long ldb, 
         gpu::gram_reduce(e, W, used, n, alpha, C, c->stream);
         return;
     }
-    if (e == gpu::F64 && A == B && lda == ldb &&
+    const bool sym = hermitian || (A == B && lda == ldb);
+    if (e == gpu::F64 && sym &&
         gpu::gram_f64_fits(static_cast<const double *>(A), lda, static_cast<const double *>(B), ldb)) {
-        // the overlap A^T A: the hand-written FP64 MFMA kernel computes only
-        // the upper tiles (5.8 against 7.5 ms for rocBLAS's full product at
-        // 729,000 x 480); general products stay rocBLAS GEMMs, which run
-        // them faster than this kernel
+        // a symmetric result (the overlap A^T A, or the projection whose
+        // Hermitian part is taken): the hand-written FP64 MFMA kernel forms
+        // only the upper tiles (5.8 against 7.5 ms for rocBLAS's full
+        // product at 729,000 x 480); general products stay rocBLAS GEMMs,
+        // which run them faster than this kernel
         const int ns = gpu::gram_f64_splits(nd, n);
         void *W = c->gwork.get((size_t)ns * n * n * sizeof(double));
         const int used = gpu::gram_f64(static_cast<const double *>(A), lda, static_cast<const double *>(B), ldb, nd,
-                                       n, static_cast<double *>(W), ns, c->stream);
+                                       n, static_cast<double *>(W), ns, true, c->stream);
         gpu::gram_reduce(e, W, used, n, alpha, C, c->stream);
         return;
     }
@@ -1285,7 +1290,7 @@ lynx_status lynx_project_hamiltonian(lynx_ctx *ctx, int dtype, const void *x, in
         void *hx = block(ctx->wB, dtype, ldw, ncol);
         apply_op(ctx, dtype, dx, ldd, nullptr, 0, hx, ldw, ncol, 1.0, 0.0, 0.0);
         void *H = ctx->s2.get((size_t)ncol * ncol * es);
-        gram(ctx, dtype, dx, ldd, hx, ldw, ncol, ctx->st.dV, H);
+        gram(ctx, dtype, dx, ldd, hx, ldw, ncol, ctx->st.dV, H, true);
         gpu::hermitian_part(dtype, H, ncol, ctx->stream);
         HIP_OK(hipMemcpyAsync(hs, H, (size_t)ncol * ncol * es, hipMemcpyDeviceToHost, ctx->stream));
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -1442,7 +1447,7 @@ lynx_status lynx_chefsi_solve(lynx_ctx *ctx, int dtype, void *psi, int ld, int n
         // a10: Hs = dV C^H H C (B <- H C)
         apply_op(ctx, dtype, C, ldw, nullptr, 0, B, ldw, nband, 1.0, 0.0, 0.0);
         void *H = ctx->s2.get((size_t)nband * nband * es);
-        gram(ctx, dtype, C, ldw, B, ldw, nband, ctx->st.dV, H);
+        gram(ctx, dtype, C, ldw, B, ldw, nband, ctx->st.dV, H, true);
         gpu::hermitian_part(dtype, H, nband, ctx->stream);
         // a11: Hs = Q diag(eig) Q^H
         subspace_eig(ctx, dtype, nband, eigvals);

@@ -120,11 +120,12 @@ int gram_bf16x3(const float *A, long lda, const float *B, long ldb, long K, int 
 int gram_bf16x3_splits(long K, int N);
 // FP64 Gram partials on the matrix cores (kernels_mfma.hip,
 // v_mfma_f64_16x16x4f64): part[s] (N x N) = A_s^T B_s over consecutive K
-// chunks, for A == B only the upper tiles (mirrored).  Columns must be
-// 16-byte aligned (gram_f64_fits).  At most nsplit chunks; returns the
-// number written (sum them with gram_reduce).
+// chunks; with sym only the upper tiles, mirrored into the lower ones
+// (exact for A == B; for a product whose Hermitian part is taken, that
+// part).  Columns must be 16-byte aligned (gram_f64_fits).  At most nsplit
+// chunks; returns the number written (sum them with gram_reduce).
 int gram_f64(const double *A, long lda, const double *B, long ldb, long K, int N, double *part, int nsplit,
-             hipStream_t st);
+             bool sym, hipStream_t st);
 int gram_f64_splits(long K, int N);
 bool gram_f64_fits(const double *A, long lda, const double *B, long ldb);
 // Split-BF16x3 rotation on the matrix cores (kernels_mfma.hip): C (M x N) =

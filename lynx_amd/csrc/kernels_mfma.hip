@@ -217,9 +217,10 @@ __global__ void __launch_bounds__(kThreads) k_gram_bf16x3(const float *__restric
 // C = A^T B for FP64 blocks on v_mfma_f64_16x16x4f64 (lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16],
 // v = 0..3; checked on gfx950).  Same decomposition as k_gram_bf16x3: one
-// workgroup per (tile pair, K chunk), for A^T A only the tiles on and above
-// the diagonal (written to both halves), one FP64 partial product per chunk
-// summed in a fixed order by gram_reduce.  A workgroup of four waves owns a
+// workgroup per (tile pair, K chunk); for A^T A, or a product whose
+// Hermitian part is wanted (upper), only the tiles on and above the
+// diagonal, their upper entries written to both halves; one FP64 partial
+// product per chunk summed in a fixed order by gram_reduce.  A workgroup of four waves owns a
 // 128 x 128 tile; wave w takes the 64 x 64 quarter (w / 2, w % 2), 4 x 4
 // products per 4-deep step, each fragment read once from LDS and used four
 // times.  Per 32-deep block the tile's 128 columns of A and of B are staged
@@ -268,8 +269,8 @@ __device__ inline void store_dslab(const DSlab &s, double *lds, int t)
 
 __global__ void __launch_bounds__(256) k_gram_f64(const double *__restrict__ A, long lda,
                                                   const double *__restrict__ B, long ldb, long K, int N, long kch,
-                                                  int tiles_n, int same, int pairs, int items, int per_xcd,
-                                                  double *__restrict__ part)
+                                                  int tiles_n, int same, int upper, int pairs, int items,
+                                                  int per_xcd, double *__restrict__ part)
 {
     const int item = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (item >= items) return;
@@ -277,7 +278,7 @@ __global__ void __launch_bounds__(256) k_gram_f64(const double *__restrict__ A, 
     __shared__ double sA[kDT * kDP], sB[kDT * kDP];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     int tm, tn;
-    tile_of(pair, tiles_n, same, tm, tn);
+    tile_of(pair, tiles_n, upper, tm, tn);
     const int m0 = tm * kDT, n0 = tn * kDT;
     const long kbeg = (long)chunk * kch, kend = std::min(K, kbeg + kch);
     const bool one = same && tm == tn;          // B's tile is A's
@@ -320,7 +321,10 @@ __global__ void __launch_bounds__(256) k_gram_f64(const double *__restrict__ A, 
         __syncthreads();
     }
     double *P = part + (size_t)chunk * N * N;
-    const bool mirror = same && tm != tn;
+    // upper: the tile's entries above the diagonal are mirrored below it;
+    // a diagonal tile of a non-symmetric product keeps its upper half only
+    const bool mirror = upper && tm != tn;
+    const bool diag_upper = upper && !same && tm == tn;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -330,8 +334,15 @@ __global__ void __launch_bounds__(256) k_gram_f64(const double *__restrict__ A, 
             for (int v = 0; v < 4; ++v) {
                 const int m = m0 + wm + 16 * i + lk + 4 * v;
                 if (m < N && n < N) {
-                    P[m + (size_t)n * N] = acc[i][j][v];
-                    if (mirror) P[n + (size_t)m * N] = acc[i][j][v];
+                    if (diag_upper) {
+                        if (m <= n) {
+                            P[m + (size_t)n * N] = acc[i][j][v];
+                            P[n + (size_t)m * N] = acc[i][j][v];
+                        }
+                    } else {
+                        P[m + (size_t)n * N] = acc[i][j][v];
+                        if (mirror) P[n + (size_t)m * N] = acc[i][j][v];
+                    }
                 }
             }
         }
@@ -566,19 +577,21 @@ bool gram_f64_fits(const double *A, long lda, const double *B, long ldb)
 }
 
 int gram_f64(const double *A, long lda, const double *B, long ldb, long K, int N, double *part, int nsplit,
-             hipStream_t st)
+             bool sym, hipStream_t st)
 {
     if (N < 1 || K < 1 || nsplit < 1) throw std::runtime_error("gram_f64: empty product");
     if (!gram_f64_fits(A, lda, B, ldb)) throw std::runtime_error("gram_f64: columns must be 16-byte aligned");
     const int tiles = (N + kDT - 1) / kDT;
     const int same = A == B && lda == ldb ? 1 : 0;
-    const int pairs = same ? tiles * (tiles + 1) / 2 : tiles * tiles;
+    const int upper = sym || same ? 1 : 0;
+    const int pairs = upper ? tiles * (tiles + 1) / 2 : tiles * tiles;
     // chunk length: a multiple of the LDS block, so 16-byte loads stay aligned
     long kch = (K + nsplit - 1) / nsplit;
     kch = (kch + kDKB - 1) / kDKB * kDKB;
     const int used = (int)((K + kch - 1) / kch);
     const int items = pairs * used, per_xcd = (items + 7) / 8;
-    k_gram_f64<<<8 * per_xcd, 256, 0, st>>>(A, lda, B, ldb, K, N, kch, tiles, same, pairs, items, per_xcd, part);
+    k_gram_f64<<<8 * per_xcd, 256, 0, st>>>(A, lda, B, ldb, K, N, kch, tiles, same, upper, pairs, items, per_xcd,
+                                            part);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw std::runtime_error(std::string("k_gram_f64: ") + hipGetErrorString(e));
     return used;
