@@ -406,7 +406,8 @@ const char *lynx_scf_eig_last_error(const lynx_scf_eig *driver);
  * nspin x nkpt x nband; *fermi the Fermi level; *precision_used the filter
  * precision of this step (LYNX_FILTER_*).  Passes over several complex
  * k-points on device blocks solve each spin's k-points together through
- * one lynx_batch_solve (created on first use, owned by the driver): FP64
+ * one lynx_batch_solve (one batch per spin, created on first use, owned
+ * by the driver): FP64
  * passes as lynx_chefsi_solve runs them, reduced-precision passes with the
  * filter in complex64 and the subspace steps in complex128 (gram_precision
  * does not apply to them).  Host blocks loop over the k-points with

@@ -26,7 +26,7 @@ struct lynx_scf_eig {
     double fermi = 0.0;
     int last_precision = LYNX_FILTER_FP64;
     std::string err;
-    lynx_batch *batch = nullptr;   // one spin's k-points: FP64 passes on device blocks
+    std::vector<lynx_batch *> batch;   // per spin: its k-points' passes on device blocks
 };
 
 namespace {
@@ -156,7 +156,9 @@ lynx_status lynx_scf_eig_create(lynx_scf_eig **out, lynx_ctx *ctx, const lynx_sc
 
 lynx_status lynx_scf_eig_destroy(lynx_scf_eig *d)
 {
-    if (d && d->batch) (void)lynx_batch_destroy(d->batch);
+    if (d)
+        for (lynx_batch *b : d->batch)
+            if (b) (void)lynx_batch_destroy(b);
     delete d;
     return LYNX_OK;
 }
@@ -218,19 +220,23 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
         // batched launches, GEMMs and rocSOLVER calls).  Reduced-precision
         // passes run its filter in complex64 and the subspace steps in
         // complex128.
+        // (one batch per spin, so each keeps its plan and graph for its own
+        // blocks)
         const bool batched = p.complex_kpt && p.nkpt > 1 && mem == LYNX_MEM_DEVICE;
-        if (batched && !d->batch) {
-            std::vector<lynx_ctx *> ctxs(p.nkpt, d->ctx);
-            const lynx_status bs = lynx_batch_create(&d->batch, p.nkpt, ctxs.data(), d->kpts.data());
-            if (bs != LYNX_OK) {
-                d->batch = nullptr;
-                throw Fail{bs, "lynx_batch_create failed"};
-            }
-        }
         if (batched) {
-            const lynx_status bs =
-                lynx_batch_set_filter_precision(d->batch, single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64);
-            if (bs != LYNX_OK) throw Fail{bs, "lynx_batch_set_filter_precision failed"};
+            d->batch.resize(p.nspin, nullptr);
+            for (int s = 0; s < p.nspin; ++s) {
+                if (!d->batch[s]) {
+                    std::vector<lynx_ctx *> ctxs(p.nkpt, d->ctx);
+                    lynx_batch *b = nullptr;
+                    const lynx_status bs = lynx_batch_create(&b, p.nkpt, ctxs.data(), d->kpts.data());
+                    if (bs != LYNX_OK) throw Fail{bs, "lynx_batch_create failed"};
+                    d->batch[s] = b;
+                }
+                const lynx_status bs =
+                    lynx_batch_set_filter_precision(d->batch[s], single ? LYNX_FILTER_FP32 : LYNX_FILTER_FP64);
+                if (bs != LYNX_OK) throw Fail{bs, "lynx_batch_set_filter_precision failed"};
+            }
         }
         for (int pass = 0; pass < passes; ++pass) {
             for (int s = 0; s < p.nspin; ++s) {
@@ -241,11 +247,11 @@ lynx_status lynx_scf_eig_step_veff(lynx_scf_eig *d, int scf_iter, double scf_res
                     std::vector<long> lds(nk, ld);
                     std::vector<int> nc(nk, N);
                     std::vector<double> lc(nk, d->lambda_c), lo(nk, d->eig_min[s]), hi(nk, d->eig_max[s]);
-                    const lynx_status bs = lynx_batch_solve(d->batch, dtype, ps.data(), lds.data(), nc.data(),
+                    const lynx_status bs = lynx_batch_solve(d->batch[s], dtype, ps.data(), lds.data(), nc.data(),
                                                             eig + (long)s * nk * N, lc.data(), lo.data(), hi.data(),
                                                             p.cheb_degree, nullptr);
                     if (bs != LYNX_OK)
-                        throw Fail{bs, std::string("lynx_batch_solve: ") + lynx_batch_last_error(d->batch)};
+                        throw Fail{bs, std::string("lynx_batch_solve: ") + lynx_batch_last_error(d->batch[s])};
                     continue;
                 }
                 for (int k = 0; k < p.nkpt; ++k) {

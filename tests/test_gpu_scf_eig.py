@@ -82,11 +82,12 @@ def test_eigen_stage_fp32_complex_kpoints(si8_16):
         assert abs(efmx - ef64) < 1e-5
 
 
-def test_eigen_stage_kpoints_on_device_blocks(si8_16):
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_eigen_stage_kpoints_on_device_blocks(si8_16, nspin):
     # FP64 passes over three k-points on device blocks run as one batched
-    # solve per pass (lynx_batch_solve): eigenvalues, bounds, Fermi level and
-    # occupations against the oracle's per-k-point stage, and one counted
-    # solve per k-point and pass
+    # solve per spin and pass (lynx_batch_solve, one batch per spin):
+    # eigenvalues, bounds, Fermi level and occupations against the oracle's
+    # per-k-point stage, and one counted solve per k-point, spin and pass
     import ctypes
     from systems import A_SI
     from lynx_amd import core, _lib as L
@@ -96,22 +97,24 @@ def test_eigen_stage_kpoints_on_device_blocks(si8_16):
     w = np.array([0.25, 0.5, 0.25])
     op, H = s.oracle_op(), s.hamiltonian()
     X0 = s.random_block(NB, 89, complex_=True)
-    ref = oracle.ScfEigStage(op, NB, NE, BETA, kpts=kp, kpt_weights=w)
-    drv = core.ScfEigenStage(H, NB, NE, BETA, kpts=kp, kpt_weights=w)
-    psi_ref = [X0.copy(order="F") for _ in range(3)]
-    dev = [DeviceArray.from_numpy(np.ascontiguousarray(X0.T)) for _ in range(3)]   # column-major blocks
+    veff = np.concatenate([s.veff, s.veff + 0.05][:nspin])
+    ref = oracle.ScfEigStage(op, NB, NE, BETA, kpts=kp, kpt_weights=w, nspin=nspin)
+    drv = core.ScfEigenStage(H, NB, NE, BETA, kpts=kp, kpt_weights=w, nspin=nspin)
+    nb = 3 * nspin
+    psi_ref = [X0.copy(order="F") for _ in range(nb)]
+    dev = [DeviceArray.from_numpy(np.ascontiguousarray(X0.T)) for _ in range(nb)]   # column-major blocks
     cnt0, cnt = ctypes.c_long(), ctypes.c_long()
     L.check(L.lib().lynx_solve_count(H.ctx, ctypes.byref(cnt0)), H.ctx)
     for it in range(3):
-        psi_ref, e_ref = ref.step(it, s.veff, psi_ref)
-        e, occ, ef, prec = drv.step_device(it, 1.0, s.veff, [d.value for d in dev], s.nd)
+        psi_ref, e_ref = ref.step(it, veff, psi_ref)
+        e, occ, ef, prec = drv.step_device(it, 1.0, veff, [d.value for d in dev], s.nd)
         assert prec == 0
         assert np.abs(e - e_ref).max() < 1e-9 * max(1, np.abs(e_ref).max()), it
         lc, mn, mx = drv.state()
-        assert abs(lc - ref.lambda_c) < 1e-9 and abs(mn[0] - ref.eig_min[0]) < 1e-9
+        assert abs(lc - ref.lambda_c) < 1e-9 and np.abs(mn - ref.eig_min).max() < 1e-9
         assert abs(ef - ref.fermi) < 1e-9 and np.abs(occ - ref.occ).max() < 1e-7
     L.check(L.lib().lynx_solve_count(H.ctx, ctypes.byref(cnt)), H.ctx)
-    assert cnt.value - cnt0.value == ref.solves == 3 * (4 + 1 + 1)
+    assert cnt.value - cnt0.value == ref.solves == nb * (4 + 1 + 1)
     for d in dev:
         d.free()
 
