@@ -2151,6 +2151,9 @@ T *dev_upload(const std::vector<T> &v, hipStream_t st)
 template <class S> constexpr int batch_rp_orth() { return base_rp<S>() / 2 > 0 ? base_rp<S>() / 2 : 1; }
 template <class S> constexpr int batch_rp_nonorth() { return base_rp<S>() / 4 > 0 ? base_rp<S>() / 4 : 1; }
 constexpr int kBatchNT = 256;
+// runs per thread of the batched run-vector kinds (two, for taller bands
+// and less halo, ran slower: more registers, fewer waves)
+constexpr int kBatchRuns = 1;
 // LDS row pitch of batched non-orthogonal units with rows of at most
 // 40 - 2R points
 constexpr int kShortNonorthPitch = 40;
@@ -2256,10 +2259,10 @@ void build_batch(BatchPlan *P, const BatchUnit *units, int n, hipStream_t st)
         StepArgs<A> &p = q.b;
         p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = U.ncol;
         if (kind[u] == 2 || kind[u] == 3) {
-            // one run per thread, band rows at lane multiples of 16
+            // kBatchRuns runs per thread, band rows at lane multiples of 16
             const int VW = kind[u] == 3 ? 2 : int(16 / sizeof(S));
             p.lpr = (g.nx / VW + 15) / 16 * 16;
-            const int rpb = kBatchNT / p.lpr;
+            const int rpb = kBatchRuns * kBatchNT / p.lpr;
             const int nb = (g.ny + rpb - 1) / rpb;
             p.tyr = (g.ny + nb - 1) / nb;
         } else {
@@ -2414,8 +2417,9 @@ void launch_stencil_batch(const BatchPlan *P, const BatchPlan::SGroup &G, const 
             constexpr int HV = HP == 3 ? 2 : 3;
             const int VW = G.kind == 3 ? 2 : int(16 / sizeof(S));
             const size_t shm = 2 * sizeof(S) * ((size_t)W * (G.max_tyr + 2 * kMaxR) + kBatchNT * VW);
-            if (G.kind == 2) go(k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 2>, kBatchNT, shm);
-            else if constexpr (sizeof(S) == 4) go(k_stencil_batch<S, kMaxR, 1, kBatchNT, HV, W, 3>, kBatchNT, shm);
+            if (G.kind == 2) go(k_stencil_batch<S, kMaxR, kBatchRuns, kBatchNT, HV, W, 2>, kBatchNT, shm);
+            else if constexpr (sizeof(S) == 4)
+                go(k_stencil_batch<S, kMaxR, kBatchRuns, kBatchNT, HV, W, 3>, kBatchNT, shm);
         }
     } else if (G.kind == 1 && G.nt == 192) {
         // short non-orthogonal rows, two halo slots per thread
