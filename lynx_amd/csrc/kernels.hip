@@ -1801,7 +1801,8 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     p.x = x; p.old = old; p.out = out; p.veff = static_cast<const A *>(veff);
     p.ldx = ldx; p.ldo = ldo; p.ldout = ldout;
     p.nx = g.nx; p.ny = g.ny; p.nz = g.nz; p.ncol = ncol;
-    const int pitch = tile_pitch(g.nx);
+    // short rows: a 40-point pitch, so more workgroups fit a CU
+    const int pitch = g.nx + 2 * kMaxR <= 40 ? 40 : tile_pitch(g.nx);
     const int max_rows = nonorth_max_rows<S>(pitch, NT);
     int nb = (int)(((long)g.nx * g.ny + pts - 1) / pts);
     nb = std::max(1, std::min(nb, g.ny));
@@ -1831,7 +1832,10 @@ void launch_step_nonorth(const StencilGeom &g, const void *x, long ldx, const vo
     const int blocks = ((p.nwork + 7) / 8) * 8;
     const size_t shm = 4 * sizeof(S) * ((size_t)pitch * (p.tyr + 2 * kMaxR) + NT);
     const bool small_halo = 2 * kMaxR * g.nx <= NT * HS;
-    if (pitch == 64) {
+    if (pitch == 40) {
+        if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 40>), dim3(blocks), dim3(NT), shm, st, q);
+        else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 40>), dim3(blocks), dim3(NT), shm, st, q);
+    } else if (pitch == 64) {
         if (small_halo) hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HS, 64>), dim3(blocks), dim3(NT), shm, st, q);
         else hipLaunchKernelGGL((k_stencil_nonorth<S, kMaxR, RP, NT, HB, 64>), dim3(blocks), dim3(NT), shm, st, q);
     } else if (pitch == 128) {
